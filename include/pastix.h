@@ -1,0 +1,123 @@
+/*
+ * pastix.h -- public C API of the MI355X PaStiX factor+solve engine.
+ *
+ * The entry points keep the names, argument meaning, return codes and step
+ * ordering of the reference API (reference include/pastix.h:56-160,
+ * common/api.c, sopalin/pastix_task_sopalin.c:654, sopalin/pastix_task_solve.c:758,
+ * refinement/pastix_task_refine.c:236).  Numerical work (coeftab fill,
+ * factorization, solve, SpMV, refinement) runs on the GPU; the integer
+ * analysis (ordering, symbolic factorization, splitting, solver structure)
+ * runs on the host.
+ *
+ * Extensions without a reference counterpart are prefixed `pastix_amd_`:
+ * device-resident solve/refine on HBM buffers, structure export, and the
+ * multi-system batched pass.
+ */
+#ifndef PASTIX_AMD_PASTIX_H
+#define PASTIX_AMD_PASTIX_H
+
+#include "pastix/datatypes.h"
+#include "pastix/api.h"
+#include "pastix/order.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Legacy one-call interface (reference sopalin/pastix.c:103). */
+int pastix( pastix_data_t **pastix_data, PASTIX_Comm pastix_comm, pastix_int_t n,
+            pastix_int_t *colptr, pastix_int_t *rowptr, void *values,
+            pastix_int_t *perm, pastix_int_t *invp, void *B, pastix_int_t nrhs,
+            pastix_int_t *iparm, double *dparm );
+
+void pastixInitParam( pastix_int_t *iparm, double *dparm );
+void pastixInit( pastix_data_t **pastix_data, PASTIX_Comm pastix_comm,
+                 pastix_int_t *iparm, double *dparm );
+void pastixInitWithAffinity( pastix_data_t **pastix_data, PASTIX_Comm pastix_comm,
+                             pastix_int_t *iparm, double *dparm, const int *bindtab );
+void pastixFinalize( pastix_data_t **pastix_data );
+
+int pastix_task_analyze( pastix_data_t *pastix_data, const spmatrix_t *spm );
+int pastix_task_numfact( pastix_data_t *pastix_data, spmatrix_t *spm );
+int pastix_task_solve( pastix_data_t *pastix_data, pastix_int_t m, pastix_int_t nrhs,
+                       void *B, pastix_int_t ldb );
+int pastix_task_refine( pastix_data_t *pastix_data, pastix_int_t n, pastix_int_t nrhs,
+                        void *B, pastix_int_t ldb, void *X, pastix_int_t ldx );
+int pastix_task_solve_and_refine( pastix_data_t *pastix_data, pastix_int_t n, pastix_int_t nrhs,
+                                  void *B, pastix_int_t ldb, void *X, pastix_int_t ldx );
+
+int pastix_subtask_order( pastix_data_t *pastix_data, const spmatrix_t *spm,
+                          pastix_order_t *myorder );
+int pastix_subtask_symbfact( pastix_data_t *pastix_data );
+int pastix_subtask_reordering( pastix_data_t *pastix_data );
+int pastix_subtask_blend( pastix_data_t *pastix_data );
+
+int pastix_subtask_spm2bcsc( pastix_data_t *pastix_data, spmatrix_t *spm );
+int pastix_subtask_bcsc2ctab( pastix_data_t *pastix_data );
+int pastix_subtask_sopalin( pastix_data_t *pastix_data );
+
+int pastix_subtask_solve_adv( pastix_data_t *pastix_data, pastix_trans_t transA,
+                              pastix_rhs_t b );
+
+int pastixRhsInit( pastix_rhs_t *rhs );
+int pastixRhsFinalize( pastix_rhs_t rhs );
+
+int  pastixCheckParam( const pastix_int_t *iparm, const double *dparm );
+void pastixDumpParam( const pastix_data_t *pastix_data );
+
+/* ---- extensions (no reference counterpart) -------------------------------- */
+
+/*
+ * Solve op(A) X = B in place on a DEVICE buffer dB (n x nrhs, leading
+ * dimension ldb, FP64) that is already resident in HBM: permutation, forward
+ * and backward sweeps run on the GPU, nothing crosses PCIe.  transA is
+ * PastixNoTrans or PastixTrans (the adjoint solve of the differentiable
+ * wrapper).  Same algorithm as pastix_task_solve.
+ */
+int pastix_amd_solve_dev( pastix_data_t *pastix_data, pastix_trans_t transA,
+                          pastix_int_t nrhs, void *dB, pastix_int_t ldb );
+
+/*
+ * Refine X on the device: FP64 GMRES/CG/SR (IPARM_REFINEMENT) with the
+ * factorization as preconditioner, device SpMV on the permuted CSC and device
+ * norms.  dB and dX are device FP64 buffers in the user numbering.
+ * Returns the number of iterations (>= 0) or -PASTIX_ERR_*.
+ */
+int pastix_amd_refine_dev( pastix_data_t *pastix_data, pastix_int_t nrhs,
+                           const void *dB, pastix_int_t ldb, void *dX, pastix_int_t ldx );
+
+/* y = alpha*op(A)*x + beta*y with the user's matrix, x/y device FP64 vectors
+ * in the user numbering. */
+int pastix_amd_spmv_dev( pastix_data_t *pastix_data, pastix_trans_t trans, double alpha,
+                         const void *dx, double beta, void *dy );
+
+/* Synchronise the library stream and return the elapsed seconds of the last
+ * factorization measured with device events (0 if none). */
+double pastix_amd_last_fact_seconds( pastix_data_t *pastix_data );
+
+/*
+ * Structure export for tests and tooling.  Fills `dims`:
+ *  [0] n, [1] cblknbr, [2] bloknbr, [3] coefnbr (per factor), [4] browtab size,
+ *  [5] number of levels of the batched schedule, [6] factotype, [7] flttype
+ * and returns PASTIX_SUCCESS once the analysis has been done.
+ */
+int pastix_amd_get_dims( const pastix_data_t *pastix_data, pastix_int_t *dims );
+/*
+ * Copies the structure into caller arrays (any may be NULL):
+ *  cblk[5*cblknbr]: fcolnum, lcolnum, fblok index, stride, coefficient offset
+ *  blok[5*bloknbr]: frownum, lrownum, lcblknm, fcblknm, coefind
+ *  perm[n], invp[n]
+ */
+int pastix_amd_get_structure( const pastix_data_t *pastix_data, pastix_int_t *cblk,
+                              pastix_int_t *blok, pastix_int_t *perm, pastix_int_t *invp );
+/*
+ * Copies a factor (side 0 = L, 1 = U^T) from HBM into `host` (coefnbr
+ * entries, FP64 even when the factorization ran in FP32).
+ */
+int pastix_amd_get_coeftab( const pastix_data_t *pastix_data, int side, double *host );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASTIX_AMD_PASTIX_H */

@@ -1,0 +1,443 @@
+/*
+ * oracle.c -- CPU restatement of PaStiX's sequential full-rank numerical path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker of the GPU engine: only
+ * the tests, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it.  The product never calls it and has no CPU fallback.
+ *
+ * It restates, in plain C with its own dense loops (no BLAS library):
+ *   - coeftab fill from the permuted CSC     reference kernels/cpucblk_zinit.c:395-480
+ *   - sequential Cholesky driver             reference sopalin/sequential_zpotrf.c:51-92
+ *   - sequential LU driver                   reference sopalin/sequential_zgetrf.c:51
+ *   - diagonal POTRF, 64-blocked, static pivoting
+ *                                            reference kernels/core_zpotrfsp.c:68-190
+ *   - diagonal GETRF, 64-blocked, static pivoting, geadd/getmo
+ *                                            reference kernels/core_zgetrfsp.c:65-272
+ *   - panel TRSM on 2D tiles                 reference kernels/core_ztrsmsp.c:149-180
+ *   - 2D scatter GEMM                        reference kernels/core_zgemmsp.c:404-481
+ *   - supernodal forward/backward solve      reference kernels/solve_ztrsmsp.c:302-661
+ *                                            and sopalin/pastix_task_solve.c:543-649
+ * It is compiled twice: FP64 (oracle_d*) and, with ORACLE_SINGLE, FP32 (oracle_s*).
+ *
+ * The structure is passed as the arrays exported by pastix_amd_get_structure:
+ *   cblk[5*k]: fcolnum, lcolnum, first block, stride, coefficient offset
+ *   blok[5*b]: frownum, lrownum, lcblknm, fcblknm, coefind
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#if defined( ORACLE_SINGLE )
+#define REAL float
+#define PFX( name ) oracle_s##name
+#else
+#define REAL double
+#define PFX( name ) oracle_d##name
+#endif
+
+typedef int64_t oint;
+
+#define C_FCOL( k ) cblk[5 * ( k ) + 0]
+#define C_LCOL( k ) cblk[5 * ( k ) + 1]
+#define C_FBLK( k ) cblk[5 * ( k ) + 2]
+#define C_STRD( k ) cblk[5 * ( k ) + 3]
+#define C_COEF( k ) cblk[5 * ( k ) + 4]
+#define B_FROW( b ) blok[5 * ( b ) + 0]
+#define B_LROW( b ) blok[5 * ( b ) + 1]
+#define B_LCBK( b ) blok[5 * ( b ) + 2]
+#define B_FCBK( b ) blok[5 * ( b ) + 3]
+#define B_COEF( b ) blok[5 * ( b ) + 4]
+#define B_NROW( b ) ( B_LROW( b ) - B_FROW( b ) + 1 )
+#define C_NCOL( k ) ( C_LCOL( k ) - C_FCOL( k ) + 1 )
+
+#define MAXSIZEOFBLOCKS 64
+
+/* ---- dense helpers (column major) --------------------------------------- */
+
+/* C(m x n) -= A(m x k) * op(B),  op(B) = B^T with B (n x k) */
+static void
+gemm_nt_minus( oint m, oint n, oint k, const REAL *A, oint lda, const REAL *B, oint ldb, REAL *C, oint ldc )
+{
+    oint i, j, l;
+    for ( j = 0; j < n; j++ ) {
+        REAL *c = C + j * ldc;
+        for ( l = 0; l < k; l++ ) {
+            REAL bjl = B[j + l * ldb];
+            const REAL *a = A + l * lda;
+            if ( bjl == (REAL)0 ) continue;
+            for ( i = 0; i < m; i++ ) c[i] -= a[i] * bjl;
+        }
+    }
+}
+
+/* C(m x n) -= A(m x k) * B(k x n) */
+static void
+gemm_nn_minus( oint m, oint n, oint k, const REAL *A, oint lda, const REAL *B, oint ldb, REAL *C, oint ldc )
+{
+    oint i, j, l;
+    for ( j = 0; j < n; j++ ) {
+        REAL *c = C + j * ldc;
+        for ( l = 0; l < k; l++ ) {
+            REAL blj = B[l + j * ldb];
+            const REAL *a = A + l * lda;
+            if ( blj == (REAL)0 ) continue;
+            for ( i = 0; i < m; i++ ) c[i] -= a[i] * blj;
+        }
+    }
+}
+
+/* X(m x n) <- X * L^{-T}, L lower non-unit (Right, Lower, Trans, NonUnit) */
+static void
+trsm_rlt( oint m, oint n, const REAL *L, oint lda, REAL *X, oint ldx )
+{
+    oint i, j, l;
+    for ( j = 0; j < n; j++ ) {
+        REAL *xj = X + j * ldx;
+        for ( l = 0; l < j; l++ ) {
+            REAL c = L[j + l * lda];
+            const REAL *xl = X + l * ldx;
+            for ( i = 0; i < m; i++ ) xj[i] -= xl[i] * c;
+        }
+        {
+            REAL d = L[j + j * lda];
+            for ( i = 0; i < m; i++ ) xj[i] /= d;
+        }
+    }
+}
+
+/* X(m x n) <- X * U^{-1}, U upper (Right, Upper, NoTrans, unit or not) */
+static void
+trsm_run( oint m, oint n, const REAL *U, oint lda, int unit, REAL *X, oint ldx )
+{
+    oint i, j, l;
+    for ( j = 0; j < n; j++ ) {
+        REAL *xj = X + j * ldx;
+        for ( l = 0; l < j; l++ ) {
+            REAL c = U[l + j * lda];
+            const REAL *xl = X + l * ldx;
+            for ( i = 0; i < m; i++ ) xj[i] -= xl[i] * c;
+        }
+        if ( !unit ) {
+            REAL d = U[j + j * lda];
+            for ( i = 0; i < m; i++ ) xj[i] /= d;
+        }
+    }
+}
+
+/* X(m x n) <- L^{-1} X, L lower unit (Left, Lower, NoTrans, Unit) */
+static void
+trsm_llnu( oint m, oint n, const REAL *L, oint lda, REAL *X, oint ldx )
+{
+    oint i, j, l;
+    for ( j = 0; j < n; j++ ) {
+        REAL *x = X + j * ldx;
+        for ( l = 0; l < m; l++ ) {
+            REAL v = x[l];
+            for ( i = l + 1; i < m; i++ ) x[i] -= L[i + l * lda] * v;
+        }
+    }
+}
+
+/* ---- diagonal factorizations ------------------------------------------- */
+
+/* unblocked Cholesky of an n x n block with static pivoting
+ * (reference kernels/core_zpotrfsp.c:68-107) */
+static void
+potf2sp( oint n, REAL *A, oint lda, oint *npiv, oint *nneg, double crit )
+{
+    oint k, i, j;
+    for ( k = 0; k < n; k++ ) {
+        REAL *akk = A + k * ( lda + 1 );
+        REAL  alpha;
+        if ( fabs( (double)*akk ) < crit ) {
+            *akk = (REAL)crit;
+            ( *npiv )++;
+        }
+        if ( *akk < (REAL)0 ) ( *nneg )++;
+        *akk  = (REAL)sqrt( (double)*akk );
+        alpha = (REAL)1 / *akk;
+        for ( i = k + 1; i < n; i++ ) akk[i - k] *= alpha;
+        /* syr: A(k+1:n, k+1:n) -= x x^T (lower) */
+        for ( j = k + 1; j < n; j++ ) {
+            REAL xj = A[j + k * lda];
+            for ( i = j; i < n; i++ ) A[i + j * lda] -= A[i + k * lda] * xj;
+        }
+    }
+}
+
+/* reference kernels/core_zpotrfsp.c:144-190 */
+static void
+potrfsp( oint n, REAL *A, oint lda, oint *npiv, oint *nneg, double crit )
+{
+    oint k, blocknbr = ( n + MAXSIZEOFBLOCKS - 1 ) / MAXSIZEOFBLOCKS;
+    for ( k = 0; k < blocknbr; k++ ) {
+        oint bs = n - k * MAXSIZEOFBLOCKS < MAXSIZEOFBLOCKS ? n - k * MAXSIZEOFBLOCKS : MAXSIZEOFBLOCKS;
+        REAL *tmp = A + ( k * MAXSIZEOFBLOCKS ) * ( lda + 1 );
+        potf2sp( bs, tmp, lda, npiv, nneg, crit );
+        if ( k * MAXSIZEOFBLOCKS + bs < n ) {
+            REAL *tmp1 = tmp + bs, *tmp2 = tmp1 + bs * lda;
+            oint  ms = n - ( k * MAXSIZEOFBLOCKS + bs ), i, j, l;
+            trsm_rlt( ms, bs, tmp, lda, tmp1, lda );
+            /* syrk lower: tmp2 -= tmp1 tmp1^T */
+            for ( j = 0; j < ms; j++ ) {
+                for ( l = 0; l < bs; l++ ) {
+                    REAL b = tmp1[j + l * lda];
+                    for ( i = j; i < ms; i++ ) tmp2[i + j * lda] -= tmp1[i + l * lda] * b;
+                }
+            }
+        }
+    }
+}
+
+/* unblocked LU of an m x n panel with static pivoting
+ * (reference kernels/core_zgetrfsp.c:65-105) */
+static void
+getf2sp( oint m, oint n, REAL *A, oint lda, oint *npiv, double crit )
+{
+    oint k, i, j, minmn = m < n ? m : n;
+    for ( k = 0; k < minmn; k++ ) {
+        REAL *akk = A + k * ( lda + 1 ), alpha;
+        if ( fabs( (double)*akk ) < crit ) {
+            *akk = ( *akk < (REAL)0 ) ? (REAL)( -crit ) : (REAL)crit;
+            ( *npiv )++;
+        }
+        alpha = (REAL)1 / *akk;
+        for ( i = k + 1; i < m; i++ ) A[i + k * lda] *= alpha;
+        for ( j = k + 1; j < n; j++ ) {
+            REAL ukj = A[k + j * lda];
+            for ( i = k + 1; i < m; i++ ) A[i + j * lda] -= A[i + k * lda] * ukj;
+        }
+    }
+}
+
+/* reference kernels/core_zgetrfsp.c:138-185 */
+static void
+getrfsp( oint n, REAL *A, oint lda, oint *npiv, double crit )
+{
+    oint k, blocknbr = ( n + MAXSIZEOFBLOCKS - 1 ) / MAXSIZEOFBLOCKS;
+    REAL *Akk = A;
+    for ( k = 0; k < blocknbr; k++ ) {
+        oint tempm = n - k * MAXSIZEOFBLOCKS;
+        oint bs    = tempm < MAXSIZEOFBLOCKS ? tempm : MAXSIZEOFBLOCKS;
+        REAL *Lik = Akk + bs, *Ukj = Akk + bs * lda, *Aij = Ukj + bs;
+        oint ms;
+        getf2sp( tempm, bs, Akk, lda, npiv, crit );
+        ms = tempm - bs;
+        if ( ms > 0 ) {
+            trsm_llnu( bs, ms, Akk, lda, Ukj, lda );
+            gemm_nn_minus( ms, ms, bs, Lik, lda, Ukj, lda, Aij, lda );
+        }
+        Akk += bs * ( lda + 1 );
+    }
+}
+
+/* ---- supernodal kernels ------------------------------------------------- */
+
+/* 2D scatter GEMM of block `bj` of cblk k into its facing cblk
+ * (reference kernels/core_zgemmsp.c:404-481); shift 1 on the U side */
+static void
+gemmsp_2d2d( const oint *cblk, const oint *blok, oint k, oint bj, int shift, const REAL *A, const REAL *B,
+             REAL *C )
+{
+    oint f = B_FCBK( bj ), fb = C_FBLK( f ), lb = C_FBLK( k + 1 ), i;
+    oint K = C_NCOL( k ), N = B_NROW( bj );
+    const REAL *Bj = B + C_COEF( k ) + B_COEF( bj );
+    REAL *Cf = C + C_COEF( f );
+    for ( i = bj + shift; i < lb; i++ ) {
+        oint M = B_NROW( i ), ldc;
+        while ( !( B_FROW( i ) >= B_FROW( fb ) && B_LROW( i ) <= B_LROW( fb ) ) ) fb++;
+        ldc = B_NROW( fb );
+        gemm_nt_minus( M, N, K, A + C_COEF( k ) + B_COEF( i ), M, Bj, N,
+                       Cf + B_COEF( fb ) + ( B_FROW( i ) - B_FROW( fb ) ) + ( B_FROW( bj ) - C_FCOL( f ) ) * ldc,
+                       ldc );
+    }
+}
+
+/* coeftab fill (reference kernels/cpucblk_zinit.c:395-480): lower part of the
+ * permuted CSC into L, strictly lower part of the transpose into U */
+int
+PFX( fill )( oint cblknbr, const oint *cblk, const oint *blok, const oint *colptr, const oint *rowtab,
+             const double *Lv, const double *Uv, REAL *L, REAL *U )
+{
+    oint k, c, e;
+    for ( k = 0; k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k ), lb = C_FBLK( k + 1 );
+        for ( c = C_FCOL( k ); c <= C_LCOL( k ); c++ ) {
+            oint b = C_FBLK( k );
+            for ( e = colptr[c]; e < colptr[c + 1]; e++ ) {
+                oint r = rowtab[e], idx;
+                if ( r < c ) continue;
+                while ( b < lb && ( B_LROW( b ) < r || B_FROW( b ) > r ) ) b++;
+                if ( b >= lb ) continue;
+                idx = C_COEF( k ) + B_COEF( b ) + ( r - B_FROW( b ) ) + ( c - C_FCOL( k ) ) * B_NROW( b );
+                L[idx] = (REAL)Lv[e];
+                if ( U && r > c ) U[idx] = (REAL)Uv[e];
+            }
+        }
+        (void)nc;
+    }
+    return 0;
+}
+
+/* sequential LL^t (reference sopalin/sequential_zpotrf.c:51-92 with
+ * cpucblk_zpotrfsp1d, kernels/core_zpotrfsp.c:220-367) */
+int
+PFX( potrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, double crit, oint *npiv, oint *nneg )
+{
+    oint k, b;
+    *npiv = 0;
+    *nneg = 0;
+    for ( k = 0; k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k );
+        REAL *D = L + C_COEF( k );
+        potrfsp( nc, D, nc, npiv, nneg, crit );
+        for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) {
+            trsm_rlt( B_NROW( b ), nc, D, nc, L + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+        }
+        for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) {
+            gemmsp_2d2d( cblk, blok, k, b, 0, L, L, L );
+        }
+    }
+    return 0;
+}
+
+/* sequential LU with static pivoting (reference sopalin/sequential_zgetrf.c,
+ * cpucblk_zgetrfsp1d, kernels/core_zgetrfsp.c:217-395) */
+int
+PFX( getrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *U, double crit, oint *npiv )
+{
+    oint k, b, i, j;
+    *npiv = 0;
+    for ( k = 0; k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k ), lb = C_FBLK( k + 1 );
+        REAL *D = L + C_COEF( k ), *Du = U + C_COEF( k );
+        /* geadd(Trans): D += Du^T */
+        for ( j = 0; j < nc; j++ )
+            for ( i = 0; i < nc; i++ ) D[i + j * nc] += Du[j + i * nc];
+        getrfsp( nc, D, nc, npiv, crit );
+        /* getmo: Du = D^T */
+        for ( j = 0; j < nc; j++ )
+            for ( i = 0; i < nc; i++ ) Du[j + i * nc] = D[i + j * nc];
+        for ( b = C_FBLK( k ) + 1; b < lb; b++ ) {
+            trsm_run( B_NROW( b ), nc, D, nc, 0, L + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+            trsm_run( B_NROW( b ), nc, Du, nc, 1, U + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+        }
+        for ( b = C_FBLK( k ) + 1; b < lb; b++ ) {
+            gemmsp_2d2d( cblk, blok, k, b, 0, L, U, L );
+            if ( b + 1 < lb ) gemmsp_2d2d( cblk, blok, k, b, 1, U, L, U );
+        }
+    }
+    return 0;
+}
+
+/* Supernodal solve in the factorization numbering, x is n x nrhs (ld).
+ * fwdC/bwdC select the factor tiles of each sweep and unit_f/unit_b the
+ * diagonal (reference sopalin/pastix_task_solve.c:543-649,
+ * kernels/solve_ztrsmsp.c:302-661). */
+static void
+solve_sweeps( oint cblknbr, const oint *cblk, const oint *blok, const REAL *fwdC, int unit_f, const REAL *bwdC,
+              int unit_b, oint nrhs, REAL *x, oint ld )
+{
+    oint k, b, r, i, j;
+    for ( k = 0; k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k ), f0 = C_FCOL( k );
+        const REAL *D = fwdC + C_COEF( k );
+        for ( r = 0; r < nrhs; r++ ) {
+            REAL *xk = x + r * ld + f0;
+            for ( j = 0; j < nc; j++ ) {
+                if ( !unit_f ) xk[j] /= D[j + j * nc];
+                for ( i = j + 1; i < nc; i++ ) xk[i] -= D[i + j * nc] * xk[j];
+            }
+            for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) {
+                const REAL *A = fwdC + C_COEF( k ) + B_COEF( b );
+                oint M = B_NROW( b );
+                REAL *y = x + r * ld + B_FROW( b );
+                for ( j = 0; j < nc; j++ )
+                    for ( i = 0; i < M; i++ ) y[i] -= A[i + j * M] * xk[j];
+            }
+        }
+    }
+    for ( k = cblknbr - 1; k >= 0; k-- ) {
+        oint nc = C_NCOL( k ), f0 = C_FCOL( k );
+        const REAL *D = bwdC + C_COEF( k );
+        for ( r = 0; r < nrhs; r++ ) {
+            REAL *xk = x + r * ld + f0;
+            for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) {
+                const REAL *A = bwdC + C_COEF( k ) + B_COEF( b );
+                oint M = B_NROW( b );
+                const REAL *y = x + r * ld + B_FROW( b );
+                for ( j = 0; j < nc; j++ ) {
+                    REAL s = 0;
+                    for ( i = 0; i < M; i++ ) s += A[i + j * M] * y[i];
+                    xk[j] -= s;
+                }
+            }
+            for ( j = nc - 1; j >= 0; j-- ) {
+                REAL s = xk[j];
+                for ( i = j + 1; i < nc; i++ ) s -= D[i + j * nc] * xk[i];
+                xk[j] = unit_b ? s : s / D[j + j * nc];
+            }
+        }
+    }
+}
+
+/* factotype: 0 = LL^t, 2 = LU; trans: 0 = A x = b, 1 = A^T x = b */
+int
+PFX( solve )( oint cblknbr, const oint *cblk, const oint *blok, int factotype, int trans, const REAL *L,
+              const REAL *U, oint nrhs, REAL *x, oint ld )
+{
+    if ( factotype == 2 ) {
+        if ( !trans ) solve_sweeps( cblknbr, cblk, blok, L, 1, U, 0, nrhs, x, ld );
+        else          solve_sweeps( cblknbr, cblk, blok, U, 0, L, 1, nrhs, x, ld );
+    }
+    else {
+        solve_sweeps( cblknbr, cblk, blok, L, 0, L, 0, nrhs, x, ld );
+    }
+    return 0;
+}
+
+/* Single update of the reference-signature GEMM (for per-call parity of
+ * gpucblk_*gemmsp): cpucblk_zgemmsp on one (cblk, blok) pair. */
+int
+PFX( gemmsp_one )( const oint *cblk, const oint *blok, oint k, oint bj, int shift, const REAL *A, const REAL *B,
+                   REAL *C )
+{
+    gemmsp_2d2d( cblk, blok, k, bj, shift, A, B, C );
+    return 0;
+}
+
+/* Single diagonal factorization (per-call parity of gpucblk_*potrfsp / getrfsp). */
+int
+PFX( potrf_diag )( oint nc, REAL *D, double crit, oint *npiv, oint *nneg )
+{
+    *npiv = 0;
+    *nneg = 0;
+    potrfsp( nc, D, nc, npiv, nneg, crit );
+    return 0;
+}
+int
+PFX( getrf_diag )( oint nc, REAL *D, REAL *Du, double crit, oint *npiv )
+{
+    oint i, j;
+    *npiv = 0;
+    for ( j = 0; j < nc; j++ )
+        for ( i = 0; i < nc; i++ ) D[i + j * nc] += Du[j + i * nc];
+    getrfsp( nc, D, nc, npiv, crit );
+    for ( j = 0; j < nc; j++ )
+        for ( i = 0; i < nc; i++ ) Du[j + i * nc] = D[i + j * nc];
+    return 0;
+}
+/* Panel TRSM of one cblk (per-call parity of gpublok_*trsmsp):
+ * mode 0: X L^{-T} (POTRF), 1: X U^{-1} (LU, L side), 2: X V^{-1} unit (LU, U side) */
+int
+PFX( trsm_cblk )( const oint *cblk, const oint *blok, oint k, int mode, const REAL *D, REAL *panel )
+{
+    oint b, nc = C_NCOL( k );
+    for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) {
+        REAL *X = panel + B_COEF( b );
+        if ( mode == 0 ) trsm_rlt( B_NROW( b ), nc, D, nc, X, B_NROW( b ) );
+        else             trsm_run( B_NROW( b ), nc, D, nc, mode == 2, X, B_NROW( b ) );
+    }
+    return 0;
+}

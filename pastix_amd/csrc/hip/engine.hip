@@ -1,0 +1,718 @@
+/*
+ * engine.hip -- device state, structure upload, level schedule and the
+ * drivers of the factorization, the solve and the FP64 vector operations.
+ *
+ * The factorization replaces the reference's sequential/static/dynamic
+ * schedulers (reference sopalin/sequential_zpotrf.c:51-325) and its
+ * StarPU/PaRSEC runtimes by a level schedule: cblks are grouped by their
+ * height in the supernodal elimination tree; a level runs as three batched
+ * launches (diagonal factorization, panel TRSM, scatter-GEMM).  Every cblk of
+ * a level only updates cblks of higher levels, so the launches of a level are
+ * dependency-free; updates of cblks of one level that meet in the same target
+ * cblk are accumulated with hardware FP atomics.
+ */
+#include <hip/hip_runtime.h>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include "kernels.hpp"
+
+extern "C" {
+#include "../host/internal.h"
+#include "pa_device.h"
+}
+
+#define PA_HIP( call )                                                                       \
+    do {                                                                                     \
+        hipError_t _e = ( call );                                                            \
+        if ( _e != hipSuccess ) {                                                            \
+            fprintf( stderr, "pastix_amd: HIP error %s at %s:%d: %s\n", hipGetErrorName( _e ), \
+                     __FILE__, __LINE__, #call );                                             \
+            return PASTIX_ERR_INTERNAL;                                                      \
+        }                                                                                    \
+    } while ( 0 )
+
+struct pa_level_t {
+    int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
+    int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks                      */
+    int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks                      */
+    int64_t solve_beg, solve_end; /* into d_solve_tasks                     */
+    int32_t ncmax;                /* widest cblk of the level               */
+};
+
+struct pa_device_s {
+    int         device;
+    hipStream_t stream;
+    int         lds_limit;
+    /* structure */
+    int32_t *d_c_fcol, *d_c_ncol, *d_c_stride, *d_c_fblok;
+    int64_t *d_c_coef;
+    int32_t *d_b_frow, *d_b_lrow, *d_b_fcblk, *d_b_rowoff;
+    int32_t *d_col2cblk, *d_perm, *d_invp;
+    pa_dstruct S;
+    int64_t  n, coefnbr;
+    int32_t  colmax;
+    /* schedule */
+    std::vector<pa_level_t> levels;
+    int32_t      *d_lev_cblks;
+    int2         *d_trsm_tasks;
+    pa_gemm_task *d_gemm_tasks;
+    int2         *d_solve_tasks;
+    /* matrix */
+    int64_t *d_colptr;
+    int32_t *d_rowtab;
+    double  *d_Lv, *d_Uv;
+    /* factors */
+    int   prec, lu;
+    void *d_L, *d_U;
+    int32_t *d_counters;
+    /* work */
+    double *d_partial;
+    double *h_partial;
+    float  *d_xs;      /* FP32 solve buffer of the mixed path */
+    int64_t xs_len;
+    double *d_xtmp;    /* permuted rhs buffer */
+    int64_t xtmp_len;
+    hipEvent_t ev0, ev1;
+    double  last_fact_event_s;
+};
+
+static inline pa_device_s *DEV( const pastix_data_t *d ) { return (pa_device_s *)d->dev; }
+
+template <typename T>
+static int
+pa_upload( T **dptr, const std::vector<T> &h )
+{
+    size_t bytes = std::max<size_t>( h.size(), 1 ) * sizeof( T );
+    PA_HIP( hipMalloc( (void **)dptr, bytes ) );
+    if ( !h.empty() ) PA_HIP( hipMemcpy( *dptr, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice ) );
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_open( pastix_data_t *pd )
+{
+    int ndev = 0;
+    if ( pd->dev != NULL ) return PASTIX_SUCCESS;
+    if ( hipGetDeviceCount( &ndev ) != hipSuccess || ndev < 1 ) {
+        fprintf( stderr, "pastix_amd: no HIP device available; this engine has no CPU fallback\n" );
+        return PASTIX_ERR_INTERNAL;
+    }
+    pa_device_s *dev = new pa_device_s();
+    memset( (void *)&dev->S, 0, sizeof( dev->S ) );
+    dev->device = 0;
+    const char *env = getenv( "PASTIX_AMD_DEVICE" );
+    if ( env ) dev->device = atoi( env );
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipStreamCreate( &dev->stream ) );
+    PA_HIP( hipEventCreate( &dev->ev0 ) );
+    PA_HIP( hipEventCreate( &dev->ev1 ) );
+    int lds = 65536;
+    hipDeviceGetAttribute( &lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev->device );
+    dev->lds_limit = lds;
+    PA_HIP( hipMalloc( (void **)&dev->d_counters, 16 * sizeof( int32_t ) ) );
+    PA_HIP( hipMalloc( (void **)&dev->d_partial, 4096 * sizeof( double ) ) );
+    PA_HIP( hipHostMalloc( (void **)&dev->h_partial, 4096 * sizeof( double ) ) );
+    pd->dev = dev;
+    return PASTIX_SUCCESS;
+}
+
+static void
+pa_free_matrix( pa_device_s *dev )
+{
+    hipFree( dev->d_colptr ); hipFree( dev->d_rowtab ); hipFree( dev->d_Lv ); hipFree( dev->d_Uv );
+    dev->d_colptr = NULL; dev->d_rowtab = NULL; dev->d_Lv = NULL; dev->d_Uv = NULL;
+}
+
+static void
+pa_free_factors( pa_device_s *dev )
+{
+    hipFree( dev->d_L ); hipFree( dev->d_U );
+    dev->d_L = dev->d_U = NULL;
+}
+
+static void
+pa_free_structure( pa_device_s *dev )
+{
+    void *ptrs[] = { dev->d_c_fcol, dev->d_c_ncol, dev->d_c_stride, dev->d_c_fblok, dev->d_c_coef,
+                     dev->d_b_frow, dev->d_b_lrow, dev->d_b_fcblk, dev->d_b_rowoff, dev->d_col2cblk,
+                     dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks,
+                     dev->d_solve_tasks };
+    for ( void *p : ptrs ) hipFree( p );
+    dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
+    dev->d_c_coef = NULL;
+    dev->d_b_frow = dev->d_b_lrow = dev->d_b_fcblk = dev->d_b_rowoff = NULL;
+    dev->d_col2cblk = dev->d_perm = dev->d_invp = NULL;
+    dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
+    dev->levels.clear();
+}
+
+extern "C" void
+pa_dev_close( pastix_data_t *pd )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( !dev ) return;
+    hipSetDevice( dev->device );
+    hipStreamSynchronize( dev->stream );
+    pa_free_matrix( dev );
+    pa_free_factors( dev );
+    pa_free_structure( dev );
+    hipFree( dev->d_counters );
+    hipFree( dev->d_partial );
+    hipHostFree( dev->h_partial );
+    hipFree( dev->d_xs );
+    hipFree( dev->d_xtmp );
+    hipEventDestroy( dev->ev0 );
+    hipEventDestroy( dev->ev1 );
+    hipStreamDestroy( dev->stream );
+    delete dev;
+    pd->dev = NULL;
+}
+
+/* ------------------------------------------------------------------------- */
+/* structure + schedule                                                      */
+/* ------------------------------------------------------------------------- */
+extern "C" int
+pa_dev_setup_structure( pastix_data_t *pd )
+{
+    pa_device_s        *dev  = DEV( pd );
+    const SolverMatrix *solv = pd->solvmatr;
+    const pastix_int_t  cblknbr = solv->cblknbr, bloknbr = solv->bloknbr, n = solv->nodenbr;
+    int rc;
+
+    if ( !dev ) return PASTIX_ERR_BADPARAMETER;
+    if ( n >= ( 1LL << 31 ) || bloknbr >= ( 1LL << 31 ) ) {
+        fprintf( stderr, "pastix_amd: problem too large for 32-bit device row indices\n" );
+        return PASTIX_ERR_INTEGER_TYPE;
+    }
+    PA_HIP( hipSetDevice( dev->device ) );
+    pa_free_structure( dev );
+
+    std::vector<int32_t> c_fcol( cblknbr + 1 ), c_ncol( cblknbr + 1 ), c_stride( cblknbr + 1 ), c_fblok( cblknbr + 1 );
+    std::vector<int64_t> c_coef( cblknbr + 1 );
+    std::vector<int32_t> b_frow( bloknbr + 1 ), b_lrow( bloknbr + 1 ), b_fcblk( bloknbr + 1 ), b_rowoff( bloknbr + 1 );
+    std::vector<int32_t> col2cblk( n + 1 ), perm( n + 1 ), invp( n + 1 );
+    int64_t coef = 0;
+    int32_t colmax = 0;
+    for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
+        const SolverCblk *c = solv->cblktab + k;
+        c_fcol[k]   = (int32_t)c->fcolnum;
+        c_ncol[k]   = (int32_t)cblk_colnbr( c );
+        c_stride[k] = (int32_t)c->stride;
+        c_fblok[k]  = (int32_t)( c->fblokptr - solv->bloktab );
+        c_coef[k]   = coef;
+        coef += (int64_t)c->stride * c_ncol[k];
+        colmax = std::max( colmax, c_ncol[k] );
+        for ( pastix_int_t j = c->fcolnum; j <= c->lcolnum; j++ ) col2cblk[j] = (int32_t)k;
+        int32_t rowoff = 0;
+        for ( const SolverBlok *b = c->fblokptr; b < c[1].fblokptr; b++ ) {
+            pastix_int_t bi = b - solv->bloktab;
+            b_frow[bi]   = (int32_t)b->frownum;
+            b_lrow[bi]   = (int32_t)b->lrownum;
+            b_fcblk[bi]  = (int32_t)b->fcblknm;
+            b_rowoff[bi] = rowoff;
+            rowoff += (int32_t)blok_rownbr( b );
+        }
+    }
+    c_fcol[cblknbr]  = (int32_t)n;
+    c_fblok[cblknbr] = (int32_t)bloknbr;
+    c_coef[cblknbr]  = coef;
+    for ( pastix_int_t i = 0; i < n; i++ ) {
+        perm[i] = (int32_t)pd->ordemesh->permtab[i];
+        invp[i] = (int32_t)pd->ordemesh->peritab[i];
+    }
+    dev->n       = n;
+    dev->coefnbr = coef;
+    dev->colmax  = colmax;
+
+    if ( ( rc = pa_upload( &dev->d_c_fcol, c_fcol ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_ncol, c_ncol ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_stride, c_stride ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_fblok, c_fblok ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_coef, c_coef ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_b_frow, b_frow ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_b_lrow, b_lrow ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_b_fcblk, b_fcblk ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_b_rowoff, b_rowoff ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_col2cblk, col2cblk ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_perm, perm ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_invp, invp ) ) ) return rc;
+
+    pa_dstruct &S = dev->S;
+    S.cblknbr  = (int32_t)cblknbr;
+    S.bloknbr  = (int32_t)bloknbr;
+    S.c_fcol   = dev->d_c_fcol;
+    S.c_ncol   = dev->d_c_ncol;
+    S.c_stride = dev->d_c_stride;
+    S.c_fblok  = dev->d_c_fblok;
+    S.c_coef   = dev->d_c_coef;
+    S.b_frow   = dev->d_b_frow;
+    S.b_lrow   = dev->d_b_lrow;
+    S.b_fcblk  = dev->d_b_fcblk;
+    S.b_rowoff = dev->d_b_rowoff;
+
+    /* ---- level schedule ---- */
+    std::vector<pastix_int_t> level( cblknbr + 1 );
+    pastix_int_t nlev = cblknbr ? pa_solver_levels( solv, level.data() ) : 0;
+    std::vector<std::vector<int32_t>> bylev( nlev );
+    for ( pastix_int_t k = 0; k < cblknbr; k++ ) bylev[level[k]].push_back( (int32_t)k );
+
+    const int lu = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    std::vector<int32_t>      lev_cblks;
+    std::vector<int2>         trsm_tasks, solve_tasks;
+    std::vector<pa_gemm_task> gemm_tasks;
+    std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
+    dev->levels.resize( nlev );
+
+    for ( pastix_int_t l = 0; l < nlev; l++ ) {
+        pa_level_t &L = dev->levels[l];
+        L.cblk_beg  = lev_cblks.size();
+        L.trsm_beg  = trsm_tasks.size();
+        L.gemm_beg  = gemm_tasks.size();
+        L.solve_beg = solve_tasks.size();
+        L.ncmax     = 0;
+        /* number of distinct source cblks of this level per target cblk */
+        std::vector<int32_t> touched;
+        for ( int32_t k : bylev[l] ) {
+            lev_cblks.push_back( k );
+            L.ncmax = std::max( L.ncmax, c_ncol[k] );
+            for ( int32_t b = c_fblok[k] + 1; b < c_fblok[k + 1]; b++ ) {
+                int32_t F = b_fcblk[b];
+                if ( lastsrc[F] != k ) {
+                    lastsrc[F] = k;
+                    if ( srccount[F]++ == 0 ) touched.push_back( F );
+                }
+            }
+        }
+        for ( int32_t k : bylev[l] ) {
+            int32_t nc = c_ncol[k], R = c_stride[k] - nc;
+            for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
+            for ( int32_t r = 0; r < R; r += PA_SOLVE_ROWS ) solve_tasks.push_back( make_int2( k, r ) );
+            /* column groups: consecutive blocks facing the same cblk */
+            int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
+            while ( b < lb ) {
+                int32_t F = b_fcblk[b], e = b;
+                while ( e + 1 < lb && b_fcblk[e + 1] == F ) e++;
+                int32_t g0 = b_rowoff[b] - nc;
+                int32_t g1 = b_rowoff[e] - nc + ( b_lrow[e] - b_frow[e] + 1 );
+                int32_t flags = ( srccount[F] > 1 ) ? PA_TASK_ATOMIC : 0;
+                for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
+                    for ( int32_t s = g0; s < g1; s += PA_GT ) {
+                        int32_t slen = std::min( PA_GT, g1 - s );
+                        /* U side needs rows of strictly later blocks only */
+                        int32_t rstart = g0;
+                        if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
+                        for ( int32_t r = rstart; r < R; r += PA_GT ) {
+                            pa_gemm_task t;
+                            t.k = k; t.F = F; t.s0 = s; t.r0 = r; t.slen = slen;
+                            t.flags = flags | ( side ? PA_TASK_USIDE : 0 );
+                            t.pad0 = t.pad1 = 0;
+                            gemm_tasks.push_back( t );
+                        }
+                    }
+                }
+                b = e + 1;
+            }
+        }
+        for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
+        L.cblk_end  = lev_cblks.size();
+        L.trsm_end  = trsm_tasks.size();
+        L.gemm_end  = gemm_tasks.size();
+        L.solve_end = solve_tasks.size();
+    }
+    dev->lu = lu;
+    if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_gemm_tasks, gemm_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_solve_tasks, solve_tasks ) ) ) return rc;
+
+    /* LDS needs of the diagonal, TRSM and solve kernels */
+    size_t need = (size_t)colmax * std::max( PA_DIAG_NB, PA_TRSM_TR ) * sizeof( double ) + 1024;
+    if ( need > (size_t)dev->lds_limit ) {
+        fprintf( stderr, "pastix_amd: cblks of %d columns exceed the LDS budget (%d bytes); lower IPARM_MAX_BLOCKSIZE\n",
+                 colmax, dev->lds_limit );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" pastix_int_t
+pa_dev_nlevels( const pastix_data_t *pd )
+{
+    return DEV( pd ) ? (pastix_int_t)DEV( pd )->levels.size() : 0;
+}
+
+extern "C" int
+pa_dev_upload_bcsc( pastix_data_t *pd )
+{
+    pa_device_s     *dev  = DEV( pd );
+    const pa_bcsc_t *bcsc = pd->bcsc;
+    int rc;
+    PA_HIP( hipSetDevice( dev->device ) );
+    pa_free_matrix( dev );
+    std::vector<int64_t> colptr( bcsc->colptr, bcsc->colptr + bcsc->n + 1 );
+    std::vector<int32_t> rowtab( bcsc->nnz );
+    for ( pastix_int_t e = 0; e < bcsc->nnz; e++ ) rowtab[e] = (int32_t)bcsc->rowtab[e];
+    if ( ( rc = pa_upload( &dev->d_colptr, colptr ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_rowtab, rowtab ) ) ) return rc;
+    std::vector<double> Lv( bcsc->Lvalues, bcsc->Lvalues + bcsc->nnz );
+    std::vector<double> Uv( bcsc->Uvalues, bcsc->Uvalues + bcsc->nnz );
+    if ( ( rc = pa_upload( &dev->d_Lv, Lv ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_Uv, Uv ) ) ) return rc;
+    return PASTIX_SUCCESS;
+}
+
+template <typename T>
+static int
+pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
+{
+    size_t bytes = (size_t)std::max<int64_t>( dev->coefnbr, 1 ) * sizeof( T );
+    PA_HIP( hipMalloc( &dev->d_L, bytes ) );
+    PA_HIP( hipMemsetAsync( dev->d_L, 0, bytes, dev->stream ) );
+    if ( lu ) {
+        PA_HIP( hipMalloc( &dev->d_U, bytes ) );
+        PA_HIP( hipMemsetAsync( dev->d_U, 0, bytes, dev->stream ) );
+    }
+    PA_HIP( hipMemsetAsync( dev->d_counters, 0, 16 * sizeof( int32_t ), dev->stream ) );
+    int32_t n = (int32_t)dev->n;
+    if ( n > 0 ) {
+        k_fill<T><<<( n + 255 ) / 256, 256, 0, dev->stream>>>(
+            dev->S, n, dev->d_colptr, dev->d_rowtab, dev->d_Lv, dev->d_Uv, (T *)dev->d_L,
+            lu ? (T *)dev->d_U : nullptr, dev->d_col2cblk, dev->d_counters + 2 );
+        PA_HIP( hipGetLastError() );
+    }
+    int32_t h[16];
+    PA_HIP( hipMemcpyAsync( h, dev->d_counters, sizeof( h ), hipMemcpyDeviceToHost, dev->stream ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    if ( overflow ) *overflow = h[2];
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
+{
+    pa_device_s *dev = DEV( pd );
+    PA_HIP( hipSetDevice( dev->device ) );
+    pa_free_factors( dev );
+    dev->prec = prec;
+    dev->lu   = lu;
+    if ( prec == PA_PREC_FLOAT ) return pa_coeftab_init_t<float>( dev, lu, overflow );
+    return pa_coeftab_init_t<double>( dev, lu, overflow );
+}
+
+/* ------------------------------------------------------------------------- */
+/* factorization                                                             */
+/* ------------------------------------------------------------------------- */
+template <typename T>
+static int
+pa_factorize_t( pa_device_s *dev, double crit )
+{
+    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
+    const int lu = dev->lu;
+    for ( const pa_level_t &lev : dev->levels ) {
+        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
+        if ( ncb == 0 ) continue;
+        size_t lds_diag = (size_t)PA_DIAG_NB * lev.ncmax * sizeof( T );
+        if ( lu )
+            k_diag<T, true><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
+                                                                 crit, dev->d_counters, dev->d_counters + 1 );
+        else
+            k_diag<T, false><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
+                                                                  crit, dev->d_counters, dev->d_counters + 1 );
+        PA_HIP( hipGetLastError() );
+        int64_t ntr = lev.trsm_end - lev.trsm_beg;
+        if ( ntr > 0 ) {
+            size_t lds_trsm = (size_t)PA_TRSM_TR * lev.ncmax * sizeof( T );
+            if ( lu ) {
+                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            }
+            else {
+                k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            }
+            PA_HIP( hipGetLastError() );
+        }
+        int64_t ng = lev.gemm_end - lev.gemm_beg;
+        if ( ng > 0 ) {
+            k_gemm_level<T><<<(unsigned)ng, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.gemm_beg, L,
+                                                                   lu ? U : nullptr );
+            PA_HIP( hipGetLastError() );
+        }
+    }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, double *seconds )
+{
+    pa_device_s *dev = DEV( pd );
+    int rc;
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipMemsetAsync( dev->d_counters, 0, 2 * sizeof( int32_t ), dev->stream ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    auto t0 = std::chrono::steady_clock::now();
+    PA_HIP( hipEventRecord( dev->ev0, dev->stream ) );
+    if ( dev->prec == PA_PREC_FLOAT ) rc = pa_factorize_t<float>( dev, crit );
+    else                              rc = pa_factorize_t<double>( dev, crit );
+    if ( rc ) return rc;
+    PA_HIP( hipEventRecord( dev->ev1, dev->stream ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    hipEventElapsedTime( &ms, dev->ev0, dev->ev1 );
+    dev->last_fact_event_s = ms * 1e-3;
+    if ( seconds ) *seconds = std::chrono::duration<double>( t1 - t0 ).count();
+    int32_t h[2];
+    PA_HIP( hipMemcpy( h, dev->d_counters, sizeof( h ), hipMemcpyDeviceToHost ) );
+    if ( npiv ) *npiv = h[0];
+    if ( nneg ) *nneg = h[1];
+    return PASTIX_SUCCESS;
+}
+
+extern "C" double
+pa_dev_last_fact_event_seconds( const pastix_data_t *pd )
+{
+    return DEV( pd ) ? DEV( pd )->last_fact_event_s : 0.;
+}
+
+/* ------------------------------------------------------------------------- */
+/* solve                                                                     */
+/* ------------------------------------------------------------------------- */
+template <typename T>
+static int
+pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b, T *x, int32_t nrhs, int64_t ld )
+{
+    const size_t nl = dev->levels.size();
+    for ( size_t li = 0; li < nl; li++ ) {
+        const pa_level_t &lev = dev->levels[li];
+        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
+        if ( !ncb ) continue;
+        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
+        if ( lds > (size_t)dev->lds_limit ) return PASTIX_ERR_BADPARAMETER;
+        if ( unit_f )
+            k_solve_diag<T, false, true><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cf, x, nrhs, ld );
+        else
+            k_solve_diag<T, false, false><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cf, x, nrhs, ld );
+        int64_t ns = lev.solve_end - lev.solve_beg;
+        if ( ns > 0 )
+            k_solve_fwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x, nrhs, ld );
+        PA_HIP( hipGetLastError() );
+    }
+    for ( size_t li = nl; li-- > 0; ) {
+        const pa_level_t &lev = dev->levels[li];
+        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
+        if ( !ncb ) continue;
+        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
+        int64_t ns = lev.solve_end - lev.solve_beg;
+        if ( ns > 0 )
+            k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cb, x, nrhs, ld );
+        if ( unit_b )
+            k_solve_diag<T, true, true><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cb, x, nrhs, ld );
+        else
+            k_solve_diag<T, true, false><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cb, x, nrhs, ld );
+        PA_HIP( hipGetLastError() );
+    }
+    return PASTIX_SUCCESS;
+}
+
+static unsigned
+pa_grid( int64_t n )
+{
+    int64_t g = ( n + 255 ) / 256;
+    return (unsigned)std::max<int64_t>( 1, std::min<int64_t>( g, 65536 ) );
+}
+
+/*
+ * Factors used by each sweep (reference sopalin/pastix_task_solve.c:543-649):
+ *   LL^t:        L y = b (non-unit),  L^t x = y (non-unit)
+ *   LU:          L y = b (unit),      U x = y (U^T tiles, non-unit)
+ *   LU, trans:   U^t y = b (U^T tiles, non-unit), L^t x = y (unit)
+ */
+extern "C" int
+pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_x, int64_t ld )
+{
+    pa_device_s *dev = DEV( pd );
+    PA_HIP( hipSetDevice( dev->device ) );
+    if ( !dev->d_L ) return PASTIX_ERR_BADPARAMETER;
+    bool lu = dev->lu, tr = ( trans != PastixNoTrans );
+    int fwd_side = ( lu && tr ) ? 1 : 0, bwd_side = ( lu && !tr ) ? 1 : 0;
+    bool unit_f = lu && !tr, unit_b = lu && tr;
+    if ( dev->prec == PA_PREC_DOUBLE ) {
+        const double *L = (const double *)dev->d_L, *U = (const double *)dev->d_U;
+        return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, d_x, nrhs, ld );
+    }
+    /* mixed precision: down-convert, FP32 sweeps, up-convert
+     * (reference sopalin/pastix_task_solve.c:583-592, pastix_rhs.c:162-242) */
+    int64_t len = dev->n * (int64_t)nrhs;
+    if ( dev->xs_len < len ) {
+        hipFree( dev->d_xs );
+        PA_HIP( hipMalloc( (void **)&dev->d_xs, std::max<int64_t>( len, 1 ) * sizeof( float ) ) );
+        dev->xs_len = len;
+    }
+    for ( int32_t r = 0; r < nrhs; r++ )
+        k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
+    const float *L = (const float *)dev->d_L, *U = (const float *)dev->d_U;
+    int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, dev->d_xs, nrhs, dev->n );
+    if ( rc ) return rc;
+    for ( int32_t r = 0; r < nrhs; r++ )
+        k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_spmv_perm( pastix_data_t *pd, pastix_trans_t trans, double alpha, const double *d_x, double beta, double *d_y )
+{
+    pa_device_s *dev = DEV( pd );
+    int64_t n = dev->n;
+    /* NoTrans uses A^T stored by columns (Uvalues), Trans uses Lvalues */
+    const double *vals = ( trans == PastixNoTrans ) ? dev->d_Uv : dev->d_Lv;
+    unsigned grid = (unsigned)( ( n * 8 + 255 ) / 256 );
+    if ( n > 0 )
+        k_spmv<<<grid, 256, 0, dev->stream>>>( (int32_t)n, dev->d_colptr, dev->d_rowtab, vals, alpha, d_x, beta, d_y );
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_apply_perm( pastix_data_t *pd, int forward, int32_t nrhs, const double *d_src, int64_t lds, double *d_dst, int64_t ldd )
+{
+    pa_device_s *dev = DEV( pd );
+    int64_t n = dev->n;
+    if ( n == 0 ) return PASTIX_SUCCESS;
+    /* forward: x_perm[i] = x[peritab[i]]  (P b); backward: x[peritab[i]] = x_perm[i] */
+    if ( forward )
+        k_gather<double, double><<<pa_grid( n * nrhs ), 256, 0, dev->stream>>>( n, nrhs, dev->d_invp, d_src, lds, d_dst, ldd );
+    else
+        k_scatter<double, double><<<pa_grid( n * nrhs ), 256, 0, dev->stream>>>( n, nrhs, dev->d_invp, d_src, lds, d_dst, ldd );
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------- */
+/* vector helpers                                                            */
+/* ------------------------------------------------------------------------- */
+extern "C" void *
+pa_dev_malloc( size_t bytes )
+{
+    void *p = NULL;
+    if ( hipMalloc( &p, std::max<size_t>( bytes, 8 ) ) != hipSuccess ) return NULL;
+    return p;
+}
+extern "C" void
+pa_dev_free( void *p )
+{
+    if ( p ) hipFree( p );
+}
+extern "C" int
+pa_dev_memcpy_h2d( pastix_data_t *pd, void *d, const void *h, size_t bytes )
+{
+    PA_HIP( hipMemcpyAsync( d, h, bytes, hipMemcpyHostToDevice, DEV( pd )->stream ) );
+    PA_HIP( hipStreamSynchronize( DEV( pd )->stream ) );
+    return PASTIX_SUCCESS;
+}
+extern "C" int
+pa_dev_memcpy_d2h( pastix_data_t *pd, void *h, const void *d, size_t bytes )
+{
+    PA_HIP( hipMemcpyAsync( h, d, bytes, hipMemcpyDeviceToHost, DEV( pd )->stream ) );
+    PA_HIP( hipStreamSynchronize( DEV( pd )->stream ) );
+    return PASTIX_SUCCESS;
+}
+extern "C" int
+pa_dev_memcpy_d2d( pastix_data_t *pd, void *d, const void *s, size_t bytes )
+{
+    PA_HIP( hipMemcpyAsync( d, s, bytes, hipMemcpyDeviceToDevice, DEV( pd )->stream ) );
+    return PASTIX_SUCCESS;
+}
+extern "C" int
+pa_dev_sync( pastix_data_t *pd )
+{
+    PA_HIP( hipStreamSynchronize( DEV( pd )->stream ) );
+    return PASTIX_SUCCESS;
+}
+
+static int
+pa_reduce_partials( pa_device_s *dev, int count, double *out, int nout )
+{
+    PA_HIP( hipMemcpyAsync( dev->h_partial, dev->d_partial, (size_t)count * nout * sizeof( double ),
+                            hipMemcpyDeviceToHost, dev->stream ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    for ( int j = 0; j < nout; j++ ) {
+        double s = 0.;
+        for ( int i = 0; i < count; i++ ) s += dev->h_partial[j * count + i];
+        out[j] = s;
+    }
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_mdot( pastix_data_t *pd, int64_t n, int32_t m, const double *X, const double *v, double *out )
+{
+    pa_device_s *dev = DEV( pd );
+    int blocks = (int)std::min<int64_t>( ( n + 255 ) / 256, 4096 / std::max( m, 1 ) );
+    blocks = std::max( blocks, 1 );
+    blocks = std::min( blocks, 256 );
+    dim3 grid( blocks, m );
+    k_mdot<<<grid, 256, 0, dev->stream>>>( n, X, n, v, dev->d_partial );
+    PA_HIP( hipGetLastError() );
+    return pa_reduce_partials( dev, blocks, out, m );
+}
+
+extern "C" double
+pa_dev_dot( pastix_data_t *pd, int64_t n, const double *x, const double *y )
+{
+    double out = 0.;
+    pa_device_s *dev = DEV( pd );
+    int blocks = (int)std::max<int64_t>( 1, std::min<int64_t>( ( n + 255 ) / 256, 256 ) );
+    k_mdot<<<dim3( blocks, 1 ), 256, 0, dev->stream>>>( n, x, n, y, dev->d_partial );
+    pa_reduce_partials( dev, blocks, &out, 1 );
+    return out;
+}
+
+extern "C" double
+pa_dev_nrm2( pastix_data_t *pd, int64_t n, const double *x )
+{
+    return sqrt( pa_dev_dot( pd, n, x, x ) );
+}
+
+extern "C" int
+pa_dev_axpby( pastix_data_t *pd, int64_t n, double a, const double *x, double b, double *y )
+{
+    k_axpby<<<pa_grid( n ), 256, 0, DEV( pd )->stream>>>( n, a, x, b, y );
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_gemv( pastix_data_t *pd, int64_t n, int32_t m, const double *X, const double *c, double alpha, double *y )
+{
+    pa_device_s *dev = DEV( pd );
+    double *dc = dev->d_partial; /* m <= 4096 coefficients */
+    PA_HIP( hipMemcpyAsync( dc, c, m * sizeof( double ), hipMemcpyHostToDevice, dev->stream ) );
+    k_gemv_n<<<pa_grid( n ), 256, 0, dev->stream>>>( n, m, X, n, dc, alpha, y );
+    PA_HIP( hipGetLastError() );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_get_coeftab( const pastix_data_t *pd, int side, double *host )
+{
+    pa_device_s *dev = DEV( pd );
+    void *src = side ? dev->d_U : dev->d_L;
+    if ( !src ) return PASTIX_ERR_BADPARAMETER;
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    if ( dev->prec == PA_PREC_DOUBLE ) {
+        PA_HIP( hipMemcpy( host, src, dev->coefnbr * sizeof( double ), hipMemcpyDeviceToHost ) );
+    }
+    else {
+        std::vector<float> tmp( dev->coefnbr );
+        PA_HIP( hipMemcpy( tmp.data(), src, dev->coefnbr * sizeof( float ), hipMemcpyDeviceToHost ) );
+        for ( int64_t i = 0; i < dev->coefnbr; i++ ) host[i] = tmp[i];
+    }
+    return PASTIX_SUCCESS;
+}

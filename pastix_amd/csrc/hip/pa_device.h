@@ -1,0 +1,70 @@
+/*
+ * pa_device.h -- the thin C-ABI between the C host library (API, analysis,
+ * refinement drivers) and the HIP device layer.  Every pointer named d_* is
+ * device memory; vectors of the refinement live in the factorization
+ * (permuted) numbering.
+ */
+#ifndef PASTIX_AMD_DEVICE_H
+#define PASTIX_AMD_DEVICE_H
+
+#include <stdint.h>
+#include "pastix.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* factor arithmetic */
+#define PA_PREC_FLOAT  2
+#define PA_PREC_DOUBLE 3
+
+int    pa_dev_open( pastix_data_t *pastix_data );
+void   pa_dev_close( pastix_data_t *pastix_data );
+/* upload the SolverMatrix, permutation and the level schedule */
+int    pa_dev_setup_structure( pastix_data_t *pastix_data );
+/* upload the permuted CSC (pattern + FP64 values) */
+int    pa_dev_upload_bcsc( pastix_data_t *pastix_data );
+/* allocate the coeftab in HBM (prec) and fill it from the device CSC */
+int    pa_dev_coeftab_init( pastix_data_t *pastix_data, int prec, int lu, int32_t *overflow );
+/* level-batched numerical factorization; returns PASTIX_SUCCESS */
+int    pa_dev_factorize( pastix_data_t *pastix_data, double criterion, int32_t *npiv,
+                         int32_t *nneg, double *seconds );
+/* in-place solve of op(A) x = b on a device FP64 buffer in the factorization
+ * numbering (nrhs columns, leading dimension ld) */
+int    pa_dev_solve_perm( pastix_data_t *pastix_data, pastix_trans_t trans, int32_t nrhs,
+                          double *d_x, int64_t ld );
+/* y = alpha op(A) x + beta y, factorization numbering */
+int    pa_dev_spmv_perm( pastix_data_t *pastix_data, pastix_trans_t trans, double alpha,
+                         const double *d_x, double beta, double *d_y );
+/* user numbering <-> factorization numbering (nrhs columns) */
+int    pa_dev_apply_perm( pastix_data_t *pastix_data, int forward, int32_t nrhs,
+                          const double *d_src, int64_t lds, double *d_dst, int64_t ldd );
+
+/* FP64 vector helpers on the library stream */
+void  *pa_dev_malloc( size_t bytes );
+void   pa_dev_free( void *d_ptr );
+int    pa_dev_memcpy_h2d( pastix_data_t *pastix_data, void *d_dst, const void *h_src, size_t bytes );
+int    pa_dev_memcpy_d2h( pastix_data_t *pastix_data, void *h_dst, const void *d_src, size_t bytes );
+int    pa_dev_memcpy_d2d( pastix_data_t *pastix_data, void *d_dst, const void *d_src, size_t bytes );
+double pa_dev_dot( pastix_data_t *pastix_data, int64_t n, const double *d_x, const double *d_y );
+double pa_dev_nrm2( pastix_data_t *pastix_data, int64_t n, const double *d_x );
+/* out[j] = X[:, j] . v for j < m (X is n x m, ld n) */
+int    pa_dev_mdot( pastix_data_t *pastix_data, int64_t n, int32_t m, const double *d_X,
+                    const double *d_v, double *out );
+/* y = a x + b y */
+int    pa_dev_axpby( pastix_data_t *pastix_data, int64_t n, double a, const double *d_x,
+                     double b, double *d_y );
+/* y += alpha X c, X n x m (ld n), c on the host */
+int    pa_dev_gemv( pastix_data_t *pastix_data, int64_t n, int32_t m, const double *d_X,
+                    const double *c, double alpha, double *d_y );
+int    pa_dev_sync( pastix_data_t *pastix_data );
+/* copy a factor from HBM to the host as FP64 (side 0: L, 1: U^T) */
+int    pa_dev_get_coeftab( const pastix_data_t *pastix_data, int side, double *host );
+pastix_int_t pa_dev_nlevels( const pastix_data_t *pastix_data );
+double pa_dev_last_fact_event_seconds( const pastix_data_t *pastix_data );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASTIX_AMD_DEVICE_H */

@@ -1,0 +1,695 @@
+/*
+ * api.c -- the PaStiX public API on top of the CPU analysis and the GPU
+ * numerical engine.
+ *
+ * Step order, parameter defaults, error codes and output parameters follow
+ * the reference (common/api.c:420-603, 905-960; blend/pastix_task_analyze.c:53;
+ * sopalin/pastix_task_sopalin.c:93-710; sopalin/pastix_task_solve.c:169-809).
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <time.h>
+#include <float.h>
+#include "internal.h"
+#include "../hip/pa_device.h"
+
+void
+pa_error( const char *fmt, ... )
+{
+    va_list ap;
+    va_start( ap, fmt );
+    fprintf( stderr, "pastix_amd error: " );
+    vfprintf( stderr, fmt, ap );
+    fprintf( stderr, "\n" );
+    va_end( ap );
+}
+
+void
+pa_warning( const char *fmt, ... )
+{
+    va_list ap;
+    va_start( ap, fmt );
+    fprintf( stderr, "pastix_amd warning: " );
+    vfprintf( stderr, fmt, ap );
+    fprintf( stderr, "\n" );
+    va_end( ap );
+}
+
+double
+pa_wtime( void )
+{
+    struct timespec ts;
+    clock_gettime( CLOCK_MONOTONIC, &ts );
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Defaults of reference common/api.c:420-603.  This build has no Scotch or
+ * Metis: the default ordering is the personal one (a grid ordering given to
+ * pastix_subtask_order, or the elimination-tree fallback). */
+void
+pastixInitParam( pastix_int_t *iparm, double *dparm )
+{
+    memset( iparm, 0, IPARM_SIZE * sizeof( pastix_int_t ) );
+    memset( dparm, 0, DPARM_SIZE * sizeof( double ) );
+    iparm[IPARM_VERBOSE]                        = PastixVerboseNo;
+    iparm[IPARM_IO_STRATEGY]                    = PastixIONo;
+    iparm[IPARM_TRACE]                          = PastixTraceNot;
+    iparm[IPARM_ORDERING]                       = PastixOrderPersonal;
+    iparm[IPARM_ORDERING_DEFAULT]               = 1;
+    iparm[IPARM_SCOTCH_MT]                      = 1;
+    iparm[IPARM_SCOTCH_SWITCH_LEVEL]            = 120;
+    iparm[IPARM_SCOTCH_CMIN]                    = 0;
+    iparm[IPARM_SCOTCH_CMAX]                    = 100000;
+    iparm[IPARM_SCOTCH_FRAT]                    = 8;
+    iparm[IPARM_METIS_CTYPE]                    = -1;
+    iparm[IPARM_METIS_RTYPE]                    = -1;
+    iparm[IPARM_METIS_NSEPS]                    = 1;
+    iparm[IPARM_METIS_NITER]                    = 10;
+    iparm[IPARM_METIS_UFACTOR]                  = 200;
+    iparm[IPARM_METIS_COMPRESS]                 = 1;
+    iparm[IPARM_METIS_SEED]                     = 3452;
+    iparm[IPARM_AMALGAMATION_LVLCBLK]           = 5;
+    iparm[IPARM_AMALGAMATION_LVLBLAS]           = 5;
+    iparm[IPARM_REORDERING_SPLIT]               = 0;
+    iparm[IPARM_REORDERING_STOP]                = PASTIX_INT_MAX;
+    iparm[IPARM_SPLITTING_STRATEGY]             = PastixSplitKway;
+    iparm[IPARM_SPLITTING_LEVELS_PROJECTIONS]   = 0;
+    iparm[IPARM_SPLITTING_LEVELS_KWAY]          = PASTIX_INT_MAX;
+    iparm[IPARM_SPLITTING_PROJECTIONS_DEPTH]    = 3;
+    iparm[IPARM_SPLITTING_PROJECTIONS_DISTANCE] = 3;
+    iparm[IPARM_SPLITTING_PROJECTIONS_WIDTH]    = 1;
+    iparm[IPARM_MIN_BLOCKSIZE]                  = 160;
+    iparm[IPARM_MAX_BLOCKSIZE]                  = 320;
+    iparm[IPARM_TASKS2D_LEVEL]                  = -1;
+    iparm[IPARM_TASKS2D_WIDTH]                  = iparm[IPARM_MIN_BLOCKSIZE];
+    iparm[IPARM_FACTORIZATION]                  = PastixFactLU;
+    iparm[IPARM_FACTO_LOOK_SIDE]                = PastixFactRightLooking;
+    iparm[IPARM_SCHUR_FACT_MODE]                = PastixFactModeLocal;
+    iparm[IPARM_TRANSPOSE_SOLVE]                = PastixNoTrans;
+    iparm[IPARM_SCHUR_SOLV_MODE]                = PastixSolvModeLocal;
+    iparm[IPARM_APPLYPERM_WS]                   = 1;
+    iparm[IPARM_REFINEMENT]                     = PastixRefineGMRES;
+    iparm[IPARM_ITERMAX]                        = 250;
+    iparm[IPARM_GMRES_IM]                       = 25;
+    iparm[IPARM_SCHEDULER]                      = PastixSchedDynamic;
+    iparm[IPARM_THREAD_NBR]                     = -1;
+    iparm[IPARM_SOCKET_NBR]                     = -1;
+    iparm[IPARM_GPU_NBR]                        = 1;
+    iparm[IPARM_GPU_MEMORY_PERCENTAGE]          = 95;
+    iparm[IPARM_GPU_MEMORY_BLOCK_SIZE]          = 32 * 1024;
+    iparm[IPARM_GLOBAL_ALLOCATION]              = 1;
+    iparm[IPARM_COMPRESS_MIN_WIDTH]             = 128;
+    iparm[IPARM_COMPRESS_MIN_HEIGHT]            = 20;
+    iparm[IPARM_COMPRESS_WHEN]                  = PastixCompressNever;
+    iparm[IPARM_COMPRESS_METHOD]                = PastixCompressMethodPQRCP;
+    iparm[IPARM_COMPRESS_ORTHO]                 = PastixCompressOrthoCGS;
+    iparm[IPARM_COMPRESS_PRESELECT]             = 1;
+    iparm[IPARM_COMPRESS_ILUK]                  = -2;
+    iparm[IPARM_MIXED]                          = 0;
+    iparm[IPARM_MPI_THREAD_LEVEL]               = PastixMpiNone;
+    iparm[IPARM_MODIFY_PARAMETER]               = 1;
+    iparm[IPARM_START_TASK]                     = PastixTaskOrdering;
+    iparm[IPARM_END_TASK]                       = PastixTaskClean;
+    iparm[IPARM_FLOAT]                          = -1;
+    iparm[IPARM_MTX_TYPE]                       = -1;
+    iparm[IPARM_DOF_NBR]                        = 1;
+    dparm[DPARM_EPSILON_REFINEMENT]             = -1.;
+    dparm[DPARM_RELATIVE_ERROR]                 = -1.;
+    dparm[DPARM_EPSILON_MAGN_CTRL]              = 0.;
+    dparm[DPARM_A_NORM]                         = -1.;
+    dparm[DPARM_COMPRESS_TOLERANCE]             = 1e-8;
+    dparm[DPARM_COMPRESS_MIN_RATIO]             = 1.;
+}
+
+int
+pastixCheckParam( const pastix_int_t *iparm, const double *dparm )
+{
+    (void)dparm;
+    if ( iparm[IPARM_FACTORIZATION] != PastixFactLLT && iparm[IPARM_FACTORIZATION] != PastixFactPOTRF &&
+         iparm[IPARM_FACTORIZATION] != PastixFactLU ) {
+        pa_error( "factorization %ld is not available in this build (LL^t and LU only)",
+                  (long)iparm[IPARM_FACTORIZATION] );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    if ( iparm[IPARM_MIN_BLOCKSIZE] < 1 || iparm[IPARM_MAX_BLOCKSIZE] < iparm[IPARM_MIN_BLOCKSIZE] ) {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( iparm[IPARM_REFINEMENT] < PastixRefineGMRES || iparm[IPARM_REFINEMENT] > PastixRefineSR ) {
+        pa_error( "refinement %ld is not available in this build (GMRES, CG, SR)",
+                  (long)iparm[IPARM_REFINEMENT] );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    return PASTIX_SUCCESS;
+}
+
+static pastix_int_t pa_instance_counter = 0;
+
+void
+pastixInitWithAffinity( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t *iparm,
+                        double *dparm, const int *bindtab )
+{
+    pastix_data_t *pd = calloc( 1, sizeof( pastix_data_t ) );
+    (void)bindtab;
+    if ( iparm[IPARM_MODIFY_PARAMETER] == 0 ) {
+        pastixInitParam( iparm, dparm );
+    }
+    pd->id          = pa_instance_counter++;
+    pd->iparm       = iparm;
+    pd->dparm       = dparm;
+    pd->pastix_comm = comm;
+    pd->steps       = STEP_INIT;
+    if ( pa_dev_open( pd ) != PASTIX_SUCCESS ) {
+        pa_error( "pastixInit: could not open the GPU" );
+    }
+    *pastix_data = pd;
+}
+
+void
+pastixInit( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t *iparm, double *dparm )
+{
+    pastixInitWithAffinity( pastix_data, comm, iparm, dparm, NULL );
+}
+
+static void
+pa_clear_analysis( pastix_data_t *pd )
+{
+    if ( pd->ordemesh ) { pastixOrderExit( pd->ordemesh ); free( pd->ordemesh ); pd->ordemesh = NULL; }
+    if ( pd->symbmtx )  { pa_symbol_exit( pd->symbmtx ); free( pd->symbmtx ); pd->symbmtx = NULL; }
+    if ( pd->solvmatr ) { pa_solver_exit( pd->solvmatr ); free( pd->solvmatr ); pd->solvmatr = NULL; }
+    if ( pd->graph )    { pa_graph_exit( pd->graph ); free( pd->graph ); pd->graph = NULL; }
+}
+
+void
+pastixFinalize( pastix_data_t **pastix_data )
+{
+    pastix_data_t *pd;
+    if ( pastix_data == NULL || *pastix_data == NULL ) return;
+    pd = *pastix_data;
+    pa_dev_close( pd );
+    pa_clear_analysis( pd );
+    if ( pd->bcsc ) { pa_bcsc_exit( pd->bcsc ); free( pd->bcsc ); }
+    free( pd );
+    *pastix_data = NULL;
+}
+
+static int
+pa_check_spm( const spmatrix_t *spm )
+{
+    if ( spm == NULL || spm->n < 0 ) return PASTIX_ERR_BADPARAMETER;
+    if ( spm->dof != 1 || spm->dofs != NULL ) {
+        pa_error( "matrices with dof != 1 are not supported" );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    if ( spm->loc2glob != NULL ) {
+        pa_error( "distributed matrices are not supported (single process build)" );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    if ( spm->fmttype != SpmCSC && spm->fmttype != SpmCSR ) {
+        pa_error( "only CSC/CSR matrices are supported" );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* Ordering step (reference order/pastix_subtask_order.c:97-470). */
+int
+pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *myorder )
+{
+    pastix_int_t  n;
+    pastix_order_t *o;
+    double t0 = pa_wtime();
+    int rc;
+
+    if ( pd == NULL || spm == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( !( pd->steps & STEP_INIT ) ) return PASTIX_ERR_BADPARAMETER;
+    if ( ( rc = pa_check_spm( spm ) ) ) return rc;
+    pd->iparm[IPARM_FLOAT] = spm->flttype;
+    pd->iparm[IPARM_MTX_TYPE] = spm->mtxtype;
+    pa_clear_analysis( pd );
+    pd->csc = spm;
+    n = spm->n;
+
+    pd->graph = calloc( 1, sizeof( pa_graph_t ) );
+    if ( ( rc = pa_graph_from_spm( pd->graph, spm ) ) ) return rc;
+
+    if ( pd->iparm[IPARM_ORDERING] != PastixOrderPersonal ) {
+        pa_error( "pastix_subtask_order: ordering %ld needs Scotch/Metis, which this build does not "
+                  "provide; use PastixOrderPersonal (e.g. with pastixOrderGrid)",
+                  (long)pd->iparm[IPARM_ORDERING] );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+
+    o = calloc( 1, sizeof( pastix_order_t ) );
+    pd->ordemesh = o;
+    pastixOrderAlloc( o, n, 0 );
+    if ( myorder != NULL ) {
+        pastixOrderBase( myorder, 0 );
+    }
+    /* permutation (reference order/order_compute_personal.c) */
+    if ( myorder == NULL || ( myorder->permtab == NULL && myorder->peritab == NULL ) ) {
+        pastix_int_t i;
+        for ( i = 0; i < n; i++ ) o->permtab[i] = o->peritab[i] = i;
+    }
+    else if ( myorder->permtab == NULL ) {
+        pastix_int_t i;
+        for ( i = 0; i < n; i++ ) o->permtab[myorder->peritab[i]] = i;
+        memcpy( o->peritab, myorder->peritab, n * sizeof( pastix_int_t ) );
+    }
+    else if ( myorder->peritab == NULL ) {
+        pastix_int_t i;
+        for ( i = 0; i < n; i++ ) o->peritab[myorder->permtab[i]] = i;
+        memcpy( o->permtab, myorder->permtab, n * sizeof( pastix_int_t ) );
+    }
+    else {
+        memcpy( o->permtab, myorder->permtab, n * sizeof( pastix_int_t ) );
+        memcpy( o->peritab, myorder->peritab, n * sizeof( pastix_int_t ) );
+    }
+    if ( myorder != NULL && myorder->rangtab != NULL && myorder->treetab != NULL ) {
+        o->cblknbr = myorder->cblknbr;
+        o->rangtab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+        o->treetab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+        memcpy( o->rangtab, myorder->rangtab, ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+        memcpy( o->treetab, myorder->treetab, o->cblknbr * sizeof( pastix_int_t ) );
+    }
+    else {
+        pa_order_from_etree( o, n, pd->graph->colptr, pd->graph->rows );
+    }
+    if ( pastixOrderCheck( o ) != PASTIX_SUCCESS ) {
+        pa_error( "pastix_subtask_order: inconsistent ordering" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    pa_order_apply_level_order( o, pd->iparm[IPARM_TASKS2D_LEVEL], pd->iparm[IPARM_TASKS2D_WIDTH] );
+    o->sndenbr = o->cblknbr;
+    o->sndetab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+    memcpy( o->sndetab, o->rangtab, ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+
+    pd->dparm[DPARM_ORDER_TIME] = pa_wtime() - t0;
+    pd->steps |= STEP_ORDERING;
+    return PASTIX_SUCCESS;
+}
+
+/* Symbolic factorization (reference symbol/pastix_subtask_symbfact.c:86-330). */
+int
+pastix_subtask_symbfact( pastix_data_t *pd )
+{
+    double t0 = pa_wtime();
+    int rc;
+    if ( pd == NULL || !( pd->steps & STEP_ORDERING ) ) return PASTIX_ERR_BADPARAMETER;
+    if ( pd->symbmtx ) { pa_symbol_exit( pd->symbmtx ); free( pd->symbmtx ); }
+    pd->symbmtx = calloc( 1, sizeof( symbol_matrix_t ) );
+    if ( ( rc = pa_symbol_fax( pd->symbmtx, pd->graph, pd->ordemesh ) ) ) return rc;
+    pa_symbol_build_browtab( pd->symbmtx );
+    if ( pa_symbol_check( pd->symbmtx ) ) {
+        pa_error( "symbolic factorization produced an inconsistent structure" );
+        return PASTIX_ERR_INTERNAL;
+    }
+    pd->iparm[IPARM_NNZEROS] = pa_symbol_nnz( pd->symbmtx );
+    pd->dparm[DPARM_FACT_THFLOPS] = pa_symbol_flops( pd->symbmtx, (int)pd->iparm[IPARM_FACTORIZATION] );
+    pd->dparm[DPARM_SYMBFACT_TIME] = pa_wtime() - t0;
+    pd->steps |= STEP_SYMBFACT;
+    return PASTIX_SUCCESS;
+}
+
+/* The reference reorders the rows inside each supernode to gather coupling
+ * blocks (symbol/symbol_reordering.c); this build keeps the ordering as is. */
+int
+pastix_subtask_reordering( pastix_data_t *pd )
+{
+    if ( pd == NULL || !( pd->steps & STEP_SYMBFACT ) ) return PASTIX_ERR_BADPARAMETER;
+    pd->dparm[DPARM_REORDER_TIME] = 0.;
+    return PASTIX_SUCCESS;
+}
+
+/* Splitting and solver structure (reference blend/pastix_subtask_blend.c:121-600). */
+int
+pastix_subtask_blend( pastix_data_t *pd )
+{
+    double t0 = pa_wtime();
+    int rc, flt;
+    if ( pd == NULL || !( pd->steps & STEP_SYMBFACT ) ) return PASTIX_ERR_BADPARAMETER;
+    if ( ( rc = pastixCheckParam( pd->iparm, pd->dparm ) ) ) return rc;
+    if ( ( rc = pa_split_symbol( pd->symbmtx, pd->iparm[IPARM_MIN_BLOCKSIZE], pd->iparm[IPARM_MAX_BLOCKSIZE] ) ) )
+        return rc;
+    pa_symbol_build_browtab( pd->symbmtx );
+    if ( pa_symbol_check( pd->symbmtx ) ) {
+        pa_error( "splitting produced an inconsistent structure" );
+        return PASTIX_ERR_INTERNAL;
+    }
+    pd->dparm[DPARM_FACT_THFLOPS] = pa_symbol_flops( pd->symbmtx, (int)pd->iparm[IPARM_FACTORIZATION] );
+    pd->iparm[IPARM_NNZEROS]      = pa_symbol_nnz( pd->symbmtx );
+    if ( pd->solvmatr ) { pa_solver_exit( pd->solvmatr ); free( pd->solvmatr ); }
+    pd->solvmatr = calloc( 1, sizeof( SolverMatrix ) );
+    flt = pd->iparm[IPARM_MIXED] ? PastixFloat : PastixDouble;
+    if ( ( rc = pa_solver_gen( pd->solvmatr, pd->symbmtx, (int)pd->iparm[IPARM_FACTORIZATION], flt ) ) ) return rc;
+    /* the analysis runs without a GPU; the structure is uploaded as soon as
+     * one is available (here, or at the first numerical step) */
+    if ( pd->dev != NULL ) {
+        if ( ( rc = pa_dev_setup_structure( pd ) ) ) return rc;
+    }
+    pd->dparm[DPARM_BLEND_TIME] = pa_wtime() - t0;
+    pd->steps |= STEP_ANALYSE;
+    return PASTIX_SUCCESS;
+}
+
+int
+pastix_task_analyze( pastix_data_t *pd, const spmatrix_t *spm )
+{
+    int rc;
+    double t0 = pa_wtime();
+    if ( pd == NULL || spm == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( !( pd->steps & STEP_ORDERING ) || pd->csc != spm ) {
+        if ( ( rc = pastix_subtask_order( pd, spm, NULL ) ) ) return rc;
+    }
+    if ( ( rc = pastix_subtask_symbfact( pd ) ) ) return rc;
+    if ( ( rc = pastix_subtask_reordering( pd ) ) ) return rc;
+    if ( ( rc = pastix_subtask_blend( pd ) ) ) return rc;
+    pd->dparm[DPARM_ANALYZE_TIME] = pa_wtime() - t0;
+    return PASTIX_SUCCESS;
+}
+
+/* Permuted CSC, ||A||_F (reference sopalin/pastix_task_sopalin.c:93-210). */
+int
+pastix_subtask_spm2bcsc( pastix_data_t *pd, spmatrix_t *spm )
+{
+    int rc;
+    if ( pd == NULL || spm == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( !( pd->steps & STEP_ANALYSE ) ) {
+        pa_error( "pastix_subtask_spm2bcsc: the analysis has to be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( spm->n != pd->ordemesh->vertnbr ) return PASTIX_ERR_BADPARAMETER;
+    if ( pd->dev == NULL ) {
+        pa_error( "pastix_task_numfact: no GPU is available; this engine has no CPU path" );
+        return PASTIX_ERR_INTERNAL;
+    }
+    if ( pa_dev_nlevels( pd ) == 0 && pd->solvmatr->cblknbr > 0 ) {
+        if ( ( rc = pa_dev_setup_structure( pd ) ) ) return rc;
+    }
+    if ( pd->bcsc ) { pa_bcsc_exit( pd->bcsc ); free( pd->bcsc ); }
+    pd->bcsc = calloc( 1, sizeof( pa_bcsc_t ) );
+    if ( ( rc = pa_bcsc_init( pd->bcsc, spm, pd->ordemesh ) ) ) return rc;
+    pd->csc = spm;
+    pd->dparm[DPARM_A_NORM] = pd->bcsc->normF;
+    if ( ( rc = pa_dev_upload_bcsc( pd ) ) ) return rc;
+    pd->steps &= ~( STEP_BCSC2CTAB | STEP_NUMFACT );
+    pd->steps |= STEP_CSC2BCSC;
+    return PASTIX_SUCCESS;
+}
+
+/* Coeftab allocation and fill in HBM (reference :213-410, sopalin/coeftab.c). */
+int
+pastix_subtask_bcsc2ctab( pastix_data_t *pd )
+{
+    int32_t overflow = 0;
+    int prec, rc;
+    if ( pd == NULL || !( pd->steps & STEP_CSC2BCSC ) ) return PASTIX_ERR_BADPARAMETER;
+    prec = pd->iparm[IPARM_MIXED] ? PA_PREC_FLOAT : PA_PREC_DOUBLE;
+    if ( pd->iparm[IPARM_FLOAT] == SpmFloat ) prec = PA_PREC_FLOAT;
+    pd->solvmatr->flttype = ( prec == PA_PREC_FLOAT ) ? PastixFloat : PastixDouble;
+    rc = pa_dev_coeftab_init( pd, prec, pd->iparm[IPARM_FACTORIZATION] == PastixFactLU, &overflow );
+    if ( rc ) return rc;
+    if ( overflow ) {
+        pa_warning( "%d coefficients overflow the single precision range", overflow );
+    }
+    pd->steps |= STEP_BCSC2CTAB;
+    return PASTIX_SUCCESS;
+}
+
+/* Numerical factorization (reference :412-650). */
+int
+pastix_subtask_sopalin( pastix_data_t *pd )
+{
+    double threshold, seconds = 0.;
+    int32_t npiv = 0, nneg = 0;
+    int rc;
+    if ( pd == NULL || !( pd->steps & STEP_ANALYSE ) || !( pd->steps & STEP_CSC2BCSC ) ||
+         !( pd->steps & STEP_BCSC2CTAB ) ) {
+        pa_error( "pastix_subtask_sopalin: all steps up to pastix_subtask_bcsc2ctab() must be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    /* static pivoting threshold: eps * ||A||_F with the FP64 epsilon, the
+     * matrix being given in double (reference :465-494) */
+    if ( pd->dparm[DPARM_EPSILON_MAGN_CTRL] < 0. ) {
+        threshold = -pd->dparm[DPARM_EPSILON_MAGN_CTRL];
+    }
+    else if ( pd->dparm[DPARM_EPSILON_MAGN_CTRL] == 0. ) {
+        double eps = ( pd->iparm[IPARM_FLOAT] == SpmFloat ) ? (double)FLT_EPSILON / 2. : DBL_EPSILON / 2.;
+        threshold = eps * pd->dparm[DPARM_A_NORM];
+    }
+    else {
+        threshold = pd->dparm[DPARM_EPSILON_MAGN_CTRL] * pd->dparm[DPARM_A_NORM];
+    }
+    pd->solvmatr->diagthreshold = threshold;
+    if ( ( rc = pa_dev_factorize( pd, threshold, &npiv, &nneg, &seconds ) ) ) return rc;
+    pd->solvmatr->nbpivots = npiv;
+    if ( nneg > 0 ) {
+        pa_error( "Negative diagonal term (%d occurrence(s)) in the Cholesky factorization", nneg );
+    }
+    pd->iparm[IPARM_STATIC_PIVOTING] = npiv;
+    pd->dparm[DPARM_FACT_TIME]  = seconds;
+    pd->dparm[DPARM_FACT_FLOPS] = seconds > 0. ? pd->dparm[DPARM_FACT_THFLOPS] / seconds / 1073741824. : 0.;
+    pd->dparm[DPARM_FACT_RLFLOPS] = pd->dparm[DPARM_FACT_THFLOPS];
+    if ( pd->iparm[IPARM_VERBOSE] > PastixVerboseNot ) {
+        fprintf( stdout, "    Time to factorize                      %e s\n", seconds );
+        fprintf( stdout, "    Pivot thresholds: %d static pivot(s)\n", npiv );
+    }
+    pd->steps |= STEP_NUMFACT;
+    return PASTIX_SUCCESS;
+}
+
+int
+pastix_task_numfact( pastix_data_t *pd, spmatrix_t *spm )
+{
+    int rc;
+    if ( pd == NULL || spm == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( !( pd->steps & STEP_ANALYSE ) ) {
+        pa_error( "pastix_task_numfact: pastix_task_analyze() has to be called first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( ( rc = pastix_subtask_spm2bcsc( pd, spm ) ) ) return rc;
+    if ( ( rc = pastix_subtask_bcsc2ctab( pd ) ) ) return rc;
+    if ( ( rc = pastix_subtask_sopalin( pd ) ) ) return rc;
+    return PASTIX_SUCCESS;
+}
+
+int
+pastixRhsInit( pastix_rhs_t *rhs )
+{
+    *rhs = calloc( 1, sizeof( struct pastix_rhs_s ) );
+    return *rhs ? PASTIX_SUCCESS : PASTIX_ERR_ALLOC;
+}
+
+int
+pastixRhsFinalize( pastix_rhs_t rhs )
+{
+    if ( rhs && rhs->allocated ) free( rhs->b );
+    free( rhs );
+    return PASTIX_SUCCESS;
+}
+
+/* In-place device solve on an HBM buffer in the user numbering. */
+int
+pastix_amd_solve_dev( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrhs, void *dB, pastix_int_t ldb )
+{
+    double *tmp;
+    pastix_int_t n;
+    int rc;
+    double t0;
+    if ( pd == NULL || !( pd->steps & STEP_NUMFACT ) ) {
+        pa_error( "pastix_task_solve: all steps up to pastix_task_numfact() must be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    n = pd->solvmatr->nodenbr;
+    if ( nrhs < 1 || ldb < n ) return PASTIX_ERR_BADPARAMETER;
+    if ( transA != PastixNoTrans && transA != PastixTrans ) return PASTIX_ERR_BADPARAMETER;
+    t0  = pa_wtime();
+    tmp = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
+    if ( !tmp ) return PASTIX_ERR_OUTOFMEMORY;
+    rc = pa_dev_apply_perm( pd, 1, (int32_t)nrhs, dB, ldb, tmp, n );
+    if ( !rc ) rc = pa_dev_solve_perm( pd, transA, (int32_t)nrhs, tmp, n );
+    if ( !rc ) rc = pa_dev_apply_perm( pd, 0, (int32_t)nrhs, tmp, n, dB, ldb );
+    if ( !rc ) rc = pa_dev_sync( pd );
+    pa_dev_free( tmp );
+    pd->dparm[DPARM_SOLV_TIME] = pa_wtime() - t0;
+    if ( !rc ) pd->steps |= STEP_SOLVE;
+    return rc;
+}
+
+/* Host-buffer solve (reference sopalin/pastix_task_solve.c:758): B is
+ * overwritten by the solution; data crosses PCIe both ways. */
+int
+pastix_task_solve( pastix_data_t *pd, pastix_int_t m, pastix_int_t nrhs, void *B, pastix_int_t ldb )
+{
+    double *dB;
+    pastix_int_t n, j;
+    int rc;
+    if ( pd == NULL || !( pd->steps & STEP_NUMFACT ) ) {
+        pa_error( "pastix_task_solve: all steps up to pastix_task_numfact() must be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    n = pd->solvmatr->nodenbr;
+    if ( m != n || ldb < n || B == NULL ) return PASTIX_ERR_BADPARAMETER;
+    dB = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
+    if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
+    for ( j = 0; j < nrhs; j++ ) pa_dev_memcpy_h2d( pd, dB + j * n, (double *)B + j * ldb, n * sizeof( double ) );
+    rc = pastix_amd_solve_dev( pd, (pastix_trans_t)pd->iparm[IPARM_TRANSPOSE_SOLVE], nrhs, dB, n );
+    for ( j = 0; !rc && j < nrhs; j++ ) pa_dev_memcpy_d2h( pd, (double *)B + j * ldb, dB + j * n, n * sizeof( double ) );
+    pa_dev_free( dB );
+    return rc;
+}
+
+int
+pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t b )
+{
+    if ( b == NULL || b->flttype != PastixDouble ) return PASTIX_ERR_BADPARAMETER;
+    {
+        double *dB = pa_dev_malloc( (size_t)b->m * b->n * sizeof( double ) );
+        int rc;
+        pastix_int_t j;
+        if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
+        for ( j = 0; j < b->n; j++ ) pa_dev_memcpy_h2d( pd, dB + j * b->m, (double *)b->b + j * b->ld, b->m * sizeof( double ) );
+        /* b is already in the factorization numbering */
+        rc = pa_dev_solve_perm( pd, transA, (int32_t)b->n, dB, b->m );
+        for ( j = 0; !rc && j < b->n; j++ ) pa_dev_memcpy_d2h( pd, (double *)b->b + j * b->ld, dB + j * b->m, b->m * sizeof( double ) );
+        pa_dev_free( dB );
+        return rc;
+    }
+}
+
+double
+pastix_amd_last_fact_seconds( pastix_data_t *pd )
+{
+    return pd ? pa_dev_last_fact_event_seconds( pd ) : 0.;
+}
+
+int
+pastix_amd_get_dims( const pastix_data_t *pd, pastix_int_t *dims )
+{
+    if ( pd == NULL || pd->solvmatr == NULL || dims == NULL ) return PASTIX_ERR_BADPARAMETER;
+    dims[0] = pd->solvmatr->nodenbr;
+    dims[1] = pd->solvmatr->cblknbr;
+    dims[2] = pd->solvmatr->bloknbr;
+    dims[3] = pd->solvmatr->coefnbr;
+    dims[4] = pd->solvmatr->brownbr;
+    dims[5] = pa_dev_nlevels( pd );
+    dims[6] = pd->iparm[IPARM_FACTORIZATION];
+    dims[7] = pd->solvmatr->flttype;
+    return PASTIX_SUCCESS;
+}
+
+int
+pastix_amd_get_structure( const pastix_data_t *pd, pastix_int_t *cblk, pastix_int_t *blok,
+                          pastix_int_t *perm, pastix_int_t *invp )
+{
+    const SolverMatrix *s;
+    pastix_int_t k, b, off = 0;
+    if ( pd == NULL || pd->solvmatr == NULL ) return PASTIX_ERR_BADPARAMETER;
+    s = pd->solvmatr;
+    for ( k = 0; cblk && k < s->cblknbr; k++ ) {
+        const SolverCblk *c = s->cblktab + k;
+        cblk[5 * k + 0] = c->fcolnum;
+        cblk[5 * k + 1] = c->lcolnum;
+        cblk[5 * k + 2] = c->fblokptr - s->bloktab;
+        cblk[5 * k + 3] = c->stride;
+        cblk[5 * k + 4] = off;
+        off += c->stride * cblk_colnbr( c );
+    }
+    for ( b = 0; blok && b < s->bloknbr; b++ ) {
+        const SolverBlok *bl = s->bloktab + b;
+        blok[5 * b + 0] = bl->frownum;
+        blok[5 * b + 1] = bl->lrownum;
+        blok[5 * b + 2] = bl->lcblknm;
+        blok[5 * b + 3] = bl->fcblknm;
+        blok[5 * b + 4] = bl->coefind;
+    }
+    if ( perm ) memcpy( perm, pd->ordemesh->permtab, s->nodenbr * sizeof( pastix_int_t ) );
+    if ( invp ) memcpy( invp, pd->ordemesh->peritab, s->nodenbr * sizeof( pastix_int_t ) );
+    return PASTIX_SUCCESS;
+}
+
+int
+pastix_amd_get_coeftab( const pastix_data_t *pd, int side, double *host )
+{
+    if ( pd == NULL || !( pd->steps & STEP_BCSC2CTAB ) ) return PASTIX_ERR_BADPARAMETER;
+    return pa_dev_get_coeftab( pd, side, host );
+}
+
+pastix_order_t *
+pastixOrderGet( const pastix_data_t *pd )
+{
+    return pd ? pd->ordemesh : NULL;
+}
+
+void
+pastixDumpParam( const pastix_data_t *pd )
+{
+    int i;
+    if ( !pd ) return;
+    for ( i = 0; i < IPARM_SIZE; i++ ) fprintf( stdout, "iparm[%d],%ld\n", i, (long)pd->iparm[i] );
+    for ( i = 0; i < DPARM_SIZE; i++ ) fprintf( stdout, "dparm[%d],%e\n", i, pd->dparm[i] );
+}
+
+/* Legacy one-call interface (reference sopalin/pastix.c:103-330): runs the
+ * steps IPARM_START_TASK..IPARM_END_TASK on a CSC given by arrays. */
+int
+pastix( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t n, pastix_int_t *colptr,
+        pastix_int_t *rowptr, void *values, pastix_int_t *perm, pastix_int_t *invp, void *B,
+        pastix_int_t nrhs, pastix_int_t *iparm, double *dparm )
+{
+    static spmatrix_t spm_store;
+    spmatrix_t *spm = &spm_store;
+    pastix_data_t *pd;
+    int rc = PASTIX_SUCCESS;
+    pastix_int_t task;
+
+    if ( iparm[IPARM_MODIFY_PARAMETER] == 0 ) {
+        pastixInitParam( iparm, dparm );
+        iparm[IPARM_MODIFY_PARAMETER] = 1;
+        return PASTIX_SUCCESS;
+    }
+    if ( iparm[IPARM_START_TASK] == PastixTaskInit || *pastix_data == NULL ) {
+        pastixInit( pastix_data, comm, iparm, dparm );
+    }
+    pd = *pastix_data;
+    memset( spm, 0, sizeof( *spm ) );
+    spm->mtxtype = ( iparm[IPARM_MTX_TYPE] == -1 ) ? SpmGeneral : (spm_mtxtype_t)iparm[IPARM_MTX_TYPE];
+    spm->flttype = ( iparm[IPARM_FLOAT] == -1 ) ? SpmDouble : (spm_coeftype_t)iparm[IPARM_FLOAT];
+    spm->fmttype = SpmCSC;
+    spm->baseval = colptr ? colptr[0] : 0;
+    spm->n = spm->gN = spm->nexp = spm->gNexp = n;
+    spm->nnz = spm->gnnz = colptr ? colptr[n] - colptr[0] : 0;
+    spm->dof     = 1;
+    spm->colptr  = colptr;
+    spm->rowptr  = rowptr;
+    spm->values  = values;
+
+    for ( task = iparm[IPARM_START_TASK]; !rc && task <= iparm[IPARM_END_TASK]; task++ ) {
+        switch ( task ) {
+        case PastixTaskOrdering: {
+            pastix_order_t o;
+            memset( &o, 0, sizeof( o ) );
+            o.vertnbr = n;
+            o.permtab = perm;
+            o.peritab = invp;
+            rc = pastix_subtask_order( pd, spm, ( perm || invp ) ? &o : NULL );
+            if ( !rc && perm ) memcpy( perm, pd->ordemesh->permtab, n * sizeof( pastix_int_t ) );
+            if ( !rc && invp ) memcpy( invp, pd->ordemesh->peritab, n * sizeof( pastix_int_t ) );
+            break;
+        }
+        case PastixTaskSymbfact: rc = pastix_subtask_symbfact( pd ); break;
+        case PastixTaskAnalyze:
+            rc = pastix_subtask_reordering( pd );
+            if ( !rc ) rc = pastix_subtask_blend( pd );
+            break;
+        case PastixTaskNumfact: rc = pastix_task_numfact( pd, spm ); break;
+        case PastixTaskSolve:   rc = pastix_task_solve( pd, n, nrhs, B, n ); break;
+        case PastixTaskRefine:  break; /* refinement needs the original b, as in the reference */
+        case PastixTaskClean:   break;
+        default: break;
+        }
+    }
+    if ( !rc && iparm[IPARM_END_TASK] >= PastixTaskClean ) {
+        pastixFinalize( pastix_data );
+    }
+    return rc;
+}

@@ -1,0 +1,176 @@
+"""GPU parity of the level-batched factorization and solve against the CPU
+oracle on the same supernodal structure (FP64 within 1e-12 relative, FP32
+within a float tolerance), and end-to-end accuracy of solve + refinement.
+"""
+import numpy as np
+import pytest
+
+import pastix_amd as pa
+from pastix_amd import problems
+from oracle import pyoracle as orc
+from helpers import Problem
+
+pytestmark = pytest.mark.gpu
+
+CASES_LLT = [
+    ("lap3d_6", (6, 6, 6), None),
+    ("lap3d_9x7x5_bs", (9, 7, 5), (4, 8)),
+    ("lap3d_20", (20, 20, 20), None),
+    ("lap2d_40x33", (40, 33, 1), (8, 16)),
+]
+
+
+def _relerr(a, b, mask=None):
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name,grid,bs", CASES_LLT, ids=[c[0] for c in CASES_LLT])
+def test_potrf_fp64_matches_oracle(name, grid, bs):
+    print("case", name, grid, bs, flush=True)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT, blocksize=bs)
+    try:
+        Lg, _ = P.gpu_factors()
+        Lo, _, npiv = P.oracle_factors()
+        mask = P.lower_mask()
+        assert _relerr(Lg, Lo, mask) < 1e-12
+        assert P.iparm[pa.iparm.static_pivoting] == npiv == 0
+    finally:
+        P.close()
+
+
+CASES_LU = [
+    ("cd27_5", (5, 5, 5), None),
+    ("cd27_9x8x7_bs", (9, 8, 7), (6, 12)),
+    ("cd27_14", (14, 14, 14), None),
+]
+
+
+@pytest.mark.parametrize("name,grid,bs", CASES_LU, ids=[c[0] for c in CASES_LU])
+def test_getrf_fp64_matches_oracle(name, grid, bs):
+    print("case", name, grid, bs, flush=True)
+    A = problems.convection_diffusion_27(*grid)
+    P = Problem(A, grid, pa.factotype.LU, blocksize=bs)
+    try:
+        Lg, Ug = P.gpu_factors()
+        Lo, Uo, npiv = P.oracle_factors()
+        assert _relerr(Lg, Lo) < 1e-12
+        assert _relerr(Ug, Uo) < 1e-12
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("ft", [pa.factotype.LLT, pa.factotype.LU], ids=["llt", "lu"])
+def test_fp32_factor_matches_oracle(ft):
+    grid = (10, 9, 8)
+    A = problems.laplacian(*grid) if ft == pa.factotype.LLT else problems.convection_diffusion_27(*grid)
+    P = Problem(A, grid, ft, mixed=True)
+    try:
+        Lg, Ug = P.gpu_factors()
+        Lo, Uo, _ = P.oracle_factors(dtype=np.float32)
+        mask = None if P.lu else P.lower_mask()
+        # same algorithm in FP32, different summation order: a few ulps times
+        # the growth of the elimination
+        assert _relerr(Lg, Lo.astype(np.float64), mask) < 2e-5
+        if P.lu:
+            assert _relerr(Ug, Uo.astype(np.float64)) < 2e-5
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("ft", [pa.factotype.LLT, pa.factotype.LU], ids=["llt", "lu"])
+def test_solve_matches_oracle_and_is_accurate(ft):
+    grid = (12, 11, 10)
+    A = problems.laplacian(*grid) if ft == pa.factotype.LLT else problems.convection_diffusion_27(*grid)
+    P = Problem(A, grid, ft)
+    try:
+        n = P.n
+        xt = problems.uniform(n, seed=0)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        Lo, Uo, _ = P.oracle_factors()
+        xo_p = orc.solve(P.S, 2 if P.lu else 0, Lo, Uo, b[P.invp])
+        xo = np.empty(n)
+        xo[P.invp] = xo_p
+        assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
+        assert problems.backward_error(A, x, b) < 1e-14
+    finally:
+        P.close()
+
+
+def test_multi_rhs_solve():
+    grid = (10, 10, 10)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT)
+    try:
+        X = problems.uniform(P.n, seed=1, nrhs=16)
+        B = np.asfortranarray(A @ X)
+        Y = pa.task_solve(P.h, P.spm, B.copy(order="F"))
+        assert np.abs(Y - X).max() < 1e-11
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("refine", [pa.refine.GMRES, pa.refine.CG, pa.refine.SR], ids=["gmres", "cg", "sr"])
+def test_mixed_precision_refinement_reaches_1e12(refine):
+    grid = (16, 16, 16)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT, mixed=True)
+    try:
+        P.iparm[pa.iparm.refinement] = refine
+        xt = problems.uniform(P.n, seed=0)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        berr0 = problems.backward_error(A, x, b)
+        assert berr0 > 1e-10  # FP32 factors alone do not reach 1e-12
+        pa.task_refine(P.h, P.spm, b, x)
+        # stopping rule ||r||_2/||b||_2 <= 1e-12 (reference refinement/z_refine_gmres.c:169)
+        assert P.dparm[pa.dparm.relative_error] <= 1e-12
+        # normwise backward error ||b-Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+        assert problems.backward_error(A, x, b) <= 1e-12
+        assert P.iparm[pa.iparm.nbiter] >= 1
+    finally:
+        P.close()
+
+
+def test_mixed_lu_gmres_convection_diffusion():
+    grid = (14, 13, 12)
+    A = problems.convection_diffusion_27(*grid)
+    P = Problem(A, grid, pa.factotype.LU, mixed=True)
+    try:
+        xt = problems.uniform(P.n, seed=0)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        pa.task_refine(P.h, P.spm, b, x)
+        assert P.dparm[pa.dparm.relative_error] <= 1e-12
+        assert problems.backward_error(A, x, b) <= 1e-12
+    finally:
+        P.close()
+
+
+def test_static_pivoting_counts_replaced_pivots():
+    """The first pivot of the factorization is exactly zero (its row and
+    column are empty): it is replaced by the threshold eps*||A||_F and counted
+    (reference kernels/core_zgetrfsp.c:81-91), on the GPU and in the oracle."""
+    grid = (5, 4, 3)
+    A = problems.convection_diffusion_27(*grid)
+    P0 = Problem(A, grid, pa.factotype.LU, analyze_only=True)
+    v = int(P0.invp[0])
+    P0.close()
+    A = A.tolil()
+    A[v, :] = 0.0
+    A[:, v] = 0.0
+    A = A.tocsc()
+    A.eliminate_zeros()
+    P = Problem(A, grid, pa.factotype.LU)
+    try:
+        Lg, Ug = P.gpu_factors()
+        Lo, Uo, npiv = P.oracle_factors()
+        assert npiv == 1
+        assert P.iparm[pa.iparm.static_pivoting] == npiv
+        assert _relerr(Lg, Lo) < 1e-12
+        assert _relerr(Ug, Uo) < 1e-12
+    finally:
+        P.close()
