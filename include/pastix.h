@@ -96,6 +96,15 @@ int pastix_amd_spmv_dev( pastix_data_t *pastix_data, pastix_trans_t trans, doubl
 double pastix_amd_last_fact_seconds( pastix_data_t *pastix_data );
 
 /*
+ * Kernel profiling of the factorization with HIP events on the library
+ * stream.  enable = 1 starts (and resets) accumulation, 0 stops, -1 only
+ * reads.  out[9] (may be NULL): accumulated milliseconds of the diagonal,
+ * TRSM and scatter-GEMM kernels, their launch counts, and the algorithmic
+ * flops of each class for one factorization (reference flop model).
+ */
+int pastix_amd_profile( pastix_data_t *pastix_data, int enable, double *out );
+
+/*
  * Structure export for tests and tooling.  Fills `dims`:
  *  [0] n, [1] cblknbr, [2] bloknbr, [3] coefnbr (per factor), [4] browtab size,
  *  [5] number of levels of the batched schedule, [6] factotype, [7] flttype

@@ -23,6 +23,11 @@
 extern "C" {
 #include "../host/internal.h"
 #include "pa_device.h"
+double pa_flops_potrf( double n );
+double pa_flops_getrf( double n );
+double pa_flops_trsm( double m, double n );
+double pa_flops_gemm( double m, double n, double k );
+double pa_flops_syrk( double k, double n );
 }
 
 #define PA_HIP( call )                                                                       \
@@ -78,6 +83,12 @@ struct pa_device_s {
     int64_t xtmp_len;
     hipEvent_t ev0, ev1;
     double  last_fact_event_s;
+    /* per kernel class profiling (diag, trsm, gemm) with HIP events */
+    int     profiling;
+    std::vector<hipEvent_t> prof_ev;   /* 6 per level */
+    double  prof_ms[3];
+    int64_t prof_launch[3];
+    double  class_flops[3];            /* algorithmic flops of a full factorization */
 };
 
 static inline pa_device_s *DEV( const pastix_data_t *d ) { return (pa_device_s *)d->dev; }
@@ -190,6 +201,8 @@ pa_dev_setup_structure( pastix_data_t *pd )
     }
     PA_HIP( hipSetDevice( dev->device ) );
     pa_free_structure( dev );
+    pa_free_factors( dev );
+    dev->prec = 0;
 
     std::vector<int32_t> c_fcol( cblknbr + 1 ), c_ncol( cblknbr + 1 ), c_stride( cblknbr + 1 ), c_fblok( cblknbr + 1 );
     std::vector<int64_t> c_coef( cblknbr + 1 );
@@ -267,6 +280,21 @@ pa_dev_setup_structure( pastix_data_t *pd )
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
     dev->levels.resize( nlev );
 
+    dev->class_flops[0] = dev->class_flops[1] = dev->class_flops[2] = 0.;
+    for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
+        /* reference flop model (symbol/symbol_cost_flops.c, common/flops.h) */
+        double N = c_ncol[k], M = c_stride[k] - c_ncol[k];
+        if ( lu ) {
+            dev->class_flops[0] += pa_flops_getrf( N );
+            dev->class_flops[1] += 2. * pa_flops_trsm( M, N );
+            dev->class_flops[2] += pa_flops_gemm( M, M, N );
+        }
+        else {
+            dev->class_flops[0] += pa_flops_potrf( N );
+            dev->class_flops[1] += pa_flops_trsm( M, N );
+            dev->class_flops[2] += pa_flops_syrk( N, M );
+        }
+    }
     for ( pastix_int_t l = 0; l < nlev; l++ ) {
         pa_level_t &L = dev->levels[l];
         L.cblk_beg  = lev_cblks.size();
@@ -370,10 +398,10 @@ static int
 pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
 {
     size_t bytes = (size_t)std::max<int64_t>( dev->coefnbr, 1 ) * sizeof( T );
-    PA_HIP( hipMalloc( &dev->d_L, bytes ) );
+    if ( !dev->d_L ) PA_HIP( hipMalloc( &dev->d_L, bytes ) );
     PA_HIP( hipMemsetAsync( dev->d_L, 0, bytes, dev->stream ) );
     if ( lu ) {
-        PA_HIP( hipMalloc( &dev->d_U, bytes ) );
+        if ( !dev->d_U ) PA_HIP( hipMalloc( &dev->d_U, bytes ) );
         PA_HIP( hipMemsetAsync( dev->d_U, 0, bytes, dev->stream ) );
     }
     PA_HIP( hipMemsetAsync( dev->d_counters, 0, 16 * sizeof( int32_t ), dev->stream ) );
@@ -396,7 +424,9 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
-    pa_free_factors( dev );
+    /* the HBM coeftab is kept across numerical factorizations of the same
+     * structure and arithmetic; it is only reallocated when they change */
+    if ( dev->prec != prec || dev->lu != lu || ( lu && !dev->d_U ) ) pa_free_factors( dev );
     dev->prec = prec;
     dev->lu   = lu;
     if ( prec == PA_PREC_FLOAT ) return pa_coeftab_init_t<float>( dev, lu, overflow );
@@ -406,15 +436,27 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
 /* ------------------------------------------------------------------------- */
 /* factorization                                                             */
 /* ------------------------------------------------------------------------- */
+#define PA_PROF_MARK( slot )                                                                      \
+    do {                                                                                          \
+        if ( dev->profiling ) PA_HIP( hipEventRecord( dev->prof_ev[6 * li + ( slot )], dev->stream ) ); \
+    } while ( 0 )
+
 template <typename T>
 static int
 pa_factorize_t( pa_device_s *dev, double crit )
 {
     T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
     const int lu = dev->lu;
-    for ( const pa_level_t &lev : dev->levels ) {
+    if ( dev->profiling && dev->prof_ev.size() < 6 * dev->levels.size() ) {
+        size_t old = dev->prof_ev.size();
+        dev->prof_ev.resize( 6 * dev->levels.size() );
+        for ( size_t i = old; i < dev->prof_ev.size(); i++ ) PA_HIP( hipEventCreate( &dev->prof_ev[i] ) );
+    }
+    for ( size_t li = 0; li < dev->levels.size(); li++ ) {
+        const pa_level_t &lev = dev->levels[li];
         int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
         if ( ncb == 0 ) continue;
+        PA_PROF_MARK( 0 );
         size_t lds_diag = (size_t)PA_DIAG_NB * lev.ncmax * sizeof( T );
         if ( lu )
             k_diag<T, true><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
@@ -423,7 +465,9 @@ pa_factorize_t( pa_device_s *dev, double crit )
             k_diag<T, false><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
                                                                   crit, dev->d_counters, dev->d_counters + 1 );
         PA_HIP( hipGetLastError() );
+        PA_PROF_MARK( 1 );
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
+        PA_PROF_MARK( 2 );
         if ( ntr > 0 ) {
             size_t lds_trsm = (size_t)PA_TRSM_TR * lev.ncmax * sizeof( T );
             if ( lu ) {
@@ -435,12 +479,36 @@ pa_factorize_t( pa_device_s *dev, double crit )
             }
             PA_HIP( hipGetLastError() );
         }
+        PA_PROF_MARK( 3 );
         int64_t ng = lev.gemm_end - lev.gemm_beg;
+        PA_PROF_MARK( 4 );
         if ( ng > 0 ) {
             k_gemm_level<T><<<(unsigned)ng, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.gemm_beg, L,
                                                                    lu ? U : nullptr );
             PA_HIP( hipGetLastError() );
         }
+        PA_PROF_MARK( 5 );
+    }
+    return PASTIX_SUCCESS;
+}
+
+static inline int64_t lu_launches( const pa_device_s *dev ) { return dev->lu ? 2 : 1; }
+
+/* accumulate the per-class kernel times of the last factorization */
+static int
+pa_collect_profile( pa_device_s *dev )
+{
+    for ( size_t li = 0; li < dev->levels.size(); li++ ) {
+        const pa_level_t &lev = dev->levels[li];
+        if ( lev.cblk_end == lev.cblk_beg ) continue;
+        for ( int c = 0; c < 3; c++ ) {
+            float ms = 0.f;
+            PA_HIP( hipEventElapsedTime( &ms, dev->prof_ev[6 * li + 2 * c], dev->prof_ev[6 * li + 2 * c + 1] ) );
+            dev->prof_ms[c] += ms;
+        }
+        dev->prof_launch[0] += 1;
+        dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lu_launches( dev ) ) : 0;
+        dev->prof_launch[2] += ( lev.gemm_end > lev.gemm_beg ) ? 1 : 0;
     }
     return PASTIX_SUCCESS;
 }
@@ -461,6 +529,9 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     PA_HIP( hipEventRecord( dev->ev1, dev->stream ) );
     PA_HIP( hipStreamSynchronize( dev->stream ) );
     auto t1 = std::chrono::steady_clock::now();
+    if ( dev->profiling ) {
+        if ( ( rc = pa_collect_profile( dev ) ) ) return rc;
+    }
     float ms = 0.f;
     hipEventElapsedTime( &ms, dev->ev0, dev->ev1 );
     dev->last_fact_event_s = ms * 1e-3;
@@ -713,6 +784,26 @@ pa_dev_get_coeftab( const pastix_data_t *pd, int side, double *host )
         std::vector<float> tmp( dev->coefnbr );
         PA_HIP( hipMemcpy( tmp.data(), src, dev->coefnbr * sizeof( float ), hipMemcpyDeviceToHost ) );
         for ( int64_t i = 0; i < dev->coefnbr; i++ ) host[i] = tmp[i];
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* profiling of the factorization kernels by class (diag, trsm, gemm) */
+extern "C" int
+pa_dev_profile( pastix_data_t *pd, int enable, double *out )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( !dev ) return PASTIX_ERR_BADPARAMETER;
+    if ( enable >= 0 ) {
+        dev->profiling = enable;
+        for ( int c = 0; c < 3; c++ ) { dev->prof_ms[c] = 0.; dev->prof_launch[c] = 0; }
+    }
+    if ( out ) {
+        for ( int c = 0; c < 3; c++ ) {
+            out[c]     = dev->prof_ms[c];
+            out[3 + c] = (double)dev->prof_launch[c];
+            out[6 + c] = dev->class_flops[c];
+        }
     }
     return PASTIX_SUCCESS;
 }

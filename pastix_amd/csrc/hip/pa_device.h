@@ -61,6 +61,10 @@ int    pa_dev_sync( pastix_data_t *pastix_data );
 /* copy a factor from HBM to the host as FP64 (side 0: L, 1: U^T) */
 int    pa_dev_get_coeftab( const pastix_data_t *pastix_data, int side, double *host );
 pastix_int_t pa_dev_nlevels( const pastix_data_t *pastix_data );
+/* enable (1) / disable (0) / keep (-1) per-class kernel timing; out[9] gets
+ * the accumulated ms of the diag/trsm/gemm kernels, their launch counts and
+ * the algorithmic flops of each class for one factorization */
+int    pa_dev_profile( pastix_data_t *pastix_data, int enable, double *out );
 double pa_dev_last_fact_event_seconds( const pastix_data_t *pastix_data );
 
 #ifdef __cplusplus

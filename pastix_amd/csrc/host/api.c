@@ -556,6 +556,13 @@ pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t
     }
 }
 
+int
+pastix_amd_profile( pastix_data_t *pd, int enable, double *out )
+{
+    if ( pd == NULL || pd->dev == NULL ) return PASTIX_ERR_BADPARAMETER;
+    return pa_dev_profile( pd, enable, out );
+}
+
 double
 pastix_amd_last_fact_seconds( pastix_data_t *pd )
 {
