@@ -31,11 +31,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_instance(pa, A, grid, mixed):
+def make_instance(pa, A, grid, mixed, eps_refine):
     ip, dp = pa.initParam()
     ip[pa.iparm.factorization] = pa.factotype.LLT
     ip[pa.iparm.verbose] = pa.verbose.Not
     ip[pa.iparm.refinement] = pa.refine.GMRES
+    # GMRES stops on ||r||_2/||b||_2 (reference refinement/z_refine_gmres.c:169);
+    # 1e-13 keeps the normwise inf-norm backward error below 1e-12 at n = 1e6
+    dp[pa.dparm.epsilon_refinement] = eps_refine
     if mixed:
         ip[pa.iparm.mixed] = 1
     h = pa.init(ip, dp)
@@ -50,19 +53,35 @@ def make_instance(pa, A, grid, mixed):
     return dict(h=h, ip=ip, dp=dp, spm=spm, order=o, t_analyze=t_an, t_first_numfact=t_nf)
 
 
-def step(pa, lib, inst, d_b, d_x):
-    """One numerical pass on HBM-resident data; returns refinement iterations."""
+def step(pa, lib, inst, d_b, d_x, stages=None):
+    """One numerical pass on HBM-resident data; returns refinement iterations.
+    With `stages` (a dict), the stages are separated by device syncs and
+    timed: fill, factor, solve, refine (ms)."""
+    import torch
     h = inst["h"]
+
+    def mark(name, t0):
+        if stages is None:
+            return t0
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stages[name] = stages.get(name, 0.0) + (t1 - t0) * 1e3
+        return t1
+    t0 = mark("start", time.perf_counter())
     rc = lib.pastix_subtask_bcsc2ctab(h)
     assert rc == 0, rc
+    t0 = mark("fill", t0)
     rc = lib.pastix_subtask_sopalin(h)
     assert rc == 0, rc
+    t0 = mark("factor", t0)
     d_x.copy_(d_b)
     rc = lib.pastix_amd_solve_dev(h, pa.trans.NoTrans, 1, C.c_void_p(d_x.data_ptr()), d_x.numel())
     assert rc == 0, rc
+    t0 = mark("solve", t0)
     it = lib.pastix_amd_refine_dev(h, 1, C.c_void_p(d_b.data_ptr()), d_b.numel(),
                                    C.c_void_p(d_x.data_ptr()), d_x.numel())
     assert it >= 0, it
+    mark("refine", t0)
     return it
 
 
@@ -85,7 +104,11 @@ def timed(torch, pa, lib, inst, d_b, d_x, steps, warmup, profile):
         lib.pastix_amd_profile(inst["h"], -1, C.c_void_p(out.ctypes.data))
         lib.pastix_amd_profile(inst["h"], 0, None)
         prof = out
-    return t, iters, prof
+    # one more pass, outside the timed steps, with device syncs between stages
+    stages = {}
+    step(pa, lib, inst, d_b, d_x, stages)
+    stages.pop("start", None)
+    return t, iters, prof, stages
 
 
 def cpu_baseline(pa, problems, seconds_target=20.0):
@@ -129,6 +152,7 @@ def main():
     ap.add_argument("--grid", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--eps", type=float, default=1e-13)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -156,13 +180,13 @@ def main():
     results = {}
     for mode in (["fp32ir"] if args.no_fp64 else ["fp32ir", "fp64"]):
         log("mode", mode, ": analysis + first factorization")
-        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"))
+        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps)
         th = float(inst["dp"][pa.dparm.fact_thflops])
-        t, iters, prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup, profile=True)
+        t, iters, prof, stages = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup, profile=True)
         x = d_x.cpu().numpy()
         berr = problems.backward_error(A, x, b)
         ferr = np.abs(x - xt).max() / np.abs(xt).max()
-        results[mode] = dict(t=t, iters=iters, prof=prof, th=th, berr=berr, ferr=ferr,
+        results[mode] = dict(t=t, iters=iters, prof=prof, th=th, berr=berr, ferr=ferr, stages=stages,
                              t_analyze=inst["t_analyze"], fact_s=float(inst["dp"][pa.dparm.fact_time]),
                              refine_res=float(inst["dp"][pa.dparm.relative_error]))
         log("mode %s: %.1f ms/step, berr %.2e, ferr %.2e, iters %s, fact %.1f ms" %
@@ -196,6 +220,7 @@ def main():
                    "thflops": r["th"], "parallelism": "single GPU"},
         "berr": r["berr"], "ferr": r["ferr"], "refine_iters": r["iters"][-1],
         "kernel_ms_per_step": {"diag": prof[0] / steps, "trsm": prof[1] / steps, "gemm": prof[2] / steps},
+        "stage_ms": r["stages"],
         "roofline": roofline,
     }
     if "fp64" in results:
@@ -208,6 +233,7 @@ def main():
         line["fp64_kernel_ms_per_step"] = {"diag": p64[0] / steps, "trsm": p64[1] / steps,
                                            "gemm": p64[2] / steps}
         line["fp64_gemm_tflops"] = p64[8] / (p64[2] / steps * 1e-3) / 1e12
+        line["fp64_stage_ms"] = r64["stages"]
     if not args.no_cpu:
         cb = cpu_baseline(pa, problems)
         line["cpu_baseline"] = cb

@@ -43,7 +43,8 @@ double pa_flops_syrk( double k, double n );
 struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks                      */
-    int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks                      */
+    int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
+    int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
     int64_t solve_beg, solve_end; /* into d_solve_tasks                     */
     int32_t ncmax;                /* widest cblk of the level               */
 };
@@ -65,7 +66,7 @@ struct pa_device_s {
     int32_t      *d_lev_cblks;
     int2         *d_trsm_tasks;
     pa_gemm_task *d_gemm_tasks;
-    int2         *d_solve_tasks;
+    int4         *d_solve_tasks;
     /* matrix */
     int64_t *d_colptr;
     int32_t *d_rowtab;
@@ -103,6 +104,31 @@ pa_upload( T **dptr, const std::vector<T> &h )
     return PASTIX_SUCCESS;
 }
 
+/* allow the kernels that stage a slab in LDS to use up to the device limit */
+template <typename K>
+static void
+pa_lds_attr( K kernel, int bytes )
+{
+    /* the dynamic limit excludes the kernel's static LDS; a refused request
+     * must not leave a sticky error behind for the next runtime call of the
+     * process (the HIP runtime is shared with the caller's framework) */
+    hipFuncAttributes fa;
+    int               stat = 0;
+    if ( hipFuncGetAttributes( &fa, reinterpret_cast<const void *>( kernel ) ) == hipSuccess ) stat = (int)fa.sharedSizeBytes;
+    (void)hipGetLastError();
+    if ( hipFuncSetAttribute( reinterpret_cast<const void *>( kernel ), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              bytes - stat ) != hipSuccess ) {
+        (void)hipGetLastError();
+    }
+}
+
+static void
+pa_set_lds_limits( int bytes )
+{
+    pa_lds_attr( k_trsm<double, 0>, bytes ); pa_lds_attr( k_trsm<double, 1>, bytes ); pa_lds_attr( k_trsm<double, 2>, bytes );
+    pa_lds_attr( k_trsm<float, 0>, bytes );  pa_lds_attr( k_trsm<float, 1>, bytes );  pa_lds_attr( k_trsm<float, 2>, bytes );
+}
+
 extern "C" int
 pa_dev_open( pastix_data_t *pd )
 {
@@ -124,6 +150,7 @@ pa_dev_open( pastix_data_t *pd )
     int lds = 65536;
     hipDeviceGetAttribute( &lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev->device );
     dev->lds_limit = lds;
+    pa_set_lds_limits( lds );
     PA_HIP( hipMalloc( (void **)&dev->d_counters, 16 * sizeof( int32_t ) ) );
     PA_HIP( hipMalloc( (void **)&dev->d_partial, 4096 * sizeof( double ) ) );
     PA_HIP( hipHostMalloc( (void **)&dev->h_partial, 4096 * sizeof( double ) ) );
@@ -274,9 +301,13 @@ pa_dev_setup_structure( pastix_data_t *pd )
     for ( pastix_int_t k = 0; k < cblknbr; k++ ) bylev[level[k]].push_back( (int32_t)k );
 
     const int lu = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    /* 128 x 128 GEMM tiles pay off in FP32; in FP64 their accumulators
+     * (64 doubles per lane) leave one wave per SIMD, so 64 x 64 is kept */
+    const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     std::vector<int32_t>      lev_cblks;
-    std::vector<int2>         trsm_tasks, solve_tasks;
-    std::vector<pa_gemm_task> gemm_tasks;
+    std::vector<int2>         trsm_tasks;
+    std::vector<int4>         solve_tasks;
+    std::vector<pa_gemm_task> gemm_tasks, big_tasks;
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
     dev->levels.resize( nlev );
 
@@ -300,6 +331,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         L.cblk_beg  = lev_cblks.size();
         L.trsm_beg  = trsm_tasks.size();
         L.gemm_beg  = gemm_tasks.size();
+        L.big_beg   = big_tasks.size();
         L.solve_beg = solve_tasks.size();
         L.ncmax     = 0;
         /* number of distinct source cblks of this level per target cblk */
@@ -318,7 +350,8 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
-            for ( int32_t r = 0; r < R; r += PA_SOLVE_ROWS ) solve_tasks.push_back( make_int2( k, r ) );
+            for ( int32_t r = 0; r < R; r += PA_SOLVE_ROWS )
+                for ( int32_t c = 0; c < nc; c += PA_SOLVE_COLS ) solve_tasks.push_back( make_int4( k, r, c, 0 ) );
             /* column groups: consecutive blocks facing the same cblk */
             int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
             while ( b < lb ) {
@@ -327,18 +360,20 @@ pa_dev_setup_structure( pastix_data_t *pd )
                 int32_t g0 = b_rowoff[b] - nc;
                 int32_t g1 = b_rowoff[e] - nc + ( b_lrow[e] - b_frow[e] + 1 );
                 int32_t flags = ( srccount[F] > 1 ) ? PA_TASK_ATOMIC : 0;
+                /* wide updates use 128 x 128 output tiles, the others 64 x 64 */
+                const int32_t TS = ( big_tiles && R - g0 >= 384 ) ? 128 : 64;
+                std::vector<pa_gemm_task> &dst = ( TS == 128 ) ? big_tasks : gemm_tasks;
                 for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
-                    for ( int32_t s = g0; s < g1; s += PA_GT ) {
-                        int32_t slen = std::min( PA_GT, g1 - s );
-                        /* U side needs rows of strictly later blocks only */
-                        int32_t rstart = g0;
-                        if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
-                        for ( int32_t r = rstart; r < R; r += PA_GT ) {
+                    /* U side needs rows of strictly later blocks only */
+                    if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
+                    for ( int32_t s = g0; s < g1; s += TS ) {
+                        int32_t slen = std::min( TS, g1 - s );
+                        for ( int32_t r = g0; r < R; r += TS ) {
                             pa_gemm_task t;
                             t.k = k; t.F = F; t.s0 = s; t.r0 = r; t.slen = slen;
                             t.flags = flags | ( side ? PA_TASK_USIDE : 0 );
                             t.pad0 = t.pad1 = 0;
-                            gemm_tasks.push_back( t );
+                            dst.push_back( t );
                         }
                     }
                 }
@@ -349,16 +384,24 @@ pa_dev_setup_structure( pastix_data_t *pd )
         L.cblk_end  = lev_cblks.size();
         L.trsm_end  = trsm_tasks.size();
         L.gemm_end  = gemm_tasks.size();
+        L.big_end   = big_tasks.size();
         L.solve_end = solve_tasks.size();
     }
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
+    /* both task lists in one array: the 128-tile tasks follow the 64-tile ones */
+    const int64_t nsmall = gemm_tasks.size();
+    for ( pa_level_t &L : dev->levels ) {
+        L.big_beg += nsmall;
+        L.big_end += nsmall;
+    }
+    gemm_tasks.insert( gemm_tasks.end(), big_tasks.begin(), big_tasks.end() );
     if ( ( rc = pa_upload( &dev->d_gemm_tasks, gemm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_solve_tasks, solve_tasks ) ) ) return rc;
 
     /* LDS needs of the diagonal, TRSM and solve kernels */
-    size_t need = (size_t)colmax * std::max( PA_DIAG_NB, PA_TRSM_TR ) * sizeof( double ) + 1024;
+    size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 1024;
     if ( need > (size_t)dev->lds_limit ) {
         fprintf( stderr, "pastix_amd: cblks of %d columns exceed the LDS budget (%d bytes); lower IPARM_MAX_BLOCKSIZE\n",
                  colmax, dev->lds_limit );
@@ -457,19 +500,23 @@ pa_factorize_t( pa_device_s *dev, double crit )
         int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
         if ( ncb == 0 ) continue;
         PA_PROF_MARK( 0 );
-        size_t lds_diag = (size_t)PA_DIAG_NB * lev.ncmax * sizeof( T );
-        if ( lu )
-            k_diag<T, true><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
-                                                                 crit, dev->d_counters, dev->d_counters + 1 );
-        else
-            k_diag<T, false><<<ncb, 256, lds_diag, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, L, U,
-                                                                  crit, dev->d_counters, dev->d_counters + 1 );
+        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax * sizeof( T );
+        const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
+        int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
+        if ( lev.ncmax > 64 ) {
+            if ( lu ) k_diag<T, true, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else      k_diag<T, false, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+        }
+        else {
+            if ( lu ) k_diag<T, true, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else      k_diag<T, false, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+        }
         PA_HIP( hipGetLastError() );
         PA_PROF_MARK( 1 );
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
         if ( ntr > 0 ) {
-            size_t lds_trsm = (size_t)PA_TRSM_TR * lev.ncmax * sizeof( T );
+            size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
             if ( lu ) {
                 k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
                 k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
@@ -480,11 +527,16 @@ pa_factorize_t( pa_device_s *dev, double crit )
             PA_HIP( hipGetLastError() );
         }
         PA_PROF_MARK( 3 );
-        int64_t ng = lev.gemm_end - lev.gemm_beg;
+        int64_t ng = lev.gemm_end - lev.gemm_beg, nbig = lev.big_end - lev.big_beg;
         PA_PROF_MARK( 4 );
+        if ( nbig > 0 ) {
+            k_gemm_level<T, 128><<<(unsigned)nbig, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.big_beg, L,
+                                                                          lu ? U : nullptr );
+            PA_HIP( hipGetLastError() );
+        }
         if ( ng > 0 ) {
-            k_gemm_level<T><<<(unsigned)ng, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.gemm_beg, L,
-                                                                   lu ? U : nullptr );
+            k_gemm_level<T, 64><<<(unsigned)ng, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.gemm_beg, L,
+                                                                       lu ? U : nullptr );
             PA_HIP( hipGetLastError() );
         }
         PA_PROF_MARK( 5 );
@@ -508,7 +560,7 @@ pa_collect_profile( pa_device_s *dev )
         }
         dev->prof_launch[0] += 1;
         dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lu_launches( dev ) ) : 0;
-        dev->prof_launch[2] += ( lev.gemm_end > lev.gemm_beg ) ? 1 : 0;
+        dev->prof_launch[2] += ( lev.gemm_end > lev.gemm_beg ? 1 : 0 ) + ( lev.big_end > lev.big_beg ? 1 : 0 );
     }
     return PASTIX_SUCCESS;
 }

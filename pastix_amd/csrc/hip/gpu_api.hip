@@ -234,7 +234,7 @@ pa_gpublok_trsmsp( pastix_side_t side, pastix_uplo_t uplo, pastix_trans_t trans,
     pa_call_rows R;
     memset( &R, 0, sizeof( R ) );
     int32_t rows = 0;
-    size_t lds = (size_t)N * PA_TRSM_TR * sizeof( T );
+    size_t lds = pa_trsm_lds<T>( N );
     for ( ; blok < lblok && blok->fcblknm == cblk_m; blok++ ) {
         int32_t b = R.nblk;
         R.pref[b] = rows;
@@ -299,9 +299,9 @@ pa_gpucblk_diag( const SolverCblk *cblk, void *L, void *U, double criterion, int
         return -1.;
     }
     int32_t nc = (int32_t)cblk_colnbr( cblk );
-    size_t lds = (size_t)PA_DIAG_NB * nc * sizeof( T );
+    size_t lds = (size_t)pa_diag_nb<T>::value * nc * sizeof( T );
     /* negative diagonal terms are counted in the second word of the counter */
-    k_diag_one<T, LU><<<1, 256, lds, stream>>>( (T *)L, (T *)U, nc, criterion, d_npiv, d_npiv + 1 );
+    k_diag_one<T, LU><<<1, 512, lds, stream>>>( (T *)L, (T *)U, nc, criterion, d_npiv, d_npiv + 1 );
     if ( hipGetLastError() != hipSuccess ) return -1.;
     return LU ? pa_fl_getrf( nc ) : pa_fl_potrf( nc );
 }

@@ -119,7 +119,7 @@ k_fill( pa_dstruct S, int32_t n, const int64_t *colptr, const int32_t *rowtab,
 }
 
 /* ------------------------------------------------------------------------- */
-/* Diagonal block factorization, one workgroup per cblk of the level.        */
+/* Diagonal block factorization, one workgroup (4 waves) per cblk.            */
 /*                                                                           */
 /* POTRF: Cholesky with static pivoting: a pivot below `crit` in magnitude   */
 /* is replaced by `crit` and counted (reference kernels/core_zpotrfsp.c:     */
@@ -128,110 +128,216 @@ k_fill( pa_dstruct S, int32_t n, const int64_t *colptr, const int32_t *rowtab,
 /* diagonal block of U^T is first merged into L and the factored block is    */
 /* transposed back into U^T (core_zgeadd / core_zgetmo, :253-264).           */
 /*                                                                           */
-/* Right-looking over column panels of NB columns held in LDS; the trailing  */
-/* update of the (L2-resident) diagonal tile is done by all 256 threads.     */
+/* Right-looking over panels of NB columns (16 in FP64, 32 in FP32) held in  */
+/* LDS: the NB x NB top block is factored by one wave in registers with      */
+/* cross-lane shuffles (no workgroup barrier per column), the rows below are */
+/* solved by all threads, and the trailing update of the (L2-resident)       */
+/* diagonal tile runs on MFMA 16x16x4 tiles.                                 */
 /* ------------------------------------------------------------------------- */
-#define PA_DIAG_NB 16
+template <typename T> struct pa_diag_nb;
+template <> struct pa_diag_nb<double> { static constexpr int value = 16; };
+template <> struct pa_diag_nb<float>  { static constexpr int value = 16; };
+#define PA_DIAG_NB_MAX 32
 
-template <typename T, bool LU>
+template <typename T> struct pa_mfma;
+/*
+ * Operand layout of the 16x16x4 MFMAs (both precisions): lane l supplies
+ * A[l % 16][l / 16] and B[l / 16][l % 16].  Output element r of lane l is
+ * D[row][l % 16] with row = 4 * (l / 16) + r for f32 but row = l / 16 + 4 * r
+ * for f64 (measured on gfx950).
+ */
+template <> struct pa_mfma<double> {
+    typedef pa_d4 acc_t;
+    static __device__ __forceinline__ acc_t run( double a, double b, acc_t c ) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64( a, b, c, 0, 0, 0 );
+    }
+    static __device__ __forceinline__ int row( int lane, int r ) { return ( lane >> 4 ) + 4 * r; }
+};
+template <> struct pa_mfma<float> {
+    typedef pa_f4 acc_t;
+    static __device__ __forceinline__ acc_t run( float a, float b, acc_t c ) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32( a, b, c, 0, 0, 0 );
+    }
+    static __device__ __forceinline__ int row( int lane, int r ) { return 4 * ( lane >> 4 ) + r; }
+};
+
+/* TH threads per workgroup: 1024 for wide diagonal blocks, 64 for the
+ * small blocks of the lower levels */
+template <typename T, bool LU, int TH>
 __device__ void
 pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg, T *P )
 {
-    int     t  = threadIdx.x;
-    int32_t mypiv = 0, myneg = 0;
-    const T tcrit = (T)crit;
+    constexpr int NB = pa_diag_nb<T>::value;
+    typedef typename pa_mfma<T>::acc_t acc_t;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int32_t   mypiv = 0, myneg = 0;
+    const T   tcrit = (T)crit;
 
     if ( LU ) {
         /* L += U^T on the full square (strict lower of U^T holds the original
          * strict upper of A, never updated; L holds the rest) */
-        for ( int32_t idx = t; idx < nc * nc; idx += 256 ) {
+        for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
             int32_t i = idx % nc, j = idx / nc;
             if ( i < j ) D[i + (int64_t)j * nc] += Du[j + (int64_t)i * nc];
         }
         __syncthreads();
     }
-    for ( int32_t p = 0; p < nc; p += PA_DIAG_NB ) {
-        int32_t m = nc - p;
-        int32_t w = m < PA_DIAG_NB ? m : PA_DIAG_NB;
 
-        for ( int32_t idx = t; idx < w * m; idx += 256 ) {
-            int32_t j = idx / m, i = idx % m;
-            P[j * m + i] = D[( p + i ) + (int64_t)( p + j ) * nc];
+    for ( int32_t p = 0; p < nc; p += NB ) {
+        const int32_t m = nc - p;
+        const int32_t w = m < NB ? m : NB;
+        const int32_t q = m - w;
+
+        for ( int32_t i = t; i < m; i += TH ) {
+            T v[NB];
+#pragma unroll
+            for ( int j = 0; j < NB; j++ ) v[j] = ( j < w ) ? D[( p + i ) + (int64_t)( p + j ) * nc] : T( 0 );
+#pragma unroll
+            for ( int j = 0; j < NB; j++ )
+                if ( j < w ) P[j * m + i] = v[j];
         }
         __syncthreads();
 
-        for ( int32_t j = 0; j < w; j++ ) {
-            if ( t == 0 ) {
-                T d = P[j * m + j];
-                if ( pa_abs( d ) < tcrit ) {
-                    d = ( LU && d < T( 0 ) ) ? -tcrit : tcrit;
-                    mypiv++;
-                }
-                if ( !LU ) {
-                    if ( d < T( 0 ) ) myneg++;
-                    d = sqrt( d );
-                }
-                P[j * m + j] = d;
-            }
-            __syncthreads();
-            T d = P[j * m + j];
-            for ( int32_t i = j + 1 + t; i < m; i += 256 ) P[j * m + i] /= d;
-            __syncthreads();
-            int32_t nl = w - j - 1;
-            if ( nl > 0 ) {
-                int32_t ni = m - j - 1;
-                for ( int32_t idx = t; idx < nl * ni; idx += 256 ) {
-                    int32_t l = j + 1 + idx / ni, i = j + 1 + idx % ni;
-                    if ( LU ) {
-                        P[l * m + i] -= P[j * m + i] * P[l * m + j];
+        /* (A) w x w block by wave 0, lane i holding row i */
+        if ( wave == 0 ) {
+            T a[NB];
+            const int i = lane;
+#pragma unroll
+            for ( int j = 0; j < NB; j++ ) a[j] = ( i < w && j < w ) ? P[j * m + i] : T( 0 );
+#pragma unroll
+            for ( int k = 0; k < NB; k++ ) {
+                if ( k < w ) {
+                    T    d   = __shfl( a[k], k );
+                    bool rep = pa_abs( d ) < tcrit;
+                    if ( rep ) d = ( LU && d < T( 0 ) ) ? -tcrit : tcrit;
+                    if ( lane == 0 && rep ) mypiv++;
+                    if ( !LU ) {
+                        if ( lane == 0 && d < T( 0 ) ) myneg++;
+                        d = sqrt( d );
                     }
-                    else if ( i >= l ) {
-                        P[l * m + i] -= P[j * m + i] * P[j * m + l];
+                    if ( i == k ) a[k] = d;
+                    else if ( i > k ) a[k] = a[k] / d;
+#pragma unroll
+                    for ( int j = k + 1; j < NB; j++ ) {
+                        if ( j < w ) {
+                            T f = LU ? __shfl( a[j], k ) : __shfl( a[k], j );
+                            if ( i > k && ( LU || i >= j ) ) a[j] -= a[k] * f;
+                        }
                     }
                 }
             }
-            __syncthreads();
+            if ( i < w ) {
+#pragma unroll
+                for ( int j = 0; j < NB; j++ )
+                    if ( j < w ) P[j * m + i] = a[j];
+            }
         }
+        __syncthreads();
 
-        for ( int32_t idx = t; idx < w * m; idx += 256 ) {
-            int32_t j = idx / m, i = idx % m;
-            D[( p + i ) + (int64_t)( p + j ) * nc] = P[j * m + i];
-        }
-        int32_t q = m - w; /* trailing size */
-        if ( q > 0 ) {
-            if ( LU ) {
-                /* U12 = L11^{-1} A12, unit lower, one column per thread */
-                for ( int32_t c = p + w + t; c < nc; c += 256 ) {
-                    T u[PA_DIAG_NB];
-                    for ( int32_t jj = 0; jj < w; jj++ ) {
-                        T x = D[( p + jj ) + (int64_t)c * nc];
-                        for ( int32_t ii = 0; ii < jj; ii++ ) x -= P[ii * m + jj] * u[ii];
-                        u[jj] = x;
-                    }
-                    for ( int32_t jj = 0; jj < w; jj++ ) D[( p + jj ) + (int64_t)c * nc] = u[jj];
+        /* (B) rows below: X L11^T = A21 (POTRF) or X U11 = A21 (LU) */
+        for ( int32_t r = w + t; r < m; r += TH ) {
+            T x[NB];
+#pragma unroll
+            for ( int j = 0; j < NB; j++ ) x[j] = ( j < w ) ? P[j * m + r] : T( 0 );
+#pragma unroll
+            for ( int j = 0; j < NB; j++ ) {
+                if ( j < w ) {
+                    T s = x[j];
+#pragma unroll
+                    for ( int l = 0; l < j; l++ ) s -= x[l] * ( LU ? P[j * m + l] : P[l * m + j] );
+                    x[j] = s / P[j * m + j];
                 }
             }
-            __syncthreads();
-            /* A22 -= L21 * (L21^T or U12) */
-            for ( int32_t idx = t; idx < q * q; idx += 256 ) {
-                int32_t i = idx % q, c = idx / q;
-                if ( !LU && i < c ) continue;
-                T acc = T( 0 );
-                if ( LU ) {
-                    for ( int32_t j = 0; j < w; j++ )
-                        acc += P[j * m + w + i] * D[( p + j ) + (int64_t)( p + w + c ) * nc];
+#pragma unroll
+            for ( int j = 0; j < NB; j++ )
+                if ( j < w ) P[j * m + r] = x[j];
+        }
+        __syncthreads();
+
+        for ( int32_t i = t; i < m; i += TH ) {
+#pragma unroll
+            for ( int j = 0; j < NB; j++ )
+                if ( j < w ) D[( p + i ) + (int64_t)( p + j ) * nc] = P[j * m + i];
+        }
+        if ( LU && q > 0 ) {
+            /* (C) U12 = L11^{-1} A12, unit lower, one column per thread */
+            for ( int32_t c = p + w + t; c < nc; c += TH ) {
+                T u[NB];
+#pragma unroll
+                for ( int j = 0; j < NB; j++ ) u[j] = ( j < w ) ? D[( p + j ) + (int64_t)c * nc] : T( 0 );
+#pragma unroll
+                for ( int j = 0; j < NB; j++ ) {
+#pragma unroll
+                    for ( int l = 0; l < j; l++ ) u[j] -= P[l * m + j] * u[l];
                 }
-                else {
-                    for ( int32_t j = 0; j < w; j++ ) acc += P[j * m + w + i] * P[j * m + w + c];
+#pragma unroll
+                for ( int j = 0; j < NB; j++ )
+                    if ( j < w ) D[( p + j ) + (int64_t)c * nc] = u[j];
+            }
+        }
+        __syncthreads();
+
+        /* (D) trailing update on 16x16 MFMA tiles, 4 tiles in flight per
+         * wave.  The product is formed as (col side) x (row side)^T so that the
+         * 16 lanes of an output row hold 16 consecutive rows of one column. */
+        if ( q > 0 ) {
+            const int32_t nt = ( q + 15 ) >> 4;
+            const int32_t ntiles = LU ? nt * nt : ( nt * ( nt + 1 ) ) / 2;
+            for ( int32_t tile0 = wave * 4; tile0 < ntiles; tile0 += ( TH / 64 ) * 4 ) {
+                acc_t acc[4];
+                T     cold[4][4];
+                int32_t row[4], col[4], tcol[4];
+#pragma unroll
+                for ( int u = 0; u < 4; u++ ) {
+                    const int32_t tile = tile0 + u;
+                    int32_t ti = 0, tc = 0;
+                    if ( tile < ntiles ) {
+                        if ( LU ) {
+                            ti = tile % nt;
+                            tc = tile / nt;
+                        }
+                        else {
+                            ti = (int32_t)( ( sqrtf( 8.f * (float)tile + 1.f ) - 1.f ) * 0.5f );
+                            while ( ( ti * ( ti + 1 ) ) / 2 > tile ) ti--;
+                            while ( ( ( ti + 1 ) * ( ti + 2 ) ) / 2 <= tile ) ti++;
+                            tc = tile - ( ti * ( ti + 1 ) ) / 2;
+                        }
+                    }
+                    row[u]  = ( tile < ntiles ) ? ti * 16 + ( lane & 15 ) : q; /* row side index */
+                    col[u]  = tc * 16 + ( lane & 15 );                          /* col side index */
+                    tcol[u] = tc * 16;
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ ) {
+                        int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
+                        cold[u][r] = ( row[u] < q && c < q ) ? D[( p + w + row[u] ) + (int64_t)( p + w + c ) * nc] : T( 0 );
+                        acc[u][r]  = T( 0 );
+                    }
                 }
-                D[( p + w + i ) + (int64_t)( p + w + c ) * nc] -= acc;
+                for ( int32_t k0 = 0; k0 < w; k0 += 4 ) {
+                    const int32_t kk = k0 + ( lane >> 4 );
+#pragma unroll
+                    for ( int u = 0; u < 4; u++ ) {
+                        T ra = ( row[u] < q && kk < w ) ? P[kk * m + w + row[u]] : T( 0 );
+                        T ca;
+                        if ( LU ) ca = ( col[u] < q && kk < w ) ? D[( p + kk ) + (int64_t)( p + w + col[u] ) * nc] : T( 0 );
+                        else      ca = ( col[u] < q && kk < w ) ? P[kk * m + w + col[u]] : T( 0 );
+                        acc[u] = pa_mfma<T>::run( ca, ra, acc[u] );
+                    }
+                }
+#pragma unroll
+                for ( int u = 0; u < 4; u++ )
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ ) {
+                        int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
+                        if ( row[u] < q && c < q ) D[( p + w + row[u] ) + (int64_t)( p + w + c ) * nc] = cold[u][r] - acc[u][r];
+                    }
             }
         }
         __syncthreads();
     }
 
     if ( LU ) {
-        for ( int32_t idx = t; idx < nc * nc; idx += 256 ) {
+        for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
             int32_t i = idx % nc, j = idx / nc;
             Du[j + (int64_t)i * nc] = D[i + (int64_t)j * nc];
         }
@@ -243,24 +349,24 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
 }
 
 /* one workgroup per cblk of a level */
-template <typename T, bool LU>
-__global__ __launch_bounds__( 256 ) void
+template <typename T, bool LU, int TH>
+__global__ __launch_bounds__( TH ) void
 k_diag( pa_dstruct S, const int32_t *cblks, T *Lc, T *Uc, double crit, int32_t *npiv,
         int32_t *nneg )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     int32_t k = cblks[blockIdx.x];
-    pa_diag_factor<T, LU>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], crit, npiv, nneg,
-                           reinterpret_cast<T *>( pa_smem_raw ) );
+    pa_diag_factor<T, LU, TH>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], crit, npiv, nneg,
+                               reinterpret_cast<T *>( pa_smem_raw ) );
 }
 
 /* single diagonal block given by pointers (reference-signature entry points) */
 template <typename T, bool LU>
-__global__ __launch_bounds__( 256 ) void
+__global__ __launch_bounds__( 512 ) void
 k_diag_one( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 {
     extern __shared__ unsigned char pa_smem_raw[];
-    pa_diag_factor<T, LU>( D, Du, nc, crit, npiv, nneg, reinterpret_cast<T *>( pa_smem_raw ) );
+    pa_diag_factor<T, LU, 512>( D, Du, nc, crit, npiv, nneg, reinterpret_cast<T *>( pa_smem_raw ) );
 }
 
 /* ------------------------------------------------------------------------- */
@@ -272,78 +378,129 @@ k_diag_one( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 /* A task is (cblk, first below-diagonal row); TR rows per workgroup are     */
 /* staged in LDS and solved with a column-blocked algorithm.                 */
 /* ------------------------------------------------------------------------- */
-#define PA_TRSM_TR 16
-#define PA_TRSM_CB 16
+/* rows per workgroup (the slab lives in LDS) */
+template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = 32; };
+#define PA_TRSM_TR 32
 
-/* Solve `rows` (<= TR) rows whose offsets/leading dimensions inside `panel`
- * are xoff/xld (in LDS); D is the nc x nc diagonal tile. */
+/*
+ * Solve `rows` (<= SR) rows whose offsets/leading dimensions inside `panel`
+ * are xoff/xld (in LDS); D is the nc x nc diagonal tile.  The slab X
+ * (SR x nc) is staged in LDS; columns are processed in blocks of 16:
+ *   X[:, jb:jb+16] -= X[:, 0:jb] C[0:jb, jb:jb+16]   (MFMA, K split among
+ *                                                     waves when SR < 64)
+ *   X[:, jb:jb+16] <- X[:, jb:jb+16] C[jb:jb+16, jb:jb+16]^{-1} (one row per thread)
+ * where C(l, j) = L[j][l] (MODE 0) or D[l][j] (MODE 1, 2); the coefficients of
+ * the block column are staged in LDS first.  LDS: SR*nc + 16*nc + 4*SR*16 T.
+ */
 template <typename T, int MODE>
 __device__ void
-pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xoff, const int32_t *xld, T *Xs )
+pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xoff, const int32_t *xld, T *smem )
 {
-    int t = threadIdx.x;
-    for ( int32_t idx = t; idx < nc * PA_TRSM_TR; idx += 256 ) {
-        int32_t r = idx % PA_TRSM_TR, j = idx / PA_TRSM_TR;
-        Xs[j * PA_TRSM_TR + r] = ( r < rows ) ? panel[xoff[r] + (int64_t)j * xld[r]] : T( 0 );
-    }
-    __syncthreads();
+    typedef typename pa_mfma<T>::acc_t acc_t;
+    constexpr int SR  = pa_trsm_rows_per_wg<T>::value;
+    constexpr int NRT = SR / 16;   /* row tiles                  */
+    constexpr int KS  = 4 / NRT;   /* K split among the waves    */
+    T *Xs  = smem;                 /* [nc][SR]                   */
+    T *Ls  = Xs + (int64_t)nc * SR; /* [16][nc]: Ls[jj*nc + l] = C(l, jb+jj) */
+    T *red = Ls + 16 * nc;         /* [KS][16][SR] partial sums  */
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 
-    for ( int32_t jb = 0; jb < nc; jb += PA_TRSM_CB ) {
-        int32_t cw = nc - jb < PA_TRSM_CB ? nc - jb : PA_TRSM_CB;
-        {
-            int32_t r = t % PA_TRSM_TR, jj = t / PA_TRSM_TR;
-            if ( jj < cw && jb > 0 ) {
-                int32_t j = jb + jj;
-                T acc = T( 0 );
-                if ( MODE == 0 ) {
-                    for ( int32_t l = 0; l < jb; l++ ) acc += Xs[l * PA_TRSM_TR + r] * D[j + (int64_t)l * nc];
-                }
-                else {
-                    const T *col = D + (int64_t)j * nc;
-                    for ( int32_t l = 0; l < jb; l++ ) acc += Xs[l * PA_TRSM_TR + r] * col[l];
-                }
-                Xs[j * PA_TRSM_TR + r] -= acc;
+    for ( int32_t idx = t; idx < nc * SR; idx += 256 ) {
+        int32_t r = idx % SR, j = idx / SR;
+        Xs[j * SR + r] = ( r < rows ) ? panel[xoff[r] + (int64_t)j * xld[r]] : T( 0 );
+    }
+
+    for ( int32_t jb = 0; jb < nc; jb += 16 ) {
+        const int32_t cw = nc - jb < 16 ? nc - jb : 16;
+        const int32_t kend = jb + cw; /* coefficients needed: l < jb + cw */
+        /* stage C(l, jb+jj) for l < kend */
+        for ( int32_t idx = t; idx < 16 * kend; idx += 256 ) {
+            int32_t jj, l;
+            if ( MODE == 0 ) { jj = idx % 16; l = idx / 16; }  /* L[jb+jj][l]: contiguous in jj */
+            else             { l = idx % kend; jj = idx / kend; } /* D[l][jb+jj]: contiguous in l  */
+            Ls[jj * nc + l] = ( jj < cw ) ? ( MODE == 0 ? D[( jb + jj ) + (int64_t)l * nc] : D[l + (int64_t)( jb + jj ) * nc] )
+                                          : T( 0 );
+        }
+        __syncthreads();
+        if ( jb > 0 ) {
+            /* partial products on MFMA: wave -> (row tile, K part) */
+            const int rt = wave % NRT, ks = wave / NRT;
+            const int32_t kchunk = ( ( jb / KS + 3 ) / 4 ) * 4;
+            const int32_t kb = ks * kchunk, ke = min( jb, kb + kchunk );
+            acc_t acc;
+            for ( int r = 0; r < 4; r++ ) acc[r] = T( 0 );
+            for ( int32_t k0 = kb; k0 < ke; k0 += 4 ) {
+                const int32_t kk = k0 + ( lane >> 4 );
+                T ca = ( kk < ke ) ? Ls[( lane & 15 ) * nc + kk] : T( 0 );         /* column side (j) */
+                T ra = ( kk < ke ) ? Xs[kk * SR + rt * 16 + ( lane & 15 )] : T( 0 ); /* row side (r)    */
+                acc = pa_mfma<T>::run( ca, ra, acc );
+            }
+            /* lane holds (row rt*16 + lane%16, column j = row(lane, r)) */
+            for ( int r = 0; r < 4; r++ ) {
+                int j = pa_mfma<T>::row( lane, r );
+                red[( ks * 16 + j ) * SR + rt * 16 + ( lane & 15 )] = acc[r];
             }
         }
         __syncthreads();
-        if ( t < rows ) {
-            int32_t r = t;
-            for ( int32_t jj = 0; jj < cw; jj++ ) {
-                int32_t j = jb + jj;
-                T x = Xs[j * PA_TRSM_TR + r];
-                for ( int32_t l = jb; l < j; l++ ) {
-                    T cf = ( MODE == 0 ) ? D[j + (int64_t)l * nc] : D[l + (int64_t)j * nc];
-                    x -= Xs[l * PA_TRSM_TR + r] * cf;
+        if ( t < SR ) {
+            const int32_t r = t;
+            T x[16];
+#pragma unroll
+            for ( int jj = 0; jj < 16; jj++ ) {
+                T v = ( jj < cw ) ? Xs[( jb + jj ) * SR + r] : T( 0 );
+                if ( jb > 0 ) {
+#pragma unroll
+                    for ( int ks = 0; ks < KS; ks++ ) v -= red[( ks * 16 + jj ) * SR + r];
                 }
-                if ( MODE != 2 ) x /= D[j + (int64_t)j * nc];
-                Xs[j * PA_TRSM_TR + r] = x;
+                x[jj] = v;
+            }
+#pragma unroll
+            for ( int jj = 0; jj < 16; jj++ ) {
+                if ( jj < cw ) {
+                    T v = x[jj];
+#pragma unroll
+                    for ( int l = 0; l < jj; l++ ) v -= x[l] * Ls[jj * nc + jb + l];
+                    if ( MODE != 2 ) v /= Ls[jj * nc + jb + jj];
+                    x[jj] = v;
+                    Xs[( jb + jj ) * SR + r] = v;
+                }
             }
         }
         __syncthreads();
     }
-    for ( int32_t idx = t; idx < nc * PA_TRSM_TR; idx += 256 ) {
-        int32_t r = idx % PA_TRSM_TR, j = idx / PA_TRSM_TR;
-        if ( r < rows ) panel[xoff[r] + (int64_t)j * xld[r]] = Xs[j * PA_TRSM_TR + r];
+    for ( int32_t idx = t; idx < nc * SR; idx += 256 ) {
+        int32_t r = idx % SR, j = idx / SR;
+        if ( r < rows ) panel[xoff[r] + (int64_t)j * xld[r]] = Xs[j * SR + r];
     }
+}
+
+/* LDS bytes of pa_trsm_rows for nc columns */
+template <typename T>
+__host__ __device__ constexpr size_t
+pa_trsm_lds( int32_t nc )
+{
+    return ( (size_t)nc * pa_trsm_rows_per_wg<T>::value + 16 * (size_t)nc +
+             4 * 16 * (size_t)pa_trsm_rows_per_wg<T>::value ) * sizeof( T );
 }
 
 template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
 k_trsm( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
 {
+    constexpr int SR = pa_trsm_rows_per_wg<T>::value;
     extern __shared__ unsigned char pa_smem_raw[];
-    __shared__ int64_t xoff[PA_TRSM_TR];
-    __shared__ int32_t xld[PA_TRSM_TR];
+    __shared__ int64_t xoff[SR];
+    __shared__ int32_t xld[SR];
 
     int2    task = tasks[blockIdx.x];
     int32_t k = task.x, r0 = task.y;
     int32_t nc = S.c_ncol[k];
     int32_t R  = S.c_stride[k] - nc;
-    int32_t rows = R - r0 < PA_TRSM_TR ? R - r0 : PA_TRSM_TR;
+    int32_t rows = R - r0 < SR ? R - r0 : SR;
     T      *panel = ( MODE == 2 ? Uc : Lc ) + S.c_coef[k];
     int t = threadIdx.x;
 
-    if ( t < PA_TRSM_TR ) {
+    if ( t < SR ) {
         if ( t < rows ) {
             int32_t x = nc + r0 + t;
             int32_t b = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, x );
@@ -386,19 +543,20 @@ pa_call_find( const pa_call_rows &R, int32_t v )
     return lo;
 }
 
-/* TRSM over the rows of a run of blocks given by value; one workgroup per TR rows. */
+/* TRSM over the rows of a run of blocks given by value; one workgroup per SR rows. */
 template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
 k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 {
+    constexpr int SR = pa_trsm_rows_per_wg<T>::value;
     extern __shared__ unsigned char pa_smem_raw[];
-    __shared__ int64_t xoff[PA_TRSM_TR];
-    __shared__ int32_t xld[PA_TRSM_TR];
+    __shared__ int64_t xoff[SR];
+    __shared__ int32_t xld[SR];
     int32_t total = R.pref[R.nblk];
-    int32_t r0 = blockIdx.x * PA_TRSM_TR;
-    int32_t rows = total - r0 < PA_TRSM_TR ? total - r0 : PA_TRSM_TR;
+    int32_t r0 = blockIdx.x * SR;
+    int32_t rows = total - r0 < SR ? total - r0 : SR;
     int t = threadIdx.x;
-    if ( t < PA_TRSM_TR ) {
+    if ( t < SR ) {
         if ( t < rows ) {
             int32_t b = pa_call_find( R, r0 + t );
             xoff[t] = R.off[b] + ( r0 + t - R.pref[b] );
@@ -431,89 +589,99 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 #define PA_GKC 16
 
 template <typename T> struct pa_mfma;
-/*
- * Operand layout of the 16x16x4 MFMAs (both precisions): lane l supplies
- * A[l % 16][l / 16] and B[l / 16][l % 16].  Output element r of lane l is
- * D[row][l % 16] with row = 4 * (l / 16) + r for f32 but row = l / 16 + 4 * r
- * for f64 (measured on gfx950).
- */
-template <> struct pa_mfma<double> {
-    typedef pa_d4 acc_t;
-    static __device__ __forceinline__ acc_t run( double a, double b, acc_t c ) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64( a, b, c, 0, 0, 0 );
-    }
-    static __device__ __forceinline__ int row( int lane, int r ) { return ( lane >> 4 ) + 4 * r; }
-};
-template <> struct pa_mfma<float> {
-    typedef pa_f4 acc_t;
-    static __device__ __forceinline__ acc_t run( float a, float b, acc_t c ) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32( a, b, c, 0, 0, 0 );
-    }
-    static __device__ __forceinline__ int row( int lane, int r ) { return 4 * ( lane >> 4 ) + r; }
-};
+
 
 /*
  * Tile core shared by the level-batched kernel and the reference-signature
- * entry points.  Row maps (in LDS) must be filled by the caller:
- *   a_off/a_ld: source of the TM rho rows inside Asrc (a_ld = 0: empty row)
- *   b_off/b_ld: source of the TN sigma rows inside Bsrc
+ * entry points.  TILE x TILE output (TILE = 64 or 128), 4 waves in a 2 x 2
+ * arrangement, K in chunks of PA_GKC staged in LDS, the next chunk prefetched
+ * into registers while the current one feeds the MFMAs.  Row maps (in LDS)
+ * must be filled by the caller:
+ *   a_off/a_ld: source of the TILE rho rows inside Asrc (a_ld = 0: empty row)
+ *   b_off/b_ld: source of the TILE sigma rows inside Bsrc
  *   c_off/c_ld: target row base and leading dimension for each rho row
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T>
+template <typename T, int TILE>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[PA_GT], T ( *Bs )[PA_GT] )
+              T ( *As )[TILE], T ( *Bs )[TILE] )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
+    constexpr int NB  = TILE / 32;        /* MFMA blocks per wave per dimension */
+    constexpr int KL  = 256 / TILE;       /* k lanes of the loader              */
+    constexpr int NQ  = PA_GKC / KL;      /* loads per operand per chunk        */
     const int t    = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
     const int wm   = wave & 1;  /* rho half   */
     const int wn   = wave >> 1; /* sigma half */
-    acc_t acc[2][2];
-    for ( int i = 0; i < 2; i++ )
-        for ( int j = 0; j < 2; j++ )
+    acc_t acc[NB][NB];
+#pragma unroll
+    for ( int i = 0; i < NB; i++ )
+#pragma unroll
+        for ( int j = 0; j < NB; j++ )
+#pragma unroll
             for ( int r = 0; r < 4; r++ ) acc[i][j][r] = T( 0 );
 
-    const int lrow = t & 63, lk = t >> 6;
+    const int lrow = t % TILE, lk = t / TILE;
     const int64_t aof = a_off[lrow], bof = b_off[lrow];
     const int32_t ald = a_ld[lrow], bld = b_ld[lrow];
+    T pa[NQ], pb[NQ];
 
+#pragma unroll
+    for ( int q = 0; q < NQ; q++ ) {
+        int32_t kk = lk + KL * q;
+        pa[q] = ( ald > 0 && kk < K ) ? Asrc[aof + (int64_t)kk * ald] : T( 0 );
+        pb[q] = ( bld > 0 && kk < K ) ? Bsrc[bof + (int64_t)kk * bld] : T( 0 );
+    }
     for ( int32_t k0 = 0; k0 < K; k0 += PA_GKC ) {
 #pragma unroll
-        for ( int q = 0; q < PA_GKC / 4; q++ ) {
-            int32_t kc = lk + 4 * q;
-            int32_t kk = k0 + kc;
-            As[kc][lrow] = ( ald > 0 && kk < K ) ? Asrc[aof + (int64_t)kk * ald] : T( 0 );
-            Bs[kc][lrow] = ( bld > 0 && kk < K ) ? Bsrc[bof + (int64_t)kk * bld] : T( 0 );
+        for ( int q = 0; q < NQ; q++ ) {
+            As[lk + KL * q][lrow] = pa[q];
+            Bs[lk + KL * q][lrow] = pb[q];
         }
         __syncthreads();
+        if ( k0 + PA_GKC < K ) {
+#pragma unroll
+            for ( int q = 0; q < NQ; q++ ) {
+                int32_t kk = k0 + PA_GKC + lk + KL * q;
+                pa[q] = ( ald > 0 && kk < K ) ? Asrc[aof + (int64_t)kk * ald] : T( 0 );
+                pb[q] = ( bld > 0 && kk < K ) ? Bsrc[bof + (int64_t)kk * bld] : T( 0 );
+            }
+        }
 #pragma unroll
         for ( int ks = 0; ks < PA_GKC / 4; ks++ ) {
             int kk = ks * 4 + ( lane >> 4 );
-            T xa[2], yb[2];
-            for ( int mi = 0; mi < 2; mi++ ) xa[mi] = Bs[kk][wn * 32 + mi * 16 + ( lane & 15 )];
-            for ( int mj = 0; mj < 2; mj++ ) yb[mj] = As[kk][wm * 32 + mj * 16 + ( lane & 15 )];
-            for ( int mi = 0; mi < 2; mi++ )
-                for ( int mj = 0; mj < 2; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
+            T xa[NB], yb[NB];
+#pragma unroll
+            for ( int mi = 0; mi < NB; mi++ ) xa[mi] = Bs[kk][wn * ( TILE / 2 ) + mi * 16 + ( lane & 15 )];
+#pragma unroll
+            for ( int mj = 0; mj < NB; mj++ ) yb[mj] = As[kk][wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 )];
+#pragma unroll
+            for ( int mi = 0; mi < NB; mi++ )
+#pragma unroll
+                for ( int mj = 0; mj < NB; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
         }
         __syncthreads();
     }
 
-    for ( int mi = 0; mi < 2; mi++ ) {
-        for ( int mj = 0; mj < 2; mj++ ) {
-            int rl = wm * 32 + mj * 16 + ( lane & 15 );
-            if ( a_ld[rl] == 0 ) continue;
-            int64_t cbase = c_off[rl];
-            int32_t cld   = c_ld[rl];
-            int32_t ablk  = a_blk[rl];
+#pragma unroll
+    for ( int mj = 0; mj < NB; mj++ ) {
+        int rl = wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 );
+        if ( a_ld[rl] == 0 ) continue;
+        int64_t cbase = c_off[rl];
+        int32_t cld   = c_ld[rl];
+        int32_t ablk  = a_blk[rl];
+#pragma unroll
+        for ( int mi = 0; mi < NB; mi++ ) {
+#pragma unroll
             for ( int r = 0; r < 4; r++ ) {
-                int sl = wn * 32 + mi * 16 + pa_mfma<T>::row( lane, r );
+                int sl = wn * ( TILE / 2 ) + mi * 16 + pa_mfma<T>::row( lane, r );
                 if ( b_ld[sl] == 0 || ablk < b_blk[sl] + shift ) continue;
                 T *dst = C + cbase + (int64_t)c_col[sl] * cld;
                 T  v   = acc[mi][mj][r];
@@ -524,15 +692,15 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     }
 }
 
-/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task. */
-template <typename T>
+/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task of TILE rows. */
+template <typename T, int TILE>
 __global__ __launch_bounds__( 256 ) void
 k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
 {
-    __shared__ T As[PA_GKC][PA_GT];
-    __shared__ T Bs[PA_GKC][PA_GT];
-    __shared__ int64_t a_off[PA_GT], b_off[PA_GT], c_off[PA_GT];
-    __shared__ int32_t a_ld[PA_GT], b_ld[PA_GT], c_ld[PA_GT], c_col[PA_GT], a_blk[PA_GT], b_blk[PA_GT];
+    __shared__ T As[PA_GKC][TILE];
+    __shared__ T Bs[PA_GKC][TILE];
+    __shared__ int64_t a_off[TILE], b_off[TILE], c_off[TILE];
+    __shared__ int32_t a_ld[TILE], b_ld[TILE], c_ld[TILE], c_col[TILE], a_blk[TILE], b_blk[TILE];
 
     const pa_gemm_task tk = tasks[blockIdx.x];
     const int32_t k = tk.k, F = tk.F, nc = S.c_ncol[k];
@@ -541,45 +709,48 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
     const bool uside = ( tk.flags & PA_TASK_USIDE ) != 0;
     const int t = threadIdx.x;
 
-    if ( t < PA_GT ) {
-        /* rho row */
-        int32_t v = tk.r0 + t;
-        if ( v < R ) {
-            int32_t x = nc + v;
-            int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
-            int32_t loc  = x - S.b_rowoff[b];
-            int32_t grow = S.b_frow[b] + loc;
-            a_off[t] = (int64_t)S.b_rowoff[b] * nc + loc;
-            a_ld[t]  = S.b_lrow[b] - S.b_frow[b] + 1;
-            a_blk[t] = b;
-            int32_t ncF = S.c_ncol[F];
-            int32_t f   = pa_find_globrow( S.b_frow, S.c_fblok[F], S.c_fblok[F + 1] - 1, grow );
-            c_off[t] = S.c_coef[F] + (int64_t)S.b_rowoff[f] * ncF + ( grow - S.b_frow[f] );
-            c_ld[t]  = S.b_lrow[f] - S.b_frow[f] + 1;
-            /* a row outside the target structure is never written */
-            if ( grow < S.b_frow[f] || grow > S.b_lrow[f] ) a_ld[t] = 0;
+    for ( int u = t; u < 2 * TILE; u += 256 ) {
+        if ( u < TILE ) {
+            /* rho row */
+            const int32_t tt = u;
+            int32_t v = tk.r0 + tt;
+            if ( v < R ) {
+                int32_t x = nc + v;
+                int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
+                int32_t loc  = x - S.b_rowoff[b];
+                int32_t grow = S.b_frow[b] + loc;
+                a_off[tt] = (int64_t)S.b_rowoff[b] * nc + loc;
+                a_ld[tt]  = S.b_lrow[b] - S.b_frow[b] + 1;
+                a_blk[tt] = b;
+                int32_t ncF = S.c_ncol[F];
+                int32_t f   = pa_find_globrow( S.b_frow, S.c_fblok[F], S.c_fblok[F + 1] - 1, grow );
+                c_off[tt] = S.c_coef[F] + (int64_t)S.b_rowoff[f] * ncF + ( grow - S.b_frow[f] );
+                c_ld[tt]  = S.b_lrow[f] - S.b_frow[f] + 1;
+                /* a row outside the target structure is never written */
+                if ( grow < S.b_frow[f] || grow > S.b_lrow[f] ) a_ld[tt] = 0;
+            }
+            else {
+                a_off[tt] = 0; a_ld[tt] = 0; a_blk[tt] = 0; c_off[tt] = 0; c_ld[tt] = 0;
+            }
         }
         else {
-            a_off[t] = 0; a_ld[t] = 0; a_blk[t] = 0; c_off[t] = 0; c_ld[t] = 0;
-        }
-    }
-    else if ( t < 2 * PA_GT ) {
-        int32_t s = t - PA_GT;
-        int32_t v = tk.s0 + s;
-        if ( s < tk.slen && v < R ) {
-            int32_t x = nc + v;
-            int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
-            int32_t loc  = x - S.b_rowoff[b];
-            int32_t grow = S.b_frow[b] + loc;
-            b_off[s] = (int64_t)S.b_rowoff[b] * nc + loc;
-            b_ld[s]  = S.b_lrow[b] - S.b_frow[b] + 1;
-            b_blk[s] = b;
-            c_col[s] = grow - S.c_fcol[F];
-            /* a column outside the target cblk is never written */
-            if ( c_col[s] < 0 || c_col[s] >= S.c_ncol[F] ) b_ld[s] = 0;
-        }
-        else {
-            b_off[s] = 0; b_ld[s] = 0; b_blk[s] = 0; c_col[s] = 0;
+            const int32_t ss = u - TILE;
+            int32_t v = tk.s0 + ss;
+            if ( ss < tk.slen && v < R ) {
+                int32_t x = nc + v;
+                int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
+                int32_t loc  = x - S.b_rowoff[b];
+                int32_t grow = S.b_frow[b] + loc;
+                b_off[ss] = (int64_t)S.b_rowoff[b] * nc + loc;
+                b_ld[ss]  = S.b_lrow[b] - S.b_frow[b] + 1;
+                b_blk[ss] = b;
+                c_col[ss] = grow - S.c_fcol[F];
+                /* a column outside the target cblk is never written */
+                if ( c_col[ss] < 0 || c_col[ss] >= S.c_ncol[F] ) b_ld[ss] = 0;
+            }
+            else {
+                b_off[ss] = 0; b_ld[ss] = 0; b_blk[ss] = 0; c_col[ss] = 0;
+            }
         }
     }
     __syncthreads();
@@ -588,8 +759,8 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
-                     uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
+    pa_gemm_tile<T, TILE>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+                           uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
 }
 
 /*
@@ -636,7 +807,7 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
         b_blk[s] = 0;
     }
     __syncthreads();
-    pa_gemm_tile<T>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
+    pa_gemm_tile<T, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
 }
 
 /* ------------------------------------------------------------------------- */
@@ -644,55 +815,99 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
 /* x is an n x nrhs column-major array in the factorization numbering.       */
 /* ------------------------------------------------------------------------- */
 
-/* x_k <- D_kk^{-1} x_k (lower, forward) or D_kk^{-T} x_k (backward), one     */
-/* workgroup per cblk of the level; x_k staged in LDS.                       */
+/* x_k <- D_kk^{-1} x_k (lower, forward) or D_kk^{-T} x_k (backward), one
+ * workgroup per cblk of the level, x_k staged in LDS.  Blocked by 16
+ * columns: the 16 x 16 triangle is loaded into registers (one row, or one
+ * column for the transposed sweep, per lane) and solved by wave 0 with
+ * shuffles; the rest of the block column is applied by all threads. */
 template <typename T, bool BACKWARD, bool UNIT>
 __global__ __launch_bounds__( 256 ) void
 k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nrhs][nc] */
-    int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    __shared__ T part[16][17];
+    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
     const T *D = Coef + S.c_coef[k];
-    int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for ( int32_t idx = t; idx < nc * nrhs; idx += 256 ) {
         int32_t i = idx % nc, r = idx / nc;
         xs[r * nc + i] = x[fcol + i + r * ldx];
     }
     __syncthreads();
-    if ( !BACKWARD ) {
-        for ( int32_t j = 0; j < nc; j++ ) {
-            if ( !UNIT ) {
-                for ( int32_t r = t; r < nrhs; r += 256 ) xs[r * nc + j] /= D[j + (int64_t)j * nc];
-                __syncthreads();
-            }
-            int32_t cnt = ( nc - j - 1 ) * nrhs;
-            for ( int32_t idx = t; idx < cnt; idx += 256 ) {
-                int32_t i = j + 1 + idx % ( nc - j - 1 ), r = idx / ( nc - j - 1 );
-                xs[r * nc + i] -= D[i + (int64_t)j * nc] * xs[r * nc + j];
-            }
-            __syncthreads();
-        }
-    }
-    else {
-        for ( int32_t j = nc - 1; j >= 0; j-- ) {
-            /* x_j -= sum_{i>j} D[i][j] x_i, then divide */
+    const int32_t nblk = ( nc + 15 ) / 16;
+    for ( int32_t bi = 0; bi < nblk; bi++ ) {
+        const int32_t jb = BACKWARD ? ( nblk - 1 - bi ) * 16 : bi * 16;
+        const int32_t w  = nc - jb < 16 ? nc - jb : 16;
+        const int32_t rest0 = jb + w, nrest = nc - rest0;
+        if ( BACKWARD && nrest > 0 ) {
+            /* x[jb..jb+w) -= D[rest, jb..jb+w)^T x[rest] */
             for ( int32_t r = 0; r < nrhs; r++ ) {
-                T part = T( 0 );
-                for ( int32_t i = j + 1 + t; i < nc; i += 256 ) part += D[i + (int64_t)j * nc] * xs[r * nc + i];
-                /* block reduction */
-                for ( int off = 32; off > 0; off >>= 1 ) part += __shfl_down( part, off, 64 );
-                __shared__ T red[4];
-                if ( ( t & 63 ) == 0 ) red[t >> 6] = part;
+                const int jj = t & 15, g = t >> 4; /* 16 columns x 16 row groups */
+                T acc = T( 0 );
+                if ( jj < w ) {
+                    const T *dc = D + (int64_t)( jb + jj ) * nc;
+#pragma unroll 4
+                    for ( int32_t i = rest0 + g; i < nc; i += 16 ) acc += dc[i] * xs[r * nc + i];
+                }
+                part[g][jj] = acc;
                 __syncthreads();
-                if ( t == 0 ) {
-                    T s = red[0] + red[1] + red[2] + red[3];
-                    T v = xs[r * nc + j] - s;
-                    if ( !UNIT ) v /= D[j + (int64_t)j * nc];
-                    xs[r * nc + j] = v;
+                if ( t < w ) {
+                    T s2 = T( 0 );
+#pragma unroll
+                    for ( int gg = 0; gg < 16; gg++ ) s2 += part[gg][t];
+                    xs[r * nc + jb + t] -= s2;
                 }
                 __syncthreads();
             }
+        }
+        if ( wave == 0 ) {
+            const int i = lane & 15;
+            /* forward: lane i holds row i of the triangle; backward: column i */
+            T tri[16];
+#pragma unroll
+            for ( int j = 0; j < 16; j++ ) {
+                bool in = ( i < w ) && ( j < w );
+                tri[j] = in ? ( BACKWARD ? D[( jb + j ) + (int64_t)( jb + i ) * nc] : D[( jb + i ) + (int64_t)( jb + j ) * nc] )
+                            : T( 1 );
+            }
+            for ( int32_t r0 = 0; r0 < nrhs; r0 += 4 ) {
+                const int32_t r = r0 + ( lane >> 4 );
+                const bool act = ( r < nrhs ) && ( i < w );
+                T xi = act ? xs[r * nc + jb + i] : T( 0 );
+                if ( !BACKWARD ) {
+#pragma unroll
+                    for ( int j = 0; j < 16; j++ ) {
+                        T xj = __shfl( xi, ( lane & 48 ) + j );
+                        if ( !UNIT ) xj /= __shfl( tri[j], ( lane & 48 ) + j );
+                        if ( i == j ) xi = xj;
+                        else if ( i > j ) xi -= tri[j] * xj;
+                    }
+                }
+                else {
+#pragma unroll
+                    for ( int j = 15; j >= 0; j-- ) {
+                        T xj = __shfl( xi, ( lane & 48 ) + j );
+                        if ( !UNIT ) xj /= __shfl( tri[j], ( lane & 48 ) + j );
+                        if ( i == j ) xi = xj;
+                        else if ( i < j ) xi -= tri[j] * xj;
+                    }
+                }
+                if ( act ) xs[r * nc + jb + i] = xi;
+            }
+        }
+        __syncthreads();
+        if ( !BACKWARD && nrest > 0 ) {
+            /* x[rest] -= D[rest, jb..jb+w) x[jb..jb+w) */
+            for ( int32_t idx = t; idx < nrest * nrhs; idx += 256 ) {
+                int32_t i = rest0 + idx % nrest, r = idx / nrest;
+                T acc = T( 0 );
+#pragma unroll
+                for ( int32_t j = 0; j < 16; j++ )
+                    if ( j < w ) acc += D[i + (int64_t)( jb + j ) * nc] * xs[r * nc + jb + j];
+                xs[r * nc + i] -= acc;
+            }
+            __syncthreads();
         }
     }
     for ( int32_t idx = t; idx < nc * nrhs; idx += 256 ) {
@@ -701,68 +916,81 @@ k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t n
     }
 }
 
-/* Forward update: for the below-diagonal rows of a cblk, x[row] -= A_row x_k.
- * Task = (cblk, first below-diagonal row); one thread per row. */
-#define PA_SOLVE_ROWS 256
+/* Off-diagonal part of the solve, tiled over (cblk, 64 rows, 64 columns).
+ * Forward:  x[row] -= A[row, cols] x_k[cols]        (scatter, atomics)
+ * Backward: x_k[cols] -= A[rows, cols]^T x[rows]    (gather, atomics)   */
+#define PA_SOLVE_ROWS 64
+#define PA_SOLVE_COLS 64
 template <typename T>
 __global__ __launch_bounds__( 256 ) void
-k_solve_fwd_update( pa_dstruct S, const int2 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
+k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
-    int2    tk = tasks[blockIdx.x];
-    int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
-    int32_t R = S.c_stride[k] - nc;
-    int32_t v = tk.y + threadIdx.x;
-    if ( v >= R ) return;
-    int32_t xr = nc + v;
-    int32_t b  = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
-    int32_t loc = xr - S.b_rowoff[b];
-    int32_t M  = S.b_lrow[b] - S.b_frow[b] + 1;
-    int32_t grow = S.b_frow[b] + loc;
-    const T *A = Coef + S.c_coef[k] + (int64_t)S.b_rowoff[b] * nc + loc;
+    __shared__ T red[4][PA_SOLVE_ROWS];
+    const int4    tk = tasks[blockIdx.x];
+    const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t R = S.c_stride[k] - nc;
+    const int t = threadIdx.x, rl = t & 63, g = t >> 6;
+    const int32_t v = tk.y + rl;
+    const bool    valid = v < R;
+    int32_t grow = 0, M = 1;
+    const T *A = Coef;
+    if ( valid ) {
+        int32_t xr  = nc + v;
+        int32_t b   = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
+        int32_t loc = xr - S.b_rowoff[b];
+        M    = S.b_lrow[b] - S.b_frow[b] + 1;
+        grow = S.b_frow[b] + loc;
+        A    = Coef + S.c_coef[k] + (int64_t)S.b_rowoff[b] * nc + loc;
+    }
+    const int32_t c0 = tk.z + g * 16;
+    const int32_t c1 = min( c0 + 16, min( tk.z + PA_SOLVE_COLS, nc ) );
     for ( int32_t r = 0; r < nrhs; r++ ) {
         T acc = T( 0 );
-        const T *xk = x + fcol + r * ldx;
-        for ( int32_t c = 0; c < nc; c++ ) acc += A[(int64_t)c * M] * xk[c];
-        pa_atomic_add( x + grow + r * ldx, -acc );
+        if ( valid ) {
+            const T *xk = x + fcol + r * ldx;
+#pragma unroll
+            for ( int32_t cc = 0; cc < 16; cc++ )
+                if ( c0 + cc < c1 ) acc += A[(int64_t)( c0 + cc ) * M] * xk[c0 + cc];
+        }
+        red[g][rl] = acc;
+        __syncthreads();
+        if ( g == 0 && valid ) pa_atomic_add( x + grow + r * ldx, -( red[0][rl] + red[1][rl] + red[2][rl] + red[3][rl] ) );
+        __syncthreads();
     }
 }
 
-/* Backward update: x_k[c] -= sum_rows A[row][c] x[row], partial sums over a
- * chunk of rows per workgroup, reduced in LDS then added atomically. */
 template <typename T>
 __global__ __launch_bounds__( 256 ) void
-k_solve_bwd_update( pa_dstruct S, const int2 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
+k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
-    __shared__ int64_t roff[PA_SOLVE_ROWS];
-    __shared__ int32_t rld[PA_SOLVE_ROWS];
-    __shared__ T xv[PA_SOLVE_ROWS];
-    __shared__ T red[4];
-    int2    tk = tasks[blockIdx.x];
-    int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
-    int32_t R = S.c_stride[k] - nc;
-    int32_t rows = R - tk.y < PA_SOLVE_ROWS ? R - tk.y : PA_SOLVE_ROWS;
-    int t = threadIdx.x;
-    int32_t grow = 0;
-    if ( t < rows ) {
-        int32_t xr = nc + tk.y + t;
-        int32_t b  = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
+    const int4    tk = tasks[blockIdx.x];
+    const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t R = S.c_stride[k] - nc;
+    const int t = threadIdx.x, rl = t & 63, g = t >> 6;
+    const int32_t v = tk.y + rl;
+    const bool    valid = v < R;
+    int32_t grow = 0, M = 1;
+    const T *A = Coef;
+    if ( valid ) {
+        int32_t xr  = nc + v;
+        int32_t b   = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
         int32_t loc = xr - S.b_rowoff[b];
-        roff[t] = (int64_t)S.b_rowoff[b] * nc + loc;
-        rld[t]  = S.b_lrow[b] - S.b_frow[b] + 1;
-        grow    = S.b_frow[b] + loc;
+        M    = S.b_lrow[b] - S.b_frow[b] + 1;
+        grow = S.b_frow[b] + loc;
+        A    = Coef + S.c_coef[k] + (int64_t)S.b_rowoff[b] * nc + loc;
     }
-    const T *A = Coef + S.c_coef[k];
+    const int32_t c0 = tk.z + g * 16;
+    const int32_t c1 = min( c0 + 16, min( tk.z + PA_SOLVE_COLS, nc ) );
+    T a[16];
+#pragma unroll
+    for ( int cc = 0; cc < 16; cc++ ) a[cc] = ( valid && c0 + cc < c1 ) ? A[(int64_t)( c0 + cc ) * M] : T( 0 );
     for ( int32_t r = 0; r < nrhs; r++ ) {
-        __syncthreads();
-        xv[t] = ( t < rows ) ? x[grow + r * ldx] : T( 0 );
-        __syncthreads();
-        for ( int32_t c = 0; c < nc; c++ ) {
-            T part = ( t < rows ) ? A[roff[t] + (int64_t)c * rld[t]] * xv[t] : T( 0 );
+        const T xv = valid ? x[grow + r * ldx] : T( 0 );
+#pragma unroll
+        for ( int cc = 0; cc < 16; cc++ ) {
+            T part = a[cc] * xv;
             for ( int off = 32; off > 0; off >>= 1 ) part += __shfl_down( part, off, 64 );
-            if ( ( t & 63 ) == 0 ) red[t >> 6] = part;
-            __syncthreads();
-            if ( t == 0 ) pa_atomic_add( x + fcol + c + r * ldx, -( red[0] + red[1] + red[2] + red[3] ) );
-            __syncthreads();
+            if ( rl == 0 && c0 + cc < c1 ) pa_atomic_add( x + fcol + c0 + cc + r * ldx, -part );
         }
     }
 }
