@@ -45,8 +45,10 @@ struct pa_level_t {
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks                      */
     int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
     int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
-    int64_t solve_beg, solve_end; /* into d_solve_tasks                     */
+    int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
+    int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
     int32_t ncmax;                /* widest cblk of the level               */
+    double  flops[3];             /* diag/trsm/gemm flops (reference model)  */
 };
 
 struct pa_device_s {
@@ -58,6 +60,9 @@ struct pa_device_s {
     int64_t *d_c_coef;
     int32_t *d_b_frow, *d_b_lrow, *d_b_fcblk, *d_b_rowoff;
     int32_t *d_col2cblk, *d_perm, *d_invp;
+    int64_t *d_c_dtoff, *d_r_base;
+    int4    *d_r_info;
+    int64_t  dtnbr;      /* entries of the transposed diagonal blocks   */
     pa_dstruct S;
     int64_t  n, coefnbr;
     int32_t  colmax;
@@ -66,7 +71,7 @@ struct pa_device_s {
     int32_t      *d_lev_cblks;
     int2         *d_trsm_tasks;
     pa_gemm_task *d_gemm_tasks;
-    int4         *d_solve_tasks;
+    int4         *d_solve_tasks, *d_bsolve_tasks;
     /* matrix */
     int64_t *d_colptr;
     int32_t *d_rowtab;
@@ -74,6 +79,7 @@ struct pa_device_s {
     /* factors */
     int   prec, lu;
     void *d_L, *d_U;
+    void *d_DT;          /* LL^t: transposed diagonal blocks (solve)     */
     int32_t *d_counters;
     /* work */
     double *d_partial;
@@ -122,11 +128,27 @@ pa_lds_attr( K kernel, int bytes )
     }
 }
 
+template <typename T, bool UNIT>
+static void
+pa_lds_attr_sd( int bytes )
+{
+    pa_lds_attr( k_solve_diag_fwd<T, UNIT, 64>, bytes );  pa_lds_attr( k_solve_diag_bwd<T, UNIT, 64>, bytes );
+    pa_lds_attr( k_solve_diag_fwd<T, UNIT, 128>, bytes ); pa_lds_attr( k_solve_diag_bwd<T, UNIT, 128>, bytes );
+    pa_lds_attr( k_solve_diag_fwd<T, UNIT, 256>, bytes ); pa_lds_attr( k_solve_diag_bwd<T, UNIT, 256>, bytes );
+    pa_lds_attr( k_solve_diag_fwd<T, UNIT, 512>, bytes ); pa_lds_attr( k_solve_diag_bwd<T, UNIT, 512>, bytes );
+}
+
 static void
 pa_set_lds_limits( int bytes )
 {
     pa_lds_attr( k_trsm<double, 0>, bytes ); pa_lds_attr( k_trsm<double, 1>, bytes ); pa_lds_attr( k_trsm<double, 2>, bytes );
     pa_lds_attr( k_trsm<float, 0>, bytes );  pa_lds_attr( k_trsm<float, 1>, bytes );  pa_lds_attr( k_trsm<float, 2>, bytes );
+    pa_lds_attr_sd<double, false>( bytes ); pa_lds_attr_sd<double, true>( bytes );
+    pa_lds_attr_sd<float, false>( bytes );  pa_lds_attr_sd<float, true>( bytes );
+    pa_lds_attr( k_solve_diag<double, false, false>, bytes ); pa_lds_attr( k_solve_diag<double, false, true>, bytes );
+    pa_lds_attr( k_solve_diag<double, true, false>, bytes );  pa_lds_attr( k_solve_diag<double, true, true>, bytes );
+    pa_lds_attr( k_solve_diag<float, false, false>, bytes );  pa_lds_attr( k_solve_diag<float, false, true>, bytes );
+    pa_lds_attr( k_solve_diag<float, true, false>, bytes );   pa_lds_attr( k_solve_diag<float, true, true>, bytes );
 }
 
 extern "C" int
@@ -168,8 +190,8 @@ pa_free_matrix( pa_device_s *dev )
 static void
 pa_free_factors( pa_device_s *dev )
 {
-    hipFree( dev->d_L ); hipFree( dev->d_U );
-    dev->d_L = dev->d_U = NULL;
+    hipFree( dev->d_L ); hipFree( dev->d_U ); hipFree( dev->d_DT );
+    dev->d_L = dev->d_U = dev->d_DT = NULL;
 }
 
 static void
@@ -178,13 +200,16 @@ pa_free_structure( pa_device_s *dev )
     void *ptrs[] = { dev->d_c_fcol, dev->d_c_ncol, dev->d_c_stride, dev->d_c_fblok, dev->d_c_coef,
                      dev->d_b_frow, dev->d_b_lrow, dev->d_b_fcblk, dev->d_b_rowoff, dev->d_col2cblk,
                      dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks,
-                     dev->d_solve_tasks };
+                     dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info };
     for ( void *p : ptrs ) hipFree( p );
     dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
     dev->d_c_coef = NULL;
     dev->d_b_frow = dev->d_b_lrow = dev->d_b_fcblk = dev->d_b_rowoff = NULL;
     dev->d_col2cblk = dev->d_perm = dev->d_invp = NULL;
     dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
+    dev->d_bsolve_tasks = NULL;
+    dev->d_c_dtoff = dev->d_r_base = NULL;
+    dev->d_r_info = NULL;
     dev->levels.clear();
 }
 
@@ -235,7 +260,9 @@ pa_dev_setup_structure( pastix_data_t *pd )
     std::vector<int64_t> c_coef( cblknbr + 1 );
     std::vector<int32_t> b_frow( bloknbr + 1 ), b_lrow( bloknbr + 1 ), b_fcblk( bloknbr + 1 ), b_rowoff( bloknbr + 1 );
     std::vector<int32_t> col2cblk( n + 1 ), perm( n + 1 ), invp( n + 1 );
-    int64_t coef = 0;
+    std::vector<int64_t> c_dtoff( cblknbr + 1 ), r_base( cblknbr + 1 );
+    std::vector<int4>    r_info;
+    int64_t coef = 0, dtoff = 0, rbase = 0;
     int32_t colmax = 0;
     for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
         const SolverCblk *c = solv->cblktab + k;
@@ -244,7 +271,10 @@ pa_dev_setup_structure( pastix_data_t *pd )
         c_stride[k] = (int32_t)c->stride;
         c_fblok[k]  = (int32_t)( c->fblokptr - solv->bloktab );
         c_coef[k]   = coef;
-        coef += (int64_t)c->stride * c_ncol[k];
+        c_dtoff[k]  = dtoff;
+        r_base[k]   = rbase;
+        dtoff += (int64_t)c_ncol[k] * c_ncol[k];
+        rbase += c->stride - c_ncol[k];
         colmax = std::max( colmax, c_ncol[k] );
         for ( pastix_int_t j = c->fcolnum; j <= c->lcolnum; j++ ) col2cblk[j] = (int32_t)k;
         int32_t rowoff = 0;
@@ -254,18 +284,31 @@ pa_dev_setup_structure( pastix_data_t *pd )
             b_lrow[bi]   = (int32_t)b->lrownum;
             b_fcblk[bi]  = (int32_t)b->fcblknm;
             b_rowoff[bi] = rowoff;
+            if ( b > c->fblokptr ) {
+                /* descriptors of the off-diagonal rows, in panel order */
+                const int32_t M = (int32_t)blok_rownbr( b );
+                for ( int32_t loc = 0; loc < M; loc++ ) {
+                    int64_t off = coef + (int64_t)rowoff * c_ncol[k] + loc;
+                    r_info.push_back( make_int4( (int32_t)b->frownum + loc, M, (int32_t)( off & 0xffffffffLL ),
+                                                 (int32_t)( off >> 32 ) ) );
+                }
+            }
             rowoff += (int32_t)blok_rownbr( b );
         }
+        coef += (int64_t)c->stride * c_ncol[k];
     }
     c_fcol[cblknbr]  = (int32_t)n;
     c_fblok[cblknbr] = (int32_t)bloknbr;
     c_coef[cblknbr]  = coef;
+    c_dtoff[cblknbr] = dtoff;
+    r_base[cblknbr]  = rbase;
     for ( pastix_int_t i = 0; i < n; i++ ) {
         perm[i] = (int32_t)pd->ordemesh->permtab[i];
         invp[i] = (int32_t)pd->ordemesh->peritab[i];
     }
     dev->n       = n;
     dev->coefnbr = coef;
+    dev->dtnbr   = dtoff;
     dev->colmax  = colmax;
 
     if ( ( rc = pa_upload( &dev->d_c_fcol, c_fcol ) ) ) return rc;
@@ -280,6 +323,9 @@ pa_dev_setup_structure( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_col2cblk, col2cblk ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_perm, perm ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_invp, invp ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_dtoff, c_dtoff ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_r_base, r_base ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_r_info, r_info ) ) ) return rc;
 
     pa_dstruct &S = dev->S;
     S.cblknbr  = (int32_t)cblknbr;
@@ -293,6 +339,9 @@ pa_dev_setup_structure( pastix_data_t *pd )
     S.b_lrow   = dev->d_b_lrow;
     S.b_fcblk  = dev->d_b_fcblk;
     S.b_rowoff = dev->d_b_rowoff;
+    S.c_dtoff  = dev->d_c_dtoff;
+    S.r_base   = dev->d_r_base;
+    S.r_info   = dev->d_r_info;
 
     /* ---- level schedule ---- */
     std::vector<pastix_int_t> level( cblknbr + 1 );
@@ -306,7 +355,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     std::vector<int32_t>      lev_cblks;
     std::vector<int2>         trsm_tasks;
-    std::vector<int4>         solve_tasks;
+    std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gemm_tasks, big_tasks;
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
     dev->levels.resize( nlev );
@@ -333,12 +382,20 @@ pa_dev_setup_structure( pastix_data_t *pd )
         L.gemm_beg  = gemm_tasks.size();
         L.big_beg   = big_tasks.size();
         L.solve_beg = solve_tasks.size();
+        L.bsolve_beg = bsolve_tasks.size();
         L.ncmax     = 0;
+        L.flops[0] = L.flops[1] = L.flops[2] = 0.;
         /* number of distinct source cblks of this level per target cblk */
         std::vector<int32_t> touched;
         for ( int32_t k : bylev[l] ) {
             lev_cblks.push_back( k );
             L.ncmax = std::max( L.ncmax, c_ncol[k] );
+            {
+                double N = c_ncol[k], M = c_stride[k] - c_ncol[k];
+                L.flops[0] += lu ? pa_flops_getrf( N ) : pa_flops_potrf( N );
+                L.flops[1] += lu ? 2. * pa_flops_trsm( M, N ) : pa_flops_trsm( M, N );
+                L.flops[2] += lu ? pa_flops_gemm( M, M, N ) : pa_flops_syrk( N, M );
+            }
             for ( int32_t b = c_fblok[k] + 1; b < c_fblok[k + 1]; b++ ) {
                 int32_t F = b_fcblk[b];
                 if ( lastsrc[F] != k ) {
@@ -350,8 +407,13 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
-            for ( int32_t r = 0; r < R; r += PA_SOLVE_ROWS )
-                for ( int32_t c = 0; c < nc; c += PA_SOLVE_COLS ) solve_tasks.push_back( make_int4( k, r, c, 0 ) );
+            /* solve update tasks: forward 256 rows x 64 columns; backward
+             * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
+            for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
+                for ( int32_t c = 0; c < nc; c += PA_SF_COLS ) solve_tasks.push_back( make_int4( k, r, c, 0 ) );
+            const int32_t ncg = PA_SB_NCG( nc );
+            for ( int32_t r = 0; r < R; r += 1024 / ncg )
+                for ( int32_t c = 0; c < nc; c += 16 * ncg ) bsolve_tasks.push_back( make_int4( k, r, c, 0 ) );
             /* column groups: consecutive blocks facing the same cblk */
             int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
             while ( b < lb ) {
@@ -386,6 +448,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         L.gemm_end  = gemm_tasks.size();
         L.big_end   = big_tasks.size();
         L.solve_end = solve_tasks.size();
+        L.bsolve_end = bsolve_tasks.size();
     }
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
@@ -399,6 +462,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     gemm_tasks.insert( gemm_tasks.end(), big_tasks.begin(), big_tasks.end() );
     if ( ( rc = pa_upload( &dev->d_gemm_tasks, gemm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_solve_tasks, solve_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
 
     /* LDS needs of the diagonal, TRSM and solve kernels */
     size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 1024;
@@ -443,6 +507,13 @@ pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
     size_t bytes = (size_t)std::max<int64_t>( dev->coefnbr, 1 ) * sizeof( T );
     if ( !dev->d_L ) PA_HIP( hipMalloc( &dev->d_L, bytes ) );
     PA_HIP( hipMemsetAsync( dev->d_L, 0, bytes, dev->stream ) );
+    if ( !lu && !dev->d_DT ) {
+        /* transposed diagonal blocks of the LL^t solve: only their upper
+         * triangles are ever written, the rest stays zero */
+        size_t dbytes = (size_t)std::max<int64_t>( dev->dtnbr, 1 ) * sizeof( T );
+        PA_HIP( hipMalloc( &dev->d_DT, dbytes ) );
+        PA_HIP( hipMemsetAsync( dev->d_DT, 0, dbytes, dev->stream ) );
+    }
     if ( lu ) {
         if ( !dev->d_U ) PA_HIP( hipMalloc( &dev->d_U, bytes ) );
         PA_HIP( hipMemsetAsync( dev->d_U, 0, bytes, dev->stream ) );
@@ -541,6 +612,11 @@ pa_factorize_t( pa_device_s *dev, double crit )
         }
         PA_PROF_MARK( 5 );
     }
+    if ( !lu && dev->S.cblknbr > 0 ) {
+        /* transposed diagonal blocks for the backward solve sweep */
+        k_diag_transpose<T><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, L, (T *)dev->d_DT );
+        PA_HIP( hipGetLastError() );
+    }
     return PASTIX_SUCCESS;
 }
 
@@ -557,6 +633,17 @@ pa_collect_profile( pa_device_s *dev )
             float ms = 0.f;
             PA_HIP( hipEventElapsedTime( &ms, dev->prof_ev[6 * li + 2 * c], dev->prof_ev[6 * li + 2 * c + 1] ) );
             dev->prof_ms[c] += ms;
+        }
+        if ( getenv( "PASTIX_AMD_LEVEL_REPORT" ) ) {
+            /* per-level timing report on stderr: the critical path by level */
+            float t[3];
+            for ( int c = 0; c < 3; c++ )
+                PA_HIP( hipEventElapsedTime( &t[c], dev->prof_ev[6 * li + 2 * c], dev->prof_ev[6 * li + 2 * c + 1] ) );
+            fprintf( stderr, "level %3zu cblks %6d ncmax %4d trsm %7d gemm64 %7d gemm128 %6d | diag %8.3f trsm %8.3f gemm %8.3f ms"
+                             " | gemm %6.1f TF trsm %6.1f TF\n",
+                     li, (int)( lev.cblk_end - lev.cblk_beg ), lev.ncmax, (int)( lev.trsm_end - lev.trsm_beg ),
+                     (int)( lev.gemm_end - lev.gemm_beg ), (int)( lev.big_end - lev.big_beg ), t[0], t[1], t[2],
+                     lev.flops[2] / ( t[2] * 1e9 ), lev.flops[1] / ( t[1] * 1e9 ) );
         }
         dev->prof_launch[0] += 1;
         dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lu_launches( dev ) ) : 0;
@@ -604,21 +691,62 @@ pa_dev_last_fact_event_seconds( const pastix_data_t *pd )
 /* ------------------------------------------------------------------------- */
 /* solve                                                                     */
 /* ------------------------------------------------------------------------- */
+/* largest cblk width handled by the register-prefetching diagonal solves */
+#define PA_SD_MAXCOL 512
+
+template <typename T, bool BWD, bool UNIT, int TH>
+static void
+pa_solve_diag_launch( pa_device_s *dev, dim3 grid, size_t lds, const int32_t *cb, const T *C, const T *CT,
+                      const int64_t *ct_off, T *x, int32_t nrhs, int32_t rchunk, int64_t ld )
+{
+    if ( BWD ) k_solve_diag_bwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, CT, ct_off, x, nrhs, rchunk, ld );
+    else       k_solve_diag_fwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, rchunk, ld );
+}
+
+/* diagonal solves of one level: the register-prefetching kernels (workgroup
+ * of the level's widest cblk rounded up to 64/128/256/512 threads) for cblks
+ * of up to PA_SD_MAXCOL columns, the LDS-blocked kernel above that */
+template <typename T, bool BWD, bool UNIT>
+static int
+pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off, T *x,
+                     int32_t nrhs, int64_t ld )
+{
+    const int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
+    const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
+    if ( lev.ncmax <= PA_SD_MAXCOL ) {
+        const size_t  avail  = (size_t)dev->lds_limit - 4096;
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)lev.ncmax * sizeof( T ) ) ) );
+        dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
+        size_t        lds = (size_t)lev.ncmax * rchunk * sizeof( T );
+        if ( lev.ncmax <= 64 )       pa_solve_diag_launch<T, BWD, UNIT, 64>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else if ( lev.ncmax <= 128 ) pa_solve_diag_launch<T, BWD, UNIT, 128>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else if ( lev.ncmax <= 256 ) pa_solve_diag_launch<T, BWD, UNIT, 256>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else                         pa_solve_diag_launch<T, BWD, UNIT, 512>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+    }
+    else {
+        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
+        if ( lds > (size_t)dev->lds_limit - 4096 ) return PASTIX_ERR_BADPARAMETER;
+        k_solve_diag<T, BWD, UNIT><<<ncb, 256, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, ld );
+    }
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+/* Cb is the factor of the backward sweep (lower storage, solved with its
+ * transpose); CbT/cbt_off locate the transposes of its diagonal blocks */
 template <typename T>
 static int
-pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b, T *x, int32_t nrhs, int64_t ld )
+pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off,
+            T *x, int32_t nrhs, int64_t ld )
 {
     const size_t nl = dev->levels.size();
+    int rc;
     for ( size_t li = 0; li < nl; li++ ) {
         const pa_level_t &lev = dev->levels[li];
-        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
-        if ( !ncb ) continue;
-        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
-        if ( lds > (size_t)dev->lds_limit ) return PASTIX_ERR_BADPARAMETER;
-        if ( unit_f )
-            k_solve_diag<T, false, true><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cf, x, nrhs, ld );
-        else
-            k_solve_diag<T, false, false><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cf, x, nrhs, ld );
+        if ( lev.cblk_end == lev.cblk_beg ) continue;
+        rc = unit_f ? pa_solve_diag_level<T, false, true>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld )
+                    : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld );
+        if ( rc ) return rc;
         int64_t ns = lev.solve_end - lev.solve_beg;
         if ( ns > 0 )
             k_solve_fwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x, nrhs, ld );
@@ -626,17 +754,14 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
     }
     for ( size_t li = nl; li-- > 0; ) {
         const pa_level_t &lev = dev->levels[li];
-        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
-        if ( !ncb ) continue;
-        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
-        int64_t ns = lev.solve_end - lev.solve_beg;
+        if ( lev.cblk_end == lev.cblk_beg ) continue;
+        int64_t ns = lev.bsolve_end - lev.bsolve_beg;
         if ( ns > 0 )
-            k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cb, x, nrhs, ld );
-        if ( unit_b )
-            k_solve_diag<T, true, true><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cb, x, nrhs, ld );
-        else
-            k_solve_diag<T, true, false><<<ncb, 256, lds, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.cblk_beg, Cb, x, nrhs, ld );
+            k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
+        rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld )
+                    : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld );
+        if ( rc ) return rc;
     }
     return PASTIX_SUCCESS;
 }
@@ -663,9 +788,14 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
     bool lu = dev->lu, tr = ( trans != PastixNoTrans );
     int fwd_side = ( lu && tr ) ? 1 : 0, bwd_side = ( lu && !tr ) ? 1 : 0;
     bool unit_f = lu && !tr, unit_b = lu && tr;
+    /* transposed diagonal blocks of the backward factor: for LU the other
+     * factor's diagonal square, for LL^t the separate transposed array */
+    const int64_t *ct_off = lu ? dev->d_c_coef : dev->d_c_dtoff;
+    void          *ct     = lu ? ( bwd_side ? dev->d_L : dev->d_U ) : dev->d_DT;
     if ( dev->prec == PA_PREC_DOUBLE ) {
         const double *L = (const double *)dev->d_L, *U = (const double *)dev->d_U;
-        return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, d_x, nrhs, ld );
+        return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const double *)ct, ct_off,
+                                   d_x, nrhs, ld );
     }
     /* mixed precision: down-convert, FP32 sweeps, up-convert
      * (reference sopalin/pastix_task_solve.c:583-592, pastix_rhs.c:162-242) */
@@ -678,7 +808,8 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
     for ( int32_t r = 0; r < nrhs; r++ )
         k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
     const float *L = (const float *)dev->d_L, *U = (const float *)dev->d_U;
-    int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, dev->d_xs, nrhs, dev->n );
+    int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const float *)ct, ct_off,
+                                dev->d_xs, nrhs, dev->n );
     if ( rc ) return rc;
     for ( int32_t r = 0; r < nrhs; r++ )
         k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );

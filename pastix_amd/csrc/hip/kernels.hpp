@@ -29,7 +29,19 @@ struct pa_dstruct {
     const int32_t *b_lrow;   /* [bloknbr]   last row (inclusive)                 */
     const int32_t *b_fcblk;  /* [bloknbr]   faced cblk                           */
     const int32_t *b_rowoff; /* [bloknbr]   rows of the blocks above in the cblk */
+    const int64_t *c_dtoff;  /* [cblknbr]   offset of the transposed diagonal
+                                            block in the LL^t transposed array   */
+    /* per off-diagonal row of every cblk (cblk k's rows start at r_base[k]):
+     * x = global row, y = height of its block, (z, w) = coeftab offset of the
+     * entry (row, first column) as low/high 32-bit halves */
+    const int64_t *r_base;   /* [cblknbr]   first row descriptor of the cblk     */
+    const int4    *r_info;   /* [sum of off-diagonal rows]                       */
 };
+
+__device__ __forceinline__ int64_t pa_rinfo_off( const int4 &ri )
+{
+    return (int64_t)( (uint64_t)(uint32_t)ri.z | ( (uint64_t)(uint32_t)ri.w << 32 ) );
+}
 
 /* One output tile of the scatter-GEMM of a level. */
 struct pa_gemm_task {
@@ -916,45 +928,321 @@ k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t n
     }
 }
 
-/* Off-diagonal part of the solve, tiled over (cblk, 64 rows, 64 columns).
- * Forward:  x[row] -= A[row, cols] x_k[cols]        (scatter, atomics)
- * Backward: x_k[cols] -= A[rows, cols]^T x[rows]    (gather, atomics)   */
-#define PA_SOLVE_ROWS 64
-#define PA_SOLVE_COLS 64
+/* Sum of v[0..15] over the 64 lanes of a wave by a transposing butterfly:
+ * on return lane l holds the wave total of column (l >> 2) & 15 (the four
+ * lanes of a quad hold the same value); 17 exchanges instead of 16 x 6. */
+template <typename T>
+__device__ __forceinline__ T
+pa_wave_sum16( T v[16], int lane )
+{
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for ( int k = 0; k < 8; k++ ) {
+        T send = b5 ? v[k] : v[k + 8];
+        T keep = b5 ? v[k + 8] : v[k];
+        v[k]   = keep + __shfl_xor( send, 32 );
+    }
+#pragma unroll
+    for ( int k = 0; k < 4; k++ ) {
+        T send = b4 ? v[k] : v[k + 4];
+        T keep = b4 ? v[k + 4] : v[k];
+        v[k]   = keep + __shfl_xor( send, 16 );
+    }
+#pragma unroll
+    for ( int k = 0; k < 2; k++ ) {
+        T send = b3 ? v[k] : v[k + 2];
+        T keep = b3 ? v[k + 2] : v[k];
+        v[k]   = keep + __shfl_xor( send, 8 );
+    }
+    T send = b2 ? v[0] : v[1];
+    T s    = ( b2 ? v[1] : v[0] ) + __shfl_xor( send, 4 );
+    s += __shfl_xor( s, 2 );
+    s += __shfl_xor( s, 1 );
+    return s;
+}
+
+/* wave-uniform broadcast of lane l's value (v_readlane, no LDS round trip) */
+__device__ __forceinline__ float
+pa_readlane( float v, int l )
+{
+    return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), l ) );
+}
+__device__ __forceinline__ double
+pa_readlane( double v, int l )
+{
+    int lo = __builtin_amdgcn_readlane( __double2loint( v ), l );
+    int hi = __builtin_amdgcn_readlane( __double2hiint( v ), l );
+    return __hiloint2double( hi, lo );
+}
+
+/*
+ * Diagonal solves for cblks of at most TH columns, one workgroup per (cblk,
+ * RHS chunk), x_k staged in LDS.  Thread t owns row t (forward) or column t
+ * (backward) of the diagonal block and keeps the 16 entries of a 16-column
+ * block in registers.  A ring of PA_SD_RING blocks is kept in flight, so the
+ * loads of block b + PA_SD_RING - 1 overlap the work on blocks b..: the
+ * sequential sweep over the blocks does not wait on global memory.  Per
+ * block: the 16 lanes owning the block's rows (columns) solve the 16 x 16
+ * triangle with v_readlane broadcasts, a barrier publishes x_block, then
+ * every thread applies it to its own entry: one barrier per block.
+ *   forward:  x_k <- D^{-1} x_k, lane t holds D[t, jb..jb+15]
+ *   backward: x_k <- D^{-T} x_k, lane t holds D[jb..jb+15, t], read from a
+ *             transposed copy of D (the other factor's diagonal block for
+ *             LU, a separate array for LL^t) so the loads are coalesced
+ * Loads use clamped in-range addresses, so the loops carry no per-element
+ * branches.
+ */
+#define PA_SD_RING 4
+
+/* The ring loads are unconditional with clamped in-range addresses: a load
+ * that sits under a lane-dependent branch makes the compiler wait for every
+ * outstanding load at the next use of any ring entry, which would serialise
+ * the prefetch.  The entries fetched for rows/columns outside the block, past
+ * nc or above the diagonal are never combined with a live value (lanes of
+ * other rows never write, lanes past nc carry zeros through the triangle,
+ * and the rest updates of a block only exist when the block is full). */
+template <typename T>
+__device__ __forceinline__ void
+pa_sd_load_rows( T (&v)[16], const T *D, int32_t nc, int t, int32_t jb )
+{
+    /* D[t, jb + j] */
+    const T *Dt = D + min( t, nc - 1 );
+#pragma unroll
+    for ( int j = 0; j < 16; j++ ) v[j] = Dt[(int64_t)min( jb + j, nc - 1 ) * nc];
+}
+
+template <typename T>
+__device__ __forceinline__ void
+pa_sd_load_cols( T (&v)[16], const T *DT, int32_t nc, int t, int32_t jb )
+{
+    /* D[jb + i, t] = DT[t, jb + i], read from the transposed block so that
+     * the lanes of a wave read consecutive addresses */
+    pa_sd_load_rows( v, DT, nc, t, jb );
+}
+
+template <typename T, bool UNIT, int TH>
+__global__ __launch_bounds__( TH ) void
+k_solve_diag_fwd( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
+{
+    extern __shared__ unsigned char pa_smem_raw[];
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
+    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
+    const T *D = Coef + S.c_coef[k];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        int32_t i = idx % nc, r = idx / nc;
+        xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
+    }
+    T ring[PA_SD_RING][16] = {};
+#pragma unroll
+    for ( int u = 0; u < PA_SD_RING - 1; u++ ) pa_sd_load_rows( ring[u], D, nc, t, u * 16 );
+    __syncthreads();
+    const int32_t nblk = ( nc + 15 ) >> 4;
+    for ( int32_t b0 = 0; b0 < nblk; b0 += PA_SD_RING ) {
+#pragma unroll
+        for ( int u = 0; u < PA_SD_RING; u++ ) {
+            const int32_t b = b0 + u;
+            if ( b >= nblk ) break;
+            pa_sd_load_rows( ring[( u + PA_SD_RING - 1 ) % PA_SD_RING], D, nc, t, ( b + PA_SD_RING - 1 ) * 16 );
+            const T(&cur)[16] = ring[u];
+            const int32_t jb = b * 16, jn = jb + 16;
+            if ( wave == ( jb >> 6 ) ) {
+                /* rows jb..jb+15 live in lanes base..base+15 of this wave.
+                 * cs[j] = L[i][j] / L[j][j] below the diagonal of the real
+                 * rows and 0 elsewhere, so lane j's running value can be
+                 * broadcast unscaled and the chain is readlane + fma. */
+                const int  base = jb & 63, i = lane - base;
+                const bool in = ( i >= 0 ) && ( i < 16 ) && ( jb + i < nc );
+                T d = T( 1 );
+#pragma unroll
+                for ( int j = 0; j < 16; j++ )
+                    if ( i == j ) d = cur[j];
+                const T dinv = ( UNIT || !in ) ? T( 1 ) : T( 1 ) / d;
+                T cs[16];
+#pragma unroll
+                for ( int j = 0; j < 16; j++ ) {
+                    const T rj = UNIT ? T( 1 ) : pa_readlane( dinv, base + j );
+                    cs[j] = ( in && i > j ) ? cur[j] * rj : T( 0 );
+                }
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    T xi = in ? xs[r * nc + jb + i] : T( 0 );
+#pragma unroll
+                    for ( int j = 0; j < 16; j++ ) xi -= cs[j] * pa_readlane( xi, base + j );
+                    if ( in ) xs[r * nc + jb + i] = xi * dinv;
+                }
+            }
+            __syncthreads();
+            if ( t < nc && t >= jn ) {
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    T s = T( 0 );
+#pragma unroll
+                    for ( int j = 0; j < 16; j++ ) s += cur[j] * xs[r * nc + jb + j];
+                    xs[r * nc + t] -= s;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        int32_t i = idx % nc, r = idx / nc;
+        x[fcol + i + ( r0 + r ) * ldx] = xs[r * nc + i];
+    }
+}
+
+template <typename T, bool UNIT, int TH>
+__global__ __launch_bounds__( TH ) void
+k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64_t *dt_off, T *x, int32_t nrhs,
+                  int32_t rchunk, int64_t ldx )
+{
+    extern __shared__ unsigned char pa_smem_raw[];
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
+    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
+    const T *D = DTab + dt_off[k]; /* the transposed diagonal block */
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        int32_t i = idx % nc, r = idx / nc;
+        xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
+    }
+    const int32_t nblk = ( nc + 15 ) >> 4;
+    /* step u of the sweep processes block nblk - 1 - u */
+    T ring[PA_SD_RING][16] = {};
+#pragma unroll
+    for ( int u = 0; u < PA_SD_RING - 1; u++ ) {
+        /* unconditional (see above): past the first block, block 0 is reloaded */
+        pa_sd_load_cols( ring[u], D, nc, t, max( nblk - 1 - u, 0 ) * 16 );
+    }
+    __syncthreads();
+    for ( int32_t s0 = 0; s0 < nblk; s0 += PA_SD_RING ) {
+#pragma unroll
+        for ( int u = 0; u < PA_SD_RING; u++ ) {
+            const int32_t st = s0 + u;
+            if ( st >= nblk ) break;
+            const int32_t b  = nblk - 1 - st;
+            const int32_t bp = b - ( PA_SD_RING - 1 );
+            pa_sd_load_cols( ring[( u + PA_SD_RING - 1 ) % PA_SD_RING], D, nc, t, max( bp, 0 ) * 16 );
+            const T(&cur)[16] = ring[u];
+            const int32_t jb = b * 16;
+            if ( wave == ( jb >> 6 ) ) {
+                /* rows jb..jb+15 of U = D^T live in lanes base..base+15;
+                 * same scaled-coefficient chain as the forward sweep */
+                const int  base = jb & 63, j = lane - base;
+                const bool in = ( j >= 0 ) && ( j < 16 ) && ( jb + j < nc );
+                T d = T( 1 );
+#pragma unroll
+                for ( int i = 0; i < 16; i++ )
+                    if ( j == i ) d = cur[i];
+                const T dinv = ( UNIT || !in ) ? T( 1 ) : T( 1 ) / d;
+                T cs[16];
+#pragma unroll
+                for ( int i = 0; i < 16; i++ ) {
+                    const T ri = UNIT ? T( 1 ) : pa_readlane( dinv, base + i );
+                    cs[i] = ( in && j < i ) ? cur[i] * ri : T( 0 );
+                }
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    T xj = in ? xs[r * nc + jb + j] : T( 0 );
+#pragma unroll
+                    for ( int i = 15; i >= 0; i-- ) xj -= cs[i] * pa_readlane( xj, base + i );
+                    if ( in ) xs[r * nc + jb + j] = xj * dinv;
+                }
+            }
+            __syncthreads();
+            if ( t < nc && t < jb ) {
+                /* the first block processed may be partial (rows past nc) */
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    T sum = T( 0 );
+#pragma unroll
+                    for ( int i = 0; i < 16; i++ ) {
+                        const int32_t row = min( jb + i, nc - 1 );
+                        sum += ( jb + i < nc ? cur[i] : T( 0 ) ) * xs[r * nc + row];
+                    }
+                    xs[r * nc + t] -= sum;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        int32_t i = idx % nc, r = idx / nc;
+        x[fcol + i + ( r0 + r ) * ldx] = xs[r * nc + i];
+    }
+}
+
+/* DT_k = D_k^T for the lower triangle of every diagonal block (LL^t), through
+ * 32 x 32 LDS tiles so that reads and writes are coalesced; one workgroup
+ * per cblk.  The upper triangle of DT_k is left as it is (zero). */
+template <typename T>
+__global__ __launch_bounds__( 256 ) void
+k_diag_transpose( pa_dstruct S, const T *Coef, T *DTab )
+{
+    __shared__ T tile[32][33];
+    const int32_t k = blockIdx.x, nc = S.c_ncol[k];
+    const T *D  = Coef + S.c_coef[k];
+    T       *DT = DTab + S.c_dtoff[k];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; /* 32 x 8 */
+    const int32_t nt = ( nc + 31 ) >> 5;
+    for ( int32_t tj = 0; tj < nt; tj++ )
+        for ( int32_t ti = tj; ti < nt; ti++ ) {
+            /* tile (ti, tj) of D: rows 32 ti.., columns 32 tj.. */
+            for ( int yy = ty; yy < 32; yy += 8 ) {
+                int32_t i = ti * 32 + tx, j = tj * 32 + yy;
+                tile[yy][tx] = ( i < nc && j < nc ) ? D[i + (int64_t)j * nc] : T( 0 );
+            }
+            __syncthreads();
+            for ( int yy = ty; yy < 32; yy += 8 ) {
+                int32_t i = ti * 32 + yy, j = tj * 32 + tx; /* DT[j, i] = D[i, j] */
+                if ( i < nc && j < nc && i >= j ) DT[j + (int64_t)i * nc] = tile[tx][yy];
+            }
+            __syncthreads();
+        }
+}
+
+/*
+ * Off-diagonal part of the solve, with the row descriptors of S.r_info (no
+ * search).  A task is (cblk k, first off-diagonal row r0, first column c0,
+ * descriptor index of row r0).
+ * Forward:  x[rows] -= A[rows, cols] x_k[cols]: 256 rows x 64 columns, one
+ *           row per lane, one atomic per row.
+ * Backward: x_k[cols] -= A[rows, cols]^T x[rows]: each wave takes 16 columns
+ *           and 256 rows (4 groups of 64) and reduces over its lanes with a
+ *           transposing butterfly, one atomic per column.  A task has
+ *           PA_SB_NCG(nc) column groups and 4 / PA_SB_NCG(nc) row parts.
+ */
+#define PA_SF_ROWS 256
+#define PA_SF_COLS 64
+#define PA_SB_NCG( nc ) ( ( nc ) <= 16 ? 1 : ( ( nc ) <= 32 ? 2 : 4 ) )
+
 template <typename T>
 __global__ __launch_bounds__( 256 ) void
 k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
-    __shared__ T red[4][PA_SOLVE_ROWS];
+    __shared__ T xk[PA_SF_COLS];
     const int4    tk = tasks[blockIdx.x];
     const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
-    const int32_t R = S.c_stride[k] - nc;
-    const int t = threadIdx.x, rl = t & 63, g = t >> 6;
-    const int32_t v = tk.y + rl;
+    const int32_t R = S.c_stride[k] - nc, c0 = tk.z, ncols = min( PA_SF_COLS, nc - c0 );
+    const int t = threadIdx.x;
+    const int32_t v = tk.y + t;
     const bool    valid = v < R;
-    int32_t grow = 0, M = 1;
-    const T *A = Coef;
-    if ( valid ) {
-        int32_t xr  = nc + v;
-        int32_t b   = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
-        int32_t loc = xr - S.b_rowoff[b];
-        M    = S.b_lrow[b] - S.b_frow[b] + 1;
-        grow = S.b_frow[b] + loc;
-        A    = Coef + S.c_coef[k] + (int64_t)S.b_rowoff[b] * nc + loc;
-    }
-    const int32_t c0 = tk.z + g * 16;
-    const int32_t c1 = min( c0 + 16, min( tk.z + PA_SOLVE_COLS, nc ) );
+    int4 ri = make_int4( 0, 1, 0, 0 );
+    if ( valid ) ri = S.r_info[S.r_base[k] + v];
+    const T *A = Coef + pa_rinfo_off( ri ) + (int64_t)c0 * ri.y;
     for ( int32_t r = 0; r < nrhs; r++ ) {
-        T acc = T( 0 );
-        if ( valid ) {
-            const T *xk = x + fcol + r * ldx;
-#pragma unroll
-            for ( int32_t cc = 0; cc < 16; cc++ )
-                if ( c0 + cc < c1 ) acc += A[(int64_t)( c0 + cc ) * M] * xk[c0 + cc];
-        }
-        red[g][rl] = acc;
+        if ( t < ncols ) xk[t] = x[fcol + c0 + t + r * ldx];
         __syncthreads();
-        if ( g == 0 && valid ) pa_atomic_add( x + grow + r * ldx, -( red[0][rl] + red[1][rl] + red[2][rl] + red[3][rl] ) );
+        if ( valid ) {
+            T acc = T( 0 );
+            int32_t c = 0;
+            for ( ; c + 16 <= ncols; c += 16 ) {
+                T a[16];
+#pragma unroll
+                for ( int u = 0; u < 16; u++ ) a[u] = A[(int64_t)( c + u ) * ri.y];
+#pragma unroll
+                for ( int u = 0; u < 16; u++ ) acc += a[u] * xk[c + u];
+            }
+            for ( ; c < ncols; c++ ) acc += A[(int64_t)c * ri.y] * xk[c];
+            pa_atomic_add( x + ri.x + r * ldx, -acc );
+        }
         __syncthreads();
     }
 }
@@ -966,32 +1254,36 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     const int4    tk = tasks[blockIdx.x];
     const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
     const int32_t R = S.c_stride[k] - nc;
-    const int t = threadIdx.x, rl = t & 63, g = t >> 6;
-    const int32_t v = tk.y + rl;
-    const bool    valid = v < R;
-    int32_t grow = 0, M = 1;
-    const T *A = Coef;
-    if ( valid ) {
-        int32_t xr  = nc + v;
-        int32_t b   = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, xr );
-        int32_t loc = xr - S.b_rowoff[b];
-        M    = S.b_lrow[b] - S.b_frow[b] + 1;
-        grow = S.b_frow[b] + loc;
-        A    = Coef + S.c_coef[k] + (int64_t)S.b_rowoff[b] * nc + loc;
+    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    const int     ncg = PA_SB_NCG( nc );
+    const int32_t cb = tk.z + ( g % ncg ) * 16, ncols = max( 0, min( 16, nc - cb ) );
+    const int32_t rw = tk.y + ( g / ncg ) * 256; /* first row of this wave */
+    const int4   *info = S.r_info + S.r_base[k];
+    int4 ri[4];
+    bool ok[4];
+#pragma unroll
+    for ( int q = 0; q < 4; q++ ) {
+        const int32_t row = rw + q * 64 + lane;
+        ok[q] = ( row < R ) && ( ncols > 0 );
+        ri[q] = ok[q] ? info[row] : make_int4( 0, 1, 0, 0 );
     }
-    const int32_t c0 = tk.z + g * 16;
-    const int32_t c1 = min( c0 + 16, min( tk.z + PA_SOLVE_COLS, nc ) );
-    T a[16];
-#pragma unroll
-    for ( int cc = 0; cc < 16; cc++ ) a[cc] = ( valid && c0 + cc < c1 ) ? A[(int64_t)( c0 + cc ) * M] : T( 0 );
     for ( int32_t r = 0; r < nrhs; r++ ) {
-        const T xv = valid ? x[grow + r * ldx] : T( 0 );
+        T v[16];
 #pragma unroll
-        for ( int cc = 0; cc < 16; cc++ ) {
-            T part = a[cc] * xv;
-            for ( int off = 32; off > 0; off >>= 1 ) part += __shfl_down( part, off, 64 );
-            if ( rl == 0 && c0 + cc < c1 ) pa_atomic_add( x + fcol + c0 + cc + r * ldx, -part );
+        for ( int cc = 0; cc < 16; cc++ ) v[cc] = T( 0 );
+#pragma unroll
+        for ( int q = 0; q < 4; q++ ) {
+            if ( ok[q] ) {
+                const T  xv = x[ri[q].x + r * ldx];
+                const T *A  = Coef + pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y;
+#pragma unroll
+                for ( int cc = 0; cc < 16; cc++ )
+                    if ( cc < ncols ) v[cc] += A[(int64_t)cc * ri[q].y] * xv;
+            }
         }
+        T s = pa_wave_sum16<T>( v, lane );
+        const int col = lane >> 2;
+        if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col + r * ldx, -s );
     }
 }
 
