@@ -8,13 +8,13 @@ solver structure) runs on the host in C.  This package holds the ctypes
 binding, the reference-style Python wrapper and the differentiable solve.
 """
 from . import enum
-from .enum import iparm, dparm, factotype, refine, verbose, trans, mtxtype
+from .enum import iparm, dparm, factotype, refine, verbose, trans, mtxtype, ordering
 from .solver import (solver, spmatrix, order, initParam, init, finalize, task_analyze,
                      task_numfact, task_solve, task_refine, get_structure, get_coeftab,
                      PastixError)
 from . import problems
 
-__all__ = ["enum", "iparm", "dparm", "factotype", "refine", "verbose", "trans", "mtxtype",
+__all__ = ["enum", "iparm", "dparm", "factotype", "refine", "verbose", "trans", "mtxtype", "ordering",
            "solver", "spmatrix", "order", "initParam", "init", "finalize", "task_analyze",
            "task_numfact", "task_solve", "task_refine", "get_structure", "get_coeftab",
            "PastixError", "problems"]

@@ -574,7 +574,10 @@ pa_factorize_t( pa_device_s *dev, double crit )
         size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax * sizeof( T );
         const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
-        if ( lev.ncmax > 64 ) {
+        if ( !lu && sizeof( T ) == 4 && lev.ncmax > 64 && lev.ncmax <= 16 * PA_DREG_NT ) {
+            k_diag_potrf_reg<<<ncb, 1024, 0, dev->stream>>>( dev->S, cb, (float *)L, crit, c0, c1 );
+        }
+        else if ( lev.ncmax > 64 ) {
             if ( lu ) k_diag<T, true, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
             else      k_diag<T, false, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
         }

@@ -301,7 +301,10 @@ pa_gpucblk_diag( const SolverCblk *cblk, void *L, void *U, double criterion, int
     int32_t nc = (int32_t)cblk_colnbr( cblk );
     size_t lds = (size_t)pa_diag_nb<T>::value * nc * sizeof( T );
     /* negative diagonal terms are counted in the second word of the counter */
-    k_diag_one<T, LU><<<1, 512, lds, stream>>>( (T *)L, (T *)U, nc, criterion, d_npiv, d_npiv + 1 );
+    if ( !LU && sizeof( T ) == 4 && nc > 64 && nc <= 16 * PA_DREG_NT )
+        k_diag_potrf_reg_one<<<1, 1024, 0, stream>>>( (float *)L, nc, criterion, d_npiv, d_npiv + 1 );
+    else
+        k_diag_one<T, LU><<<1, 512, lds, stream>>>( (T *)L, (T *)U, nc, criterion, d_npiv, d_npiv + 1 );
     if ( hipGetLastError() != hipSuccess ) return -1.;
     return LU ? pa_fl_getrf( nc ) : pa_fl_potrf( nc );
 }

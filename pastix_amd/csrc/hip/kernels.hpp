@@ -65,6 +65,21 @@ namespace {
 /* ------------------------------------------------------------------------- */
 template <typename T> __device__ __forceinline__ T pa_abs( T x ) { return x < T( 0 ) ? -x : x; }
 
+/* wave-uniform broadcast of lane l's value (v_readlane, no LDS round trip) */
+__device__ __forceinline__ float
+pa_readlane( float v, int l )
+{
+    return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), l ) );
+}
+__device__ __forceinline__ double
+pa_readlane( double v, int l )
+{
+    int lo = __builtin_amdgcn_readlane( __double2loint( v ), l );
+    int hi = __builtin_amdgcn_readlane( __double2hiint( v ), l );
+    return __hiloint2double( hi, lo );
+}
+
+
 __device__ __forceinline__ void pa_atomic_add( double *p, double v ) { unsafeAtomicAdd( p, v ); }
 __device__ __forceinline__ void pa_atomic_add( float *p, float v ) { unsafeAtomicAdd( p, v ); }
 
@@ -379,6 +394,171 @@ k_diag_one( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     pa_diag_factor<T, LU, 512>( D, Du, nc, crit, npiv, nneg, reinterpret_cast<T *>( pa_smem_raw ) );
+}
+
+/* ------------------------------------------------------------------------- */
+/* Register-resident POTRF of an FP32 diagonal block of up to                */
+/* 16 * PA_DREG_NT columns by one workgroup of 1024 threads.                 */
+/*                                                                           */
+/* The lower triangle lives in registers as 16 x 16 tiles in MFMA            */
+/* accumulator layout: tile (i, j) (column-major lower index idx) is held by */
+/* wave idx % 16 in slot idx / 16.  A register tile holds C(i, j)^T: element */
+/* r of lane l is C[16 i + l % 16][16 j + 4 (l / 16) + r], so lanes 0..15    */
+/* hold 16 consecutive rows of one column and global accesses coalesce.      */
+/* Per 16-column panel p: the owners stage tile column p in LDS (double      */
+/* buffered), wave 0 factors the 16 x 16 diagonal tile with a v_readlane     */
+/* chain (static pivoting as the reference kernel), every thread solves one  */
+/* panel row against it by substitution, the finished panel is stored and    */
+/* the owners apply it to their trailing tiles with MFMA 16x16x4.  Three     */
+/* barriers per panel and no global round trip inside the sweep.            */
+/* ------------------------------------------------------------------------- */
+#define PA_DREG_NT 22 /* 352 columns */
+#define PA_DREG_S  16 /* slots per wave: 16 * 16 >= 22 * 23 / 2 tiles */
+#define PA_DREG_LD 17
+
+__device__ void
+pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
+{
+    typedef pa_mfma<float>::acc_t acc_t;
+    __shared__ float P[2][PA_DREG_NT * 16][PA_DREG_LD];
+    __shared__ float Rinv[16];
+    const int     t = threadIdx.x, lane = t & 63;
+    const int     wave = __builtin_amdgcn_readfirstlane( t >> 6 );
+    const int32_t nt = ( nc + 15 ) >> 4, ntiles = ( nt * ( nt + 1 ) ) >> 1;
+    const float   tcrit = (float)crit;
+    int32_t       mypiv = 0, myneg = 0;
+
+    int32_t ti[PA_DREG_S], tj[PA_DREG_S];
+    acc_t   acc[PA_DREG_S];
+#pragma unroll
+    for ( int s = 0; s < PA_DREG_S; s++ ) {
+        int32_t idx = s * 16 + wave, j = 0, rem = idx;
+        while ( j < nt && rem >= nt - j ) {
+            rem -= nt - j;
+            j++;
+        }
+        const bool used = idx < ntiles;
+        ti[s] = used ? j + rem : -1;
+        tj[s] = used ? j : -1;
+#pragma unroll
+        for ( int r = 0; r < 4; r++ ) {
+            const int32_t row = 16 * ti[s] + ( lane & 15 ), col = 16 * tj[s] + 4 * ( lane >> 4 ) + r;
+            acc[s][r] = ( used && row < nc && col < nc ) ? D[row + (int64_t)col * nc] : 0.f;
+        }
+    }
+
+    for ( int32_t p = 0; p < nt; p++ ) {
+        float( *Pp )[PA_DREG_LD] = P[p & 1];
+        const int32_t w = min( 16, nc - 16 * p ); /* columns of the panel */
+        const int32_t m = nc - 16 * p;            /* rows of the panel    */
+        /* (1) stage tile column p: P[16 (i - p) + row][col] */
+#pragma unroll
+        for ( int s = 0; s < PA_DREG_S; s++ ) {
+            if ( tj[s] == p ) {
+#pragma unroll
+                for ( int r = 0; r < 4; r++ ) Pp[16 * ( ti[s] - p ) + ( lane & 15 )][4 * ( lane >> 4 ) + r] = acc[s][r];
+            }
+        }
+        __syncthreads();
+        /* (2) 16 x 16 diagonal tile by wave 0, lane i (mod 16) holding row i */
+        if ( wave == 0 ) {
+            const int i = lane & 15;
+            float     a[16];
+#pragma unroll
+            for ( int j = 0; j < 16; j++ ) a[j] = Pp[i][j];
+            float rinv = 1.f;
+#pragma unroll
+            for ( int k = 0; k < 16; k++ ) {
+                if ( k < w ) {
+                    /* l_kk = sqrt(a_kk), column scaled by 1 / l_kk as the
+                     * reference kernel (hardware sqrt / reciprocal: the FP32
+                     * factor is refined in FP64 anyway) */
+                    float      d   = pa_readlane( a[k], k );
+                    const bool rep = fabsf( d ) < tcrit;
+                    if ( rep ) d = tcrit;
+                    if ( lane == 0 ) {
+                        mypiv += rep ? 1 : 0;
+                        myneg += ( d < 0.f ) ? 1 : 0;
+                    }
+                    const float l     = __builtin_amdgcn_sqrtf( d );
+                    const float alpha = __builtin_amdgcn_rcpf( l );
+                    if ( i == k ) {
+                        a[k] = l;
+                        rinv = alpha;
+                    }
+                    else if ( i > k ) a[k] *= alpha;
+#pragma unroll
+                    for ( int j = k + 1; j < 16; j++ ) {
+                        if ( j < w ) {
+                            const float ljk = pa_readlane( a[k], j );
+                            if ( i >= j ) a[j] -= a[k] * ljk;
+                        }
+                    }
+                }
+            }
+            if ( lane < 16 ) {
+#pragma unroll
+                for ( int j = 0; j < 16; j++ ) Pp[i][j] = a[j];
+                Rinv[i] = rinv;
+            }
+        }
+        __syncthreads();
+        /* (3) panel rows below the diagonal tile: X L^T = A by substitution,
+         * column by column (x_j -= x_c l_jc in increasing c, the order of
+         * the reference kernel) so the dependent chain is 2 ops per column */
+        if ( t < m - 16 ) {
+            float x[16];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) x[c] = Pp[16 + t][c];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) {
+                if ( c < w ) {
+                    x[c] *= Rinv[c];
+#pragma unroll
+                    for ( int j = c + 1; j < 16; j++ ) x[j] -= x[c] * Pp[j][c];
+                }
+            }
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) Pp[16 + t][c] = x[c];
+        }
+        __syncthreads();
+        /* (4) store the finished panel (lower part of the diagonal tile) */
+        if ( t < m ) {
+            for ( int c = 0; c < w; c++ )
+                if ( t >= c ) D[( 16 * p + t ) + (int64_t)( 16 * p + c ) * nc] = Pp[t][c];
+        }
+        /* (5) trailing update of the owned tiles right of the panel:
+         * C(i, j)^T -= L(j, p) L(i, p)^T */
+#pragma unroll
+        for ( int s = 0; s < PA_DREG_S; s++ ) {
+            if ( tj[s] > p ) {
+                const int32_t ra = 16 * ( tj[s] - p ) + ( lane & 15 ), rb = 16 * ( ti[s] - p ) + ( lane & 15 );
+#pragma unroll
+                for ( int q = 0; q < 4; q++ ) {
+                    const float a = -Pp[ra][4 * q + ( lane >> 4 )];
+                    const float b = Pp[rb][4 * q + ( lane >> 4 )];
+                    acc[s] = pa_mfma<float>::run( a, b, acc[s] );
+                }
+            }
+        }
+    }
+    if ( t == 0 ) {
+        if ( mypiv ) atomicAdd( npiv, mypiv );
+        if ( myneg ) atomicAdd( nneg, myneg );
+    }
+}
+
+__global__ __launch_bounds__( 1024 ) void
+k_diag_potrf_reg( pa_dstruct S, const int32_t *cblks, float *Lc, double crit, int32_t *npiv, int32_t *nneg )
+{
+    const int32_t k = cblks[blockIdx.x];
+    pa_diag_potrf_reg( Lc + S.c_coef[k], S.c_ncol[k], crit, npiv, nneg );
+}
+
+__global__ __launch_bounds__( 1024 ) void
+k_diag_potrf_reg_one( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
+{
+    pa_diag_potrf_reg( D, nc, crit, npiv, nneg );
 }
 
 /* ------------------------------------------------------------------------- */
@@ -961,19 +1141,6 @@ pa_wave_sum16( T v[16], int lane )
     return s;
 }
 
-/* wave-uniform broadcast of lane l's value (v_readlane, no LDS round trip) */
-__device__ __forceinline__ float
-pa_readlane( float v, int l )
-{
-    return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), l ) );
-}
-__device__ __forceinline__ double
-pa_readlane( double v, int l )
-{
-    int lo = __builtin_amdgcn_readlane( __double2loint( v ), l );
-    int hi = __builtin_amdgcn_readlane( __double2hiint( v ), l );
-    return __hiloint2double( hi, lo );
-}
 
 /*
  * Diagonal solves for cblks of at most TH columns, one workgroup per (cblk,

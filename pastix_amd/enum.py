@@ -108,6 +108,14 @@ class factotype:
     LDLH = 4
 
 
+class ordering:
+    Scotch = 0
+    Metis = 1
+    Personal = 2
+    PtScotch = 3
+    ParMetis = 4
+
+
 class refine:
     GMRES = 0
     CG = 1

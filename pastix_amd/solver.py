@@ -90,6 +90,21 @@ class order(object):
         _check(rc, "pastixOrderGrid")
         return o
 
+    @staticmethod
+    def from_permutation(perm):
+        """A personal ordering given by perm[old] = new only; the analysis
+        builds the supernode partition from the elimination tree."""
+        perm = np.asarray(perm, dtype=np.int64)
+        n = len(perm)
+        o = order()
+        lib = _lib.load()
+        _check(lib.pastixOrderAlloc(C.byref(o.o), n, 0), "pastixOrderAlloc")
+        pt = np.ctypeslib.as_array(o.o.permtab, (n,))
+        it = np.ctypeslib.as_array(o.o.peritab, (n,))
+        pt[:] = perm
+        it[perm] = np.arange(n)
+        return o
+
     def arrays(self):
         n, c = self.o.vertnbr, self.o.cblknbr
         return (np.ctypeslib.as_array(self.o.permtab, (n,)).copy(),
