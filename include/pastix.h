@@ -91,6 +91,18 @@ int pastix_amd_refine_dev( pastix_data_t *pastix_data, pastix_int_t nrhs,
 int pastix_amd_spmv_dev( pastix_data_t *pastix_data, pastix_trans_t trans, double alpha,
                          const void *dx, double beta, void *dy );
 
+/*
+ * Adjoint helper of the differentiable solve: out[e] = alpha * sum_r
+ * u[row(e), r] * v[col(e), r] for every stored entry e of the user's CSC
+ * pattern (colptr[n+1], rows[nnz]: pastix_int_t device arrays; u, v: device
+ * FP64 n x nrhs, leading dimensions ldu, ldv; out: device FP64 [nnz]).
+ * With u = lambda = A^{-T} g, v = x and alpha = -1 this is dLoss/dA on the
+ * pattern.  Runs on the library stream and synchronises it.
+ */
+int pastix_amd_sampled_outer( pastix_data_t *pastix_data, const void *d_colptr, const void *d_rows,
+                              pastix_int_t nrhs, const void *d_u, pastix_int_t ldu, const void *d_v,
+                              pastix_int_t ldv, double alpha, void *d_out );
+
 /* Synchronise the library stream and return the elapsed seconds of the last
  * factorization measured with device events (0 if none). */
 double pastix_amd_last_fact_seconds( pastix_data_t *pastix_data );

@@ -62,6 +62,7 @@ struct pa_device_s {
     int32_t *d_col2cblk, *d_perm, *d_invp;
     int64_t *d_c_dtoff, *d_r_base;
     int4    *d_r_info;
+    int32_t *d_r_blk;
     int64_t  dtnbr;      /* entries of the transposed diagonal blocks   */
     pa_dstruct S;
     int64_t  n, coefnbr;
@@ -143,6 +144,8 @@ pa_set_lds_limits( int bytes )
 {
     pa_lds_attr( k_trsm<double, 0>, bytes ); pa_lds_attr( k_trsm<double, 1>, bytes ); pa_lds_attr( k_trsm<double, 2>, bytes );
     pa_lds_attr( k_trsm<float, 0>, bytes );  pa_lds_attr( k_trsm<float, 1>, bytes );  pa_lds_attr( k_trsm<float, 2>, bytes );
+    pa_lds_attr( k_trsm_w<double, 0>, bytes ); pa_lds_attr( k_trsm_w<double, 1>, bytes ); pa_lds_attr( k_trsm_w<double, 2>, bytes );
+    pa_lds_attr( k_trsm_w<float, 0>, bytes );  pa_lds_attr( k_trsm_w<float, 1>, bytes );  pa_lds_attr( k_trsm_w<float, 2>, bytes );
     pa_lds_attr_sd<double, false>( bytes ); pa_lds_attr_sd<double, true>( bytes );
     pa_lds_attr_sd<float, false>( bytes );  pa_lds_attr_sd<float, true>( bytes );
     pa_lds_attr( k_solve_diag<double, false, false>, bytes ); pa_lds_attr( k_solve_diag<double, false, true>, bytes );
@@ -200,7 +203,8 @@ pa_free_structure( pa_device_s *dev )
     void *ptrs[] = { dev->d_c_fcol, dev->d_c_ncol, dev->d_c_stride, dev->d_c_fblok, dev->d_c_coef,
                      dev->d_b_frow, dev->d_b_lrow, dev->d_b_fcblk, dev->d_b_rowoff, dev->d_col2cblk,
                      dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks,
-                     dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info };
+                     dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info,
+                     dev->d_r_blk };
     for ( void *p : ptrs ) hipFree( p );
     dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
     dev->d_c_coef = NULL;
@@ -210,6 +214,7 @@ pa_free_structure( pa_device_s *dev )
     dev->d_bsolve_tasks = NULL;
     dev->d_c_dtoff = dev->d_r_base = NULL;
     dev->d_r_info = NULL;
+    dev->d_r_blk  = NULL;
     dev->levels.clear();
 }
 
@@ -262,6 +267,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     std::vector<int32_t> col2cblk( n + 1 ), perm( n + 1 ), invp( n + 1 );
     std::vector<int64_t> c_dtoff( cblknbr + 1 ), r_base( cblknbr + 1 );
     std::vector<int4>    r_info;
+    std::vector<int32_t> r_blk;
     int64_t coef = 0, dtoff = 0, rbase = 0;
     int32_t colmax = 0;
     for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
@@ -291,6 +297,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
                     int64_t off = coef + (int64_t)rowoff * c_ncol[k] + loc;
                     r_info.push_back( make_int4( (int32_t)b->frownum + loc, M, (int32_t)( off & 0xffffffffLL ),
                                                  (int32_t)( off >> 32 ) ) );
+                    r_blk.push_back( (int32_t)bi );
                 }
             }
             rowoff += (int32_t)blok_rownbr( b );
@@ -326,6 +333,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_c_dtoff, c_dtoff ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_r_base, r_base ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_r_info, r_info ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_r_blk, r_blk ) ) ) return rc;
 
     pa_dstruct &S = dev->S;
     S.cblknbr  = (int32_t)cblknbr;
@@ -342,6 +350,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     S.c_dtoff  = dev->d_c_dtoff;
     S.r_base   = dev->d_r_base;
     S.r_info   = dev->d_r_info;
+    S.r_blk    = dev->d_r_blk;
 
     /* ---- level schedule ---- */
     std::vector<pastix_int_t> level( cblknbr + 1 );
@@ -406,7 +415,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         }
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
-            for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
+            for ( int32_t r = 0; r < R; r += PA_TRSM_WR ) trsm_tasks.push_back( make_int2( k, r ) );
             /* solve update tasks: forward 256 rows x 64 columns; backward
              * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
             for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
@@ -465,7 +474,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
 
     /* LDS needs of the diagonal, TRSM and solve kernels */
-    size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 1024;
+    size_t need = std::max( pa_trsm_w_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {
         fprintf( stderr, "pastix_amd: cblks of %d columns exceed the LDS budget (%d bytes); lower IPARM_MAX_BLOCKSIZE\n",
                  colmax, dev->lds_limit );
@@ -590,13 +599,13 @@ pa_factorize_t( pa_device_s *dev, double crit )
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
         if ( ntr > 0 ) {
-            size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
+            size_t lds_trsm = pa_trsm_w_lds<T>( lev.ncmax );
             if ( lu ) {
-                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_trsm_w<T, 1><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_trsm_w<T, 2><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
             }
             else {
-                k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_trsm_w<T, 0><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
             }
             PA_HIP( hipGetLastError() );
         }
@@ -991,5 +1000,25 @@ pa_dev_profile( pastix_data_t *pd, int enable, double *out )
             out[6 + c] = dev->class_flops[c];
         }
     }
+    return PASTIX_SUCCESS;
+}
+
+/* out[e] = alpha sum_r u[row(e), r] v[col(e), r] on a CSC pattern in the user
+ * numbering (device arrays), on the library stream */
+extern "C" int
+pastix_amd_sampled_outer( pastix_data_t *pd, const void *d_colptr, const void *d_rows, pastix_int_t nrhs,
+                          const void *d_u, pastix_int_t ldu, const void *d_v, pastix_int_t ldv, double alpha,
+                          void *d_out )
+{
+    if ( pd == NULL || DEV( pd ) == NULL ) return PASTIX_ERR_BADPARAMETER;
+    pa_device_s *dev = DEV( pd );
+    const int64_t n = dev->n;
+    if ( n <= 0 || nrhs < 1 || ldu < n || ldv < n ) return PASTIX_ERR_BADPARAMETER;
+    PA_HIP( hipSetDevice( dev->device ) );
+    k_sampled_outer<<<(unsigned)( ( n + 255 ) / 256 ), 256, 0, dev->stream>>>(
+        n, (const int64_t *)d_colptr, (const int64_t *)d_rows, (int32_t)nrhs, (const double *)d_u, ldu,
+        (const double *)d_v, ldv, alpha, (double *)d_out );
+    PA_HIP( hipGetLastError() );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
     return PASTIX_SUCCESS;
 }

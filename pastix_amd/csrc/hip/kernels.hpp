@@ -36,7 +36,12 @@ struct pa_dstruct {
      * entry (row, first column) as low/high 32-bit halves */
     const int64_t *r_base;   /* [cblknbr]   first row descriptor of the cblk     */
     const int4    *r_info;   /* [sum of off-diagonal rows]                       */
+    const int32_t *r_blk;    /* [sum of off-diagonal rows] block of the row      */
 };
+
+/* LDS row padding of the GEMM operand tiles: 64 bytes, so that the four
+ * k-rows read by one MFMA operand load fall in different banks */
+template <typename T> struct pa_gpad { static constexpr int value = 64 / (int)sizeof( T ); };
 
 __device__ __forceinline__ int64_t pa_rinfo_off( const int4 &ri )
 {
@@ -44,6 +49,10 @@ __device__ __forceinline__ int64_t pa_rinfo_off( const int4 &ri )
 }
 
 /* One output tile of the scatter-GEMM of a level. */
+/* target-cblk blocks staged in LDS by the level GEMM (larger targets search
+ * the block table in global memory) */
+#define PA_GF_STAGE 512
+
 struct pa_gemm_task {
     int32_t k;      /* source cblk                                             */
     int32_t F;      /* target (faced) cblk of the sigma rows                   */
@@ -709,6 +718,101 @@ k_trsm( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
     pa_trsm_rows<T, MODE>( panel, panel, nc, rows, xoff, xld, reinterpret_cast<T *>( pa_smem_raw ) );
 }
 
+/*
+ * Level TRSM, one wave per 16-row slab of a cblk's off-diagonal rows
+ * (row descriptors from S.r_info):
+ *   X[:, J] <- ( X[:, J] - sum_{P<J} X[:, P] C[P, J] ) C[J, J]^{-1}
+ * over 16-column blocks J, with C(l, j) = L[j][l] (MODE 0) or D[l][j]
+ * (MODE 1, 2) as in pa_trsm_rows.  The slab lives in LDS; the block products
+ * run on MFMA 16x16x4 (register tile = X[:, J]^T in accumulator layout,
+ * A operand = C[P, J]^T from the L2-resident diagonal tile, B operand =
+ * X[:, P]^T from LDS); the 16 x 16 triangle is solved by the 16 row lanes,
+ * column by column (x_c scaled, then x_j -= x_c C(c, j) for j > c, the
+ * reference's order), with wave-uniform coefficients.  Rows are independent,
+ * so a wave never waits on another one.
+ */
+#define PA_TRSM_WR 16
+template <typename T> __host__ __device__ constexpr size_t pa_trsm_w_lds( int32_t nc )
+{
+    return (size_t)PA_TRSM_WR * ( nc + 1 ) * sizeof( T );
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__( 64 ) void
+k_trsm_w( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
+{
+    typedef typename pa_mfma<T>::acc_t acc_t;
+    extern __shared__ unsigned char pa_smem_raw[];
+    T *X = reinterpret_cast<T *>( pa_smem_raw ); /* [16][nc + 1] */
+    const int2    tk = tasks[blockIdx.x];
+    const int32_t k = tk.x, r0 = tk.y, nc = S.c_ncol[k], R = S.c_stride[k] - nc;
+    const int32_t rows = min( PA_TRSM_WR, R - r0 ), ldx = nc + 1;
+    T            *C = ( MODE == 2 ? Uc : Lc );
+    const T      *D = C + S.c_coef[k];
+    const int     lane = threadIdx.x, rl = lane & 15, cg = lane >> 4;
+    const bool    rok = rl < rows;
+    const int4    ri = rok ? S.r_info[S.r_base[k] + r0 + rl] : make_int4( 0, 1, 0, 0 );
+    const int64_t roff = pa_rinfo_off( ri );
+
+    for ( int32_t c = cg; c < nc; c += 4 ) X[rl * ldx + c] = rok ? C[roff + (int64_t)c * ri.y] : T( 0 );
+    __syncthreads();
+
+    const int32_t nb = ( nc + 15 ) >> 4;
+    for ( int32_t j = 0; j < nb; j++ ) {
+        const int32_t cj = 16 * j;
+        acc_t acc;
+#pragma unroll
+        for ( int r = 0; r < 4; r++ ) {
+            const int32_t a = cj + pa_mfma<T>::row( lane, r );
+            acc[r] = ( a < nc ) ? X[rl * ldx + a] : T( 0 );
+        }
+        /* column a = cj + lane % 16 of C for the A operand */
+        const int32_t a  = min( cj + rl, nc - 1 );
+        const bool    aok = cj + rl < nc;
+#pragma unroll 2
+        for ( int32_t p = 0; p < j; p++ ) {
+#pragma unroll
+            for ( int q = 0; q < 4; q++ ) {
+                const int32_t kk = 16 * p + 4 * q + cg;
+                const T       cv = MODE == 0 ? D[a + (int64_t)kk * nc] : D[kk + (int64_t)a * nc];
+                acc = pa_mfma<T>::run( aok ? -cv : T( 0 ), X[rl * ldx + kk], acc );
+            }
+        }
+#pragma unroll
+        for ( int r = 0; r < 4; r++ ) {
+            const int32_t aa = cj + pa_mfma<T>::row( lane, r );
+            if ( aa < nc ) X[rl * ldx + aa] = acc[r];
+        }
+        __syncthreads();
+        if ( lane < 16 ) {
+            const int32_t w = min( 16, nc - cj );
+            T x[16];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) x[c] = ( c < w ) ? X[lane * ldx + cj + c] : T( 0 );
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) {
+                if ( c < w ) {
+                    if ( MODE != 2 ) x[c] *= T( 1 ) / D[( cj + c ) + (int64_t)( cj + c ) * nc];
+#pragma unroll
+                    for ( int jj = c + 1; jj < 16; jj++ ) {
+                        if ( jj < w ) {
+                            const T cc = MODE == 0 ? D[( cj + jj ) + (int64_t)( cj + c ) * nc]
+                                                   : D[( cj + c ) + (int64_t)( cj + jj ) * nc];
+                            x[jj] -= x[c] * cc;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for ( int c = 0; c < 16; c++ )
+                if ( c < w ) X[lane * ldx + cj + c] = x[c];
+        }
+        __syncthreads();
+    }
+    if ( rok )
+        for ( int32_t c = cg; c < nc; c += 4 ) C[roff + (int64_t)c * ri.y] = X[rl * ldx + c];
+}
+
 /* Per-call description of a run of blocks (reference-signature entry points):
  * block i holds rows [pref[i], pref[i+1]) of the concatenated rows, starts at
  * off[i] with leading dimension ld[i]; for targets, toff/tld describe where
@@ -801,7 +905,7 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[TILE], T ( *Bs )[TILE] )
+              T ( *As )[TILE + pa_gpad<T>::value], T ( *Bs )[TILE + pa_gpad<T>::value] )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int NB  = TILE / 32;        /* MFMA blocks per wave per dimension */
@@ -889,65 +993,84 @@ template <typename T, int TILE>
 __global__ __launch_bounds__( 256 ) void
 k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
 {
-    __shared__ T As[PA_GKC][TILE];
-    __shared__ T Bs[PA_GKC][TILE];
+    constexpr int PAD = pa_gpad<T>::value;
+    __shared__ T As[PA_GKC][TILE + PAD];
+    __shared__ T Bs[PA_GKC][TILE + PAD];
     __shared__ int64_t a_off[TILE], b_off[TILE], c_off[TILE];
-    __shared__ int32_t a_ld[TILE], b_ld[TILE], c_ld[TILE], c_col[TILE], a_blk[TILE], b_blk[TILE];
+    __shared__ int32_t a_ld[TILE], b_ld[TILE], c_ld[TILE], c_col[TILE], a_blk[TILE], b_blk[TILE], grow_s[TILE];
+    /* first rows / last rows / row offsets of the target cblk's blocks */
+    __shared__ int32_t f_frow[PA_GF_STAGE], f_lrow[PA_GF_STAGE], f_roff[PA_GF_STAGE];
 
     const pa_gemm_task tk = tasks[blockIdx.x];
     const int32_t k = tk.k, F = tk.F, nc = S.c_ncol[k];
     const int32_t R = S.c_stride[k] - nc;
-    const int32_t fb = S.c_fblok[k], lb = S.c_fblok[k + 1];
     const bool uside = ( tk.flags & PA_TASK_USIDE ) != 0;
     const int t = threadIdx.x;
+    const int64_t cbase = S.c_coef[k], rb = S.r_base[k];
+    const int32_t fb0 = S.c_fblok[F], nfb = S.c_fblok[F + 1] - fb0;
+    const bool staged = nfb <= PA_GF_STAGE;
 
+    /* row descriptors from S.r_info (rho rows: u < TILE, sigma rows: u >= TILE) */
     for ( int u = t; u < 2 * TILE; u += 256 ) {
-        if ( u < TILE ) {
-            /* rho row */
-            const int32_t tt = u;
-            int32_t v = tk.r0 + tt;
-            if ( v < R ) {
-                int32_t x = nc + v;
-                int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
-                int32_t loc  = x - S.b_rowoff[b];
-                int32_t grow = S.b_frow[b] + loc;
-                a_off[tt] = (int64_t)S.b_rowoff[b] * nc + loc;
-                a_ld[tt]  = S.b_lrow[b] - S.b_frow[b] + 1;
-                a_blk[tt] = b;
-                int32_t ncF = S.c_ncol[F];
-                int32_t f   = pa_find_globrow( S.b_frow, S.c_fblok[F], S.c_fblok[F + 1] - 1, grow );
-                c_off[tt] = S.c_coef[F] + (int64_t)S.b_rowoff[f] * ncF + ( grow - S.b_frow[f] );
-                c_ld[tt]  = S.b_lrow[f] - S.b_frow[f] + 1;
-                /* a row outside the target structure is never written */
-                if ( grow < S.b_frow[f] || grow > S.b_lrow[f] ) a_ld[tt] = 0;
-            }
-            else {
-                a_off[tt] = 0; a_ld[tt] = 0; a_blk[tt] = 0; c_off[tt] = 0; c_ld[tt] = 0;
-            }
+        const bool rho = u < TILE;
+        const int32_t tt = rho ? u : u - TILE;
+        const int32_t v  = ( rho ? tk.r0 : tk.s0 ) + tt;
+        const bool ok = ( v < R ) && ( rho || tt < tk.slen );
+        int4 ri = make_int4( 0, 0, 0, 0 );
+        int32_t blk = 0;
+        if ( ok ) {
+            ri  = S.r_info[rb + v];
+            blk = S.r_blk[rb + v];
+        }
+        const int64_t off = ok ? pa_rinfo_off( ri ) - cbase : 0;
+        if ( rho ) {
+            a_off[tt] = off; a_ld[tt] = ri.y; a_blk[tt] = blk; grow_s[tt] = ok ? ri.x : -1;
         }
         else {
-            const int32_t ss = u - TILE;
-            int32_t v = tk.s0 + ss;
-            if ( ss < tk.slen && v < R ) {
-                int32_t x = nc + v;
-                int32_t b = pa_find_rowblk( S.b_rowoff, fb + 1, lb - 1, x );
-                int32_t loc  = x - S.b_rowoff[b];
-                int32_t grow = S.b_frow[b] + loc;
-                b_off[ss] = (int64_t)S.b_rowoff[b] * nc + loc;
-                b_ld[ss]  = S.b_lrow[b] - S.b_frow[b] + 1;
-                b_blk[ss] = b;
-                c_col[ss] = grow - S.c_fcol[F];
-                /* a column outside the target cblk is never written */
-                if ( c_col[ss] < 0 || c_col[ss] >= S.c_ncol[F] ) b_ld[ss] = 0;
-            }
-            else {
-                b_off[ss] = 0; b_ld[ss] = 0; b_blk[ss] = 0; c_col[ss] = 0;
-            }
+            const int32_t col = ri.x - S.c_fcol[F];
+            b_off[tt] = off; b_blk[tt] = blk; c_col[tt] = col;
+            /* a column outside the target cblk is never written */
+            b_ld[tt] = ( ok && col >= 0 && col < S.c_ncol[F] ) ? ri.y : 0;
+        }
+    }
+    if ( staged ) {
+        for ( int i = t; i < nfb; i += 256 ) {
+            f_frow[i] = S.b_frow[fb0 + i];
+            f_lrow[i] = S.b_lrow[fb0 + i];
+            f_roff[i] = S.b_rowoff[fb0 + i];
         }
     }
     __syncthreads();
+    /* target block of each rho row in F */
+    const int32_t ncF = S.c_ncol[F];
+    const int64_t coefF = S.c_coef[F];
+    for ( int tt = t; tt < TILE; tt += 256 ) {
+        const int32_t grow = grow_s[tt];
+        if ( grow < 0 ) {
+            a_ld[tt] = 0; c_off[tt] = 0; c_ld[tt] = 0;
+            continue;
+        }
+        int32_t fr, lr, ro;
+        if ( staged ) {
+            int32_t lo = 0, hi = nfb - 1;
+            while ( lo < hi ) {
+                int32_t mid = ( lo + hi + 1 ) >> 1;
+                if ( f_frow[mid] <= grow ) lo = mid;
+                else                       hi = mid - 1;
+            }
+            fr = f_frow[lo]; lr = f_lrow[lo]; ro = f_roff[lo];
+        }
+        else {
+            int32_t f = pa_find_globrow( S.b_frow, fb0, fb0 + nfb - 1, grow );
+            fr = S.b_frow[f]; lr = S.b_lrow[f]; ro = S.b_rowoff[f];
+        }
+        c_off[tt] = coefF + (int64_t)ro * ncF + ( grow - fr );
+        c_ld[tt]  = lr - fr + 1;
+        /* a row outside the target structure is never written */
+        if ( grow < fr || grow > lr ) a_ld[tt] = 0;
+    }
+    __syncthreads();
 
-    const int64_t cbase = S.c_coef[k];
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
@@ -965,8 +1088,8 @@ template <typename T>
 __global__ __launch_bounds__( 256 ) void
 k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_rows Br, int32_t ccol0 )
 {
-    __shared__ T As[PA_GKC][PA_GT];
-    __shared__ T Bs[PA_GKC][PA_GT];
+    __shared__ T As[PA_GKC][PA_GT + pa_gpad<T>::value];
+    __shared__ T Bs[PA_GKC][PA_GT + pa_gpad<T>::value];
     __shared__ int64_t a_off[PA_GT], b_off[PA_GT], c_off[PA_GT];
     __shared__ int32_t a_ld[PA_GT], b_ld[PA_GT], c_ld[PA_GT], c_col[PA_GT], a_blk[PA_GT], b_blk[PA_GT];
     const int t = threadIdx.x;
@@ -1542,6 +1665,23 @@ k_scatter( int64_t n, int32_t nrhs, const int32_t *perm, const TS *src, int64_t 
         dst[perm[j] + r * ldd] = (TD)src[j + r * lds];
     }
 }
+/* Sampled outer product on a CSC pattern (the adjoint of a sparse solve):
+ * out[e] = alpha * sum_r u[row(e), r] v[col(e), r] for every stored entry e
+ * of column col(e); one thread per column. */
+__global__ __launch_bounds__( 256 ) void
+k_sampled_outer( int64_t n, const int64_t *colptr, const int64_t *rows, int32_t nrhs, const double *u, int64_t ldu,
+                 const double *v, int64_t ldv, double alpha, double *out )
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( c >= n ) return;
+    for ( int64_t e = colptr[c]; e < colptr[c + 1]; e++ ) {
+        const int64_t r = rows[e];
+        double        s = 0.;
+        for ( int32_t q = 0; q < nrhs; q++ ) s += u[r + q * ldu] * v[c + q * ldv];
+        out[e] = alpha * s;
+    }
+}
+
 template <typename TD, typename TS>
 __global__ void
 k_convert( int64_t n, const TS *src, TD *dst )

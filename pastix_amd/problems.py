@@ -85,8 +85,8 @@ def anisotropic_diffusion(nx, ny, nz, theta, ratio_z=10.0):
     theta, K = diag(1, 1, ratio_z) theta.  The face coefficient between cells
     p and q is the arithmetic mean (theta_p + theta_q)/2 times the directional
     factor; a Dirichlet boundary face adds theta_p times the factor to the
-    diagonal.  SPD.  ``theta`` may be a numpy array or a torch tensor (the
-    assembly is then differentiable, see ``anisotropic_diffusion_torch``)."""
+    diagonal.  SPD.  The values are linear in theta: see
+    ``anisotropic_diffusion_jacobian`` for the map theta -> A.data."""
     n = nx * ny * nz
     theta = np.asarray(theta, dtype=np.float64)
     ii, jj, kk = _grid_index(nx, ny, nz)
@@ -107,6 +107,39 @@ def anisotropic_diffusion(nx, ny, nz, theta, ratio_z=10.0):
     A.sum_duplicates()
     A.sort_indices()
     return A
+
+
+def anisotropic_diffusion_jacobian(nx, ny, nz, ratio_z=10.0):
+    """The sparse linear map of ``anisotropic_diffusion``: returns (pattern,
+    J) with ``pattern`` the sorted CSC of A (values for theta = 1) and J an
+    (nnz x n) CSR matrix such that A(theta).data = J @ theta, so that
+    dLoss/dtheta = J^T dLoss/dA.data."""
+    n = nx * ny * nz
+    P = anisotropic_diffusion(nx, ny, nz, np.ones(n), ratio_z)
+    keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(P.indptr)) * n + P.indices.astype(np.int64)
+    ii, jj, kk = _grid_index(nx, ny, nz)
+    p = ii + nx * jj + nx * ny * kk
+    er, ec, ev = [], [], []   # rows of J (CSC positions), columns (cells), coefficients
+
+    def pos(r, c):
+        return np.searchsorted(keys, c.astype(np.int64) * n + r)
+    for (dx, dy, dz), fac in (((1, 0, 0), 1.0), ((-1, 0, 0), 1.0), ((0, 1, 0), 1.0),
+                              ((0, -1, 0), 1.0), ((0, 0, 1), ratio_z), ((0, 0, -1), ratio_z)):
+        i2, j2, k2 = ii + dx, jj + dy, kk + dz
+        m = (i2 >= 0) & (i2 < nx) & (j2 >= 0) & (j2 < ny) & (k2 >= 0) & (k2 < nz)
+        pm = p[m]
+        q = i2[m] + nx * j2[m] + nx * ny * k2[m]
+        off = pos(pm, q)
+        dia = pos(pm, pm)
+        for cell in (pm, q):
+            er += [off, dia]
+            ec += [cell, cell]
+            ev += [np.full(len(pm), -0.5 * fac), np.full(len(pm), 0.5 * fac)]
+        pb = p[~m]
+        er.append(pos(pb, pb)); ec.append(pb); ev.append(np.full(len(pb), fac))
+    J = sp.csr_matrix((np.concatenate(ev), (np.concatenate(er), np.concatenate(ec))), shape=(P.nnz, n))
+    J.sum_duplicates()
+    return P, J
 
 
 def uniform(n, seed=0, nrhs=1, low=-1.0, high=1.0):
