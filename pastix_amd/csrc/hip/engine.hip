@@ -42,7 +42,8 @@ double pa_flops_syrk( double k, double n );
 
 struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
-    int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks                      */
+    int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
+    int64_t trsm_w_beg, trsm_w_end; /* into d_trsm_w_tasks (64-row slabs)   */
     int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
     int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
@@ -50,6 +51,17 @@ struct pa_level_t {
     int32_t ncmax;                /* widest cblk of the level               */
     double  flops[3];             /* diag/trsm/gemm flops (reference model)  */
 };
+
+/* A captured launch sequence and the arguments it was captured with. */
+struct pa_graph_entry {
+    hipGraph_t     graph = nullptr;
+    hipGraphExec_t exec  = nullptr;
+    const void    *key[4] = { nullptr, nullptr, nullptr, nullptr };
+    int64_t        ikey[4] = { 0, 0, 0, 0 };
+    double         dkey = 0.;
+    uint64_t       stamp = 0;
+};
+#define PA_SOLVE_GRAPHS 6
 
 struct pa_device_s {
     int         device;
@@ -70,7 +82,7 @@ struct pa_device_s {
     /* schedule */
     std::vector<pa_level_t> levels;
     int32_t      *d_lev_cblks;
-    int2         *d_trsm_tasks;
+    int2         *d_trsm_tasks, *d_trsm_w_tasks;
     pa_gemm_task *d_gemm_tasks;
     int4         *d_solve_tasks, *d_bsolve_tasks;
     /* matrix */
@@ -81,6 +93,12 @@ struct pa_device_s {
     int   prec, lu;
     void *d_L, *d_U;
     void *d_DT;          /* LL^t: transposed diagonal blocks (solve)     */
+    /* hipGraphs of the factorization and of recent solves (the launch
+     * sequences are long: ~5 launches per level) */
+    int            use_graphs = 1;
+    pa_graph_entry fact_graph;
+    pa_graph_entry solve_graphs[PA_SOLVE_GRAPHS];
+    uint64_t       graph_clock = 0;
     int32_t *d_counters;
     /* work */
     double *d_partial;
@@ -167,6 +185,8 @@ pa_dev_open( pastix_data_t *pd )
     memset( (void *)&dev->S, 0, sizeof( dev->S ) );
     dev->device = 0;
     const char *env = getenv( "PASTIX_AMD_DEVICE" );
+    const char *genv = getenv( "PASTIX_AMD_GRAPHS" );
+    if ( genv && atoi( genv ) == 0 ) dev->use_graphs = 0;
     if ( env ) dev->device = atoi( env );
     PA_HIP( hipSetDevice( dev->device ) );
     PA_HIP( hipStreamCreate( &dev->stream ) );
@@ -184,6 +204,50 @@ pa_dev_open( pastix_data_t *pd )
 }
 
 static void
+pa_graph_drop( pa_graph_entry &g )
+{
+    if ( g.exec ) hipGraphExecDestroy( g.exec );
+    if ( g.graph ) hipGraphDestroy( g.graph );
+    g = pa_graph_entry();
+}
+
+static void
+pa_graphs_drop( pa_device_s *dev )
+{
+    pa_graph_drop( dev->fact_graph );
+    for ( auto &g : dev->solve_graphs ) pa_graph_drop( g );
+}
+
+/* Run `launch()` on the library stream through a cached hipGraph captured
+ * from the same launch sequence (capture on the first call with a new key). */
+template <typename F>
+static int
+pa_graph_run( pa_device_s *dev, pa_graph_entry &g, bool hit, F launch )
+{
+    if ( !hit ) {
+        pa_graph_drop( g );
+        hipGraph_t graph = nullptr;
+        PA_HIP( hipStreamBeginCapture( dev->stream, hipStreamCaptureModeRelaxed ) );
+        int rc = launch();
+        hipError_t ec = hipStreamEndCapture( dev->stream, &graph );
+        if ( rc ) {
+            if ( graph ) hipGraphDestroy( graph );
+            return rc;
+        }
+        PA_HIP( ec );
+        hipError_t ei = hipGraphInstantiate( &g.exec, graph, nullptr, nullptr, 0 );
+        if ( ei != hipSuccess ) {
+            hipGraphDestroy( graph );
+            g.exec = nullptr;
+            PA_HIP( ei );
+        }
+        g.graph = graph;
+    }
+    PA_HIP( hipGraphLaunch( g.exec, dev->stream ) );
+    return PASTIX_SUCCESS;
+}
+
+static void
 pa_free_matrix( pa_device_s *dev )
 {
     hipFree( dev->d_colptr ); hipFree( dev->d_rowtab ); hipFree( dev->d_Lv ); hipFree( dev->d_Uv );
@@ -193,6 +257,7 @@ pa_free_matrix( pa_device_s *dev )
 static void
 pa_free_factors( pa_device_s *dev )
 {
+    pa_graphs_drop( dev );
     hipFree( dev->d_L ); hipFree( dev->d_U ); hipFree( dev->d_DT );
     dev->d_L = dev->d_U = dev->d_DT = NULL;
 }
@@ -200,9 +265,10 @@ pa_free_factors( pa_device_s *dev )
 static void
 pa_free_structure( pa_device_s *dev )
 {
+    pa_graphs_drop( dev );
     void *ptrs[] = { dev->d_c_fcol, dev->d_c_ncol, dev->d_c_stride, dev->d_c_fblok, dev->d_c_coef,
                      dev->d_b_frow, dev->d_b_lrow, dev->d_b_fcblk, dev->d_b_rowoff, dev->d_col2cblk,
-                     dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks,
+                     dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_trsm_w_tasks, dev->d_gemm_tasks,
                      dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info,
                      dev->d_r_blk };
     for ( void *p : ptrs ) hipFree( p );
@@ -211,6 +277,7 @@ pa_free_structure( pa_device_s *dev )
     dev->d_b_frow = dev->d_b_lrow = dev->d_b_fcblk = dev->d_b_rowoff = NULL;
     dev->d_col2cblk = dev->d_perm = dev->d_invp = NULL;
     dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
+    dev->d_trsm_w_tasks = NULL;
     dev->d_bsolve_tasks = NULL;
     dev->d_c_dtoff = dev->d_r_base = NULL;
     dev->d_r_info = NULL;
@@ -225,6 +292,7 @@ pa_dev_close( pastix_data_t *pd )
     if ( !dev ) return;
     hipSetDevice( dev->device );
     hipStreamSynchronize( dev->stream );
+    pa_graphs_drop( dev );
     pa_free_matrix( dev );
     pa_free_factors( dev );
     pa_free_structure( dev );
@@ -363,7 +431,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
      * (64 doubles per lane) leave one wave per SIMD, so 64 x 64 is kept */
     const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     std::vector<int32_t>      lev_cblks;
-    std::vector<int2>         trsm_tasks;
+    std::vector<int2>         trsm_tasks, trsm_w_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gemm_tasks, big_tasks;
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
@@ -388,6 +456,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         pa_level_t &L = dev->levels[l];
         L.cblk_beg  = lev_cblks.size();
         L.trsm_beg  = trsm_tasks.size();
+        L.trsm_w_beg = trsm_w_tasks.size();
         L.gemm_beg  = gemm_tasks.size();
         L.big_beg   = big_tasks.size();
         L.solve_beg = solve_tasks.size();
@@ -415,7 +484,8 @@ pa_dev_setup_structure( pastix_data_t *pd )
         }
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
-            for ( int32_t r = 0; r < R; r += PA_TRSM_WR ) trsm_tasks.push_back( make_int2( k, r ) );
+            for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
+            for ( int32_t r = 0; r < R; r += PA_TRSM_RW ) trsm_w_tasks.push_back( make_int2( k, r ) );
             /* solve update tasks: forward 256 rows x 64 columns; backward
              * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
             for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
@@ -454,6 +524,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
         L.cblk_end  = lev_cblks.size();
         L.trsm_end  = trsm_tasks.size();
+        L.trsm_w_end = trsm_w_tasks.size();
         L.gemm_end  = gemm_tasks.size();
         L.big_end   = big_tasks.size();
         L.solve_end = solve_tasks.size();
@@ -462,6 +533,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_trsm_w_tasks, trsm_w_tasks ) ) ) return rc;
     /* both task lists in one array: the 128-tile tasks follow the 64-tile ones */
     const int64_t nsmall = gemm_tasks.size();
     for ( pa_level_t &L : dev->levels ) {
@@ -474,7 +546,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
 
     /* LDS needs of the diagonal, TRSM and solve kernels */
-    size_t need = std::max( pa_trsm_w_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 4096;
+    size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {
         fprintf( stderr, "pastix_amd: cblks of %d columns exceed the LDS budget (%d bytes); lower IPARM_MAX_BLOCKSIZE\n",
                  colmax, dev->lds_limit );
@@ -599,13 +671,30 @@ pa_factorize_t( pa_device_s *dev, double crit )
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
         if ( ntr > 0 ) {
-            size_t lds_trsm = pa_trsm_w_lds<T>( lev.ncmax );
-            if ( lu ) {
-                k_trsm_w<T, 1><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                k_trsm_w<T, 2><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            /* the staged kernel (64-row slabs) while its slab and coefficient
+             * buffers fit in LDS, the 32-row kernel otherwise; both take the
+             * 32-row task list (a 64-row task starts at every other entry) */
+            const size_t lds_w = pa_trsm_w_lds<T>( lev.ncmax );
+            if ( lds_w + 4096 <= (size_t)dev->lds_limit && lev.ncmax <= PA_TRSM_MAX && lev.trsm_w_end > lev.trsm_w_beg ) {
+                const unsigned nw = (unsigned)( lev.trsm_w_end - lev.trsm_w_beg );
+                const int2    *tw = dev->d_trsm_w_tasks + lev.trsm_w_beg;
+                if ( lu ) {
+                    k_trsm_w<T, 1><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
+                    k_trsm_w<T, 2><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
+                }
+                else {
+                    k_trsm_w<T, 0><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
+                }
             }
             else {
-                k_trsm_w<T, 0><<<(unsigned)ntr, 64, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
+                if ( lu ) {
+                    k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                    k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                }
+                else {
+                    k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                }
             }
             PA_HIP( hipGetLastError() );
         }
@@ -674,8 +763,23 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     PA_HIP( hipStreamSynchronize( dev->stream ) );
     auto t0 = std::chrono::steady_clock::now();
     PA_HIP( hipEventRecord( dev->ev0, dev->stream ) );
-    if ( dev->prec == PA_PREC_FLOAT ) rc = pa_factorize_t<float>( dev, crit );
-    else                              rc = pa_factorize_t<double>( dev, crit );
+    auto launch = [&]() {
+        return ( dev->prec == PA_PREC_FLOAT ) ? pa_factorize_t<float>( dev, crit ) : pa_factorize_t<double>( dev, crit );
+    };
+    if ( dev->use_graphs && !dev->profiling ) {
+        /* the launch sequence depends on the factors' addresses and the
+         * pivoting criterion only */
+        pa_graph_entry &g = dev->fact_graph;
+        bool hit = g.exec && g.key[0] == dev->d_L && g.key[1] == dev->d_U && g.key[2] == dev->d_DT &&
+                   g.ikey[0] == dev->prec && g.ikey[1] == dev->lu && g.dkey == crit;
+        rc = pa_graph_run( dev, g, hit, launch );
+        if ( rc ) return rc;
+        g.key[0] = dev->d_L; g.key[1] = dev->d_U; g.key[2] = dev->d_DT;
+        g.ikey[0] = dev->prec; g.ikey[1] = dev->lu; g.dkey = crit;
+    }
+    else {
+        rc = launch();
+    }
     if ( rc ) return rc;
     PA_HIP( hipEventRecord( dev->ev1, dev->stream ) );
     PA_HIP( hipStreamSynchronize( dev->stream ) );
@@ -804,28 +908,49 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
      * factor's diagonal square, for LL^t the separate transposed array */
     const int64_t *ct_off = lu ? dev->d_c_coef : dev->d_c_dtoff;
     void          *ct     = lu ? ( bwd_side ? dev->d_L : dev->d_U ) : dev->d_DT;
-    if ( dev->prec == PA_PREC_DOUBLE ) {
-        const double *L = (const double *)dev->d_L, *U = (const double *)dev->d_U;
-        return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const double *)ct, ct_off,
-                                   d_x, nrhs, ld );
-    }
-    /* mixed precision: down-convert, FP32 sweeps, up-convert
-     * (reference sopalin/pastix_task_solve.c:583-592, pastix_rhs.c:162-242) */
     int64_t len = dev->n * (int64_t)nrhs;
-    if ( dev->xs_len < len ) {
+    if ( dev->prec != PA_PREC_DOUBLE && dev->xs_len < len ) {
         hipFree( dev->d_xs );
         PA_HIP( hipMalloc( (void **)&dev->d_xs, std::max<int64_t>( len, 1 ) * sizeof( float ) ) );
         dev->xs_len = len;
     }
-    for ( int32_t r = 0; r < nrhs; r++ )
-        k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
-    const float *L = (const float *)dev->d_L, *U = (const float *)dev->d_U;
-    int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const float *)ct, ct_off,
-                                dev->d_xs, nrhs, dev->n );
+    auto launch = [&]() -> int {
+        if ( dev->prec == PA_PREC_DOUBLE ) {
+            const double *L = (const double *)dev->d_L, *U = (const double *)dev->d_U;
+            return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const double *)ct,
+                                       ct_off, d_x, nrhs, ld );
+        }
+        /* mixed precision: down-convert, FP32 sweeps, up-convert
+         * (reference sopalin/pastix_task_solve.c:583-592, pastix_rhs.c:162-242) */
+        for ( int32_t r = 0; r < nrhs; r++ )
+            k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
+        const float *L = (const float *)dev->d_L, *U = (const float *)dev->d_U;
+        int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const float *)ct, ct_off,
+                                    dev->d_xs, nrhs, dev->n );
+        if ( rc ) return rc;
+        for ( int32_t r = 0; r < nrhs; r++ )
+            k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );
+        PA_HIP( hipGetLastError() );
+        return PASTIX_SUCCESS;
+    };
+    if ( !dev->use_graphs ) return launch();
+    /* cached graph per (buffer, nrhs, ld, trans) among the most recent ones */
+    pa_graph_entry *slot = nullptr, *oldest = &dev->solve_graphs[0];
+    for ( auto &g : dev->solve_graphs ) {
+        if ( g.exec && g.key[0] == d_x && g.key[1] == dev->d_xs && g.key[2] == dev->d_L && g.ikey[0] == nrhs &&
+             g.ikey[1] == ld && g.ikey[2] == (int64_t)trans && g.ikey[3] == dev->prec ) {
+            slot = &g;
+            break;
+        }
+        if ( g.stamp < oldest->stamp ) oldest = &g;
+    }
+    bool hit = slot != nullptr;
+    if ( !hit ) slot = oldest;
+    int rc = pa_graph_run( dev, *slot, hit, launch );
     if ( rc ) return rc;
-    for ( int32_t r = 0; r < nrhs; r++ )
-        k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );
-    PA_HIP( hipGetLastError() );
+    slot->key[0] = d_x; slot->key[1] = dev->d_xs; slot->key[2] = dev->d_L;
+    slot->ikey[0] = nrhs; slot->ikey[1] = ld; slot->ikey[2] = (int64_t)trans; slot->ikey[3] = dev->prec;
+    slot->stamp = ++dev->graph_clock;
     return PASTIX_SUCCESS;
 }
 
