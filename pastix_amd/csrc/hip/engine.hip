@@ -43,7 +43,6 @@ double pa_flops_syrk( double k, double n );
 struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
-    int64_t trsm_w_beg, trsm_w_end; /* into d_trsm_w_tasks (64-row slabs)   */
     int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
     int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
@@ -82,7 +81,7 @@ struct pa_device_s {
     /* schedule */
     std::vector<pa_level_t> levels;
     int32_t      *d_lev_cblks;
-    int2         *d_trsm_tasks, *d_trsm_w_tasks;
+    int2         *d_trsm_tasks;
     pa_gemm_task *d_gemm_tasks;
     int4         *d_solve_tasks, *d_bsolve_tasks;
     /* matrix */
@@ -162,8 +161,6 @@ pa_set_lds_limits( int bytes )
 {
     pa_lds_attr( k_trsm<double, 0>, bytes ); pa_lds_attr( k_trsm<double, 1>, bytes ); pa_lds_attr( k_trsm<double, 2>, bytes );
     pa_lds_attr( k_trsm<float, 0>, bytes );  pa_lds_attr( k_trsm<float, 1>, bytes );  pa_lds_attr( k_trsm<float, 2>, bytes );
-    pa_lds_attr( k_trsm_w<double, 0>, bytes ); pa_lds_attr( k_trsm_w<double, 1>, bytes ); pa_lds_attr( k_trsm_w<double, 2>, bytes );
-    pa_lds_attr( k_trsm_w<float, 0>, bytes );  pa_lds_attr( k_trsm_w<float, 1>, bytes );  pa_lds_attr( k_trsm_w<float, 2>, bytes );
     pa_lds_attr_sd<double, false>( bytes ); pa_lds_attr_sd<double, true>( bytes );
     pa_lds_attr_sd<float, false>( bytes );  pa_lds_attr_sd<float, true>( bytes );
     pa_lds_attr( k_solve_diag<double, false, false>, bytes ); pa_lds_attr( k_solve_diag<double, false, true>, bytes );
@@ -268,7 +265,7 @@ pa_free_structure( pa_device_s *dev )
     pa_graphs_drop( dev );
     void *ptrs[] = { dev->d_c_fcol, dev->d_c_ncol, dev->d_c_stride, dev->d_c_fblok, dev->d_c_coef,
                      dev->d_b_frow, dev->d_b_lrow, dev->d_b_fcblk, dev->d_b_rowoff, dev->d_col2cblk,
-                     dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_trsm_w_tasks, dev->d_gemm_tasks,
+                     dev->d_perm, dev->d_invp, dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks,
                      dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info,
                      dev->d_r_blk };
     for ( void *p : ptrs ) hipFree( p );
@@ -277,7 +274,6 @@ pa_free_structure( pa_device_s *dev )
     dev->d_b_frow = dev->d_b_lrow = dev->d_b_fcblk = dev->d_b_rowoff = NULL;
     dev->d_col2cblk = dev->d_perm = dev->d_invp = NULL;
     dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
-    dev->d_trsm_w_tasks = NULL;
     dev->d_bsolve_tasks = NULL;
     dev->d_c_dtoff = dev->d_r_base = NULL;
     dev->d_r_info = NULL;
@@ -431,7 +427,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
      * (64 doubles per lane) leave one wave per SIMD, so 64 x 64 is kept */
     const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     std::vector<int32_t>      lev_cblks;
-    std::vector<int2>         trsm_tasks, trsm_w_tasks;
+    std::vector<int2>         trsm_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gemm_tasks, big_tasks;
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
@@ -456,7 +452,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
         pa_level_t &L = dev->levels[l];
         L.cblk_beg  = lev_cblks.size();
         L.trsm_beg  = trsm_tasks.size();
-        L.trsm_w_beg = trsm_w_tasks.size();
         L.gemm_beg  = gemm_tasks.size();
         L.big_beg   = big_tasks.size();
         L.solve_beg = solve_tasks.size();
@@ -485,7 +480,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
-            for ( int32_t r = 0; r < R; r += PA_TRSM_RW ) trsm_w_tasks.push_back( make_int2( k, r ) );
             /* solve update tasks: forward 256 rows x 64 columns; backward
              * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
             for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
@@ -524,7 +518,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
         L.cblk_end  = lev_cblks.size();
         L.trsm_end  = trsm_tasks.size();
-        L.trsm_w_end = trsm_w_tasks.size();
         L.gemm_end  = gemm_tasks.size();
         L.big_end   = big_tasks.size();
         L.solve_end = solve_tasks.size();
@@ -533,7 +526,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
-    if ( ( rc = pa_upload( &dev->d_trsm_w_tasks, trsm_w_tasks ) ) ) return rc;
     /* both task lists in one array: the 128-tile tasks follow the 64-tile ones */
     const int64_t nsmall = gemm_tasks.size();
     for ( pa_level_t &L : dev->levels ) {
@@ -655,7 +647,8 @@ pa_factorize_t( pa_device_s *dev, double crit )
         size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax * sizeof( T );
         const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
-        if ( !lu && sizeof( T ) == 4 && lev.ncmax > 64 && lev.ncmax <= 16 * PA_DREG_NT ) {
+        const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax > 64 && lev.ncmax <= 16 * PA_DREG_NT;
+        if ( regdiag ) {
             k_diag_potrf_reg<<<ncb, 1024, 0, dev->stream>>>( dev->S, cb, (float *)L, crit, c0, c1 );
         }
         else if ( lev.ncmax > 64 ) {
@@ -671,30 +664,13 @@ pa_factorize_t( pa_device_s *dev, double crit )
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
         if ( ntr > 0 ) {
-            /* the staged kernel (64-row slabs) while its slab and coefficient
-             * buffers fit in LDS, the 32-row kernel otherwise; both take the
-             * 32-row task list (a 64-row task starts at every other entry) */
-            const size_t lds_w = pa_trsm_w_lds<T>( lev.ncmax );
-            if ( lds_w + 4096 <= (size_t)dev->lds_limit && lev.ncmax <= PA_TRSM_MAX && lev.trsm_w_end > lev.trsm_w_beg ) {
-                const unsigned nw = (unsigned)( lev.trsm_w_end - lev.trsm_w_beg );
-                const int2    *tw = dev->d_trsm_w_tasks + lev.trsm_w_beg;
-                if ( lu ) {
-                    k_trsm_w<T, 1><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
-                    k_trsm_w<T, 2><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
-                }
-                else {
-                    k_trsm_w<T, 0><<<nw, 256, lds_w, dev->stream>>>( dev->S, tw, L, U );
-                }
+            size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
+            if ( lu ) {
+                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
             }
             else {
-                size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
-                if ( lu ) {
-                    k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                    k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                }
-                else {
-                    k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                }
+                k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
             }
             PA_HIP( hipGetLastError() );
         }

@@ -718,154 +718,6 @@ k_trsm( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
     pa_trsm_rows<T, MODE>( panel, panel, nc, rows, xoff, xld, reinterpret_cast<T *>( pa_smem_raw ) );
 }
 
-/*
- * Level TRSM, one workgroup of 4 waves per 64-row slab of a cblk's
- * off-diagonal rows (row descriptors from S.r_info):
- *   X[:, J] <- ( X[:, J] - sum_{P<J} X[:, P] C[P, J] ) C[J, J]^{-1}
- * over 16-column blocks J, with C(l, j) = L[j][l] (MODE 0) or D[l][j]
- * (MODE 1, 2) as in pa_trsm_rows.  The slab lives in LDS; the coefficient
- * column block C[0:J+16, J] is staged in LDS (double buffered: block J+1 is
- * loaded while block J is processed).  Each wave owns 16 rows: the block
- * products run on MFMA 16x16x4 (register tile = X[:, J]^T in accumulator
- * layout, A operand = C[P, J]^T, B operand = X[:, P]^T, both from LDS), the
- * 16 x 16 triangle is solved by the 16 row lanes column by column (x_c
- * scaled, then x_j -= x_c C(c, j) for j > c: the reference's order).
- */
-#define PA_TRSM_RW  64  /* rows per workgroup               */
-#define PA_TRSM_MAX 512 /* widest cblk of the staged kernel */
-#define PA_TRSM_PF  ( ( PA_TRSM_MAX + 16 ) * 16 / 256 + 1 )
-template <typename T> __host__ __device__ constexpr size_t pa_trsm_w_lds( int32_t nc )
-{
-    return ( (size_t)PA_TRSM_RW * ( nc + 1 ) + 2 * (size_t)( nc + 16 ) * 16 ) * sizeof( T );
-}
-
-template <typename T, int MODE>
-__device__ __forceinline__ T
-pa_trsm_coef( const T *D, int32_t nc, int32_t l, int32_t j )
-{
-    return MODE == 0 ? D[j + (int64_t)l * nc] : D[l + (int64_t)j * nc];
-}
-
-template <typename T, int MODE>
-__global__ __launch_bounds__( 256 ) void
-k_trsm_w( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
-{
-    typedef typename pa_mfma<T>::acc_t acc_t;
-    extern __shared__ unsigned char pa_smem_raw[];
-    __shared__ int64_t xoff[PA_TRSM_RW];
-    __shared__ int32_t xld[PA_TRSM_RW];
-    const int2    tk = tasks[blockIdx.x];
-    const int32_t k = tk.x, r0 = tk.y, nc = S.c_ncol[k], R = S.c_stride[k] - nc;
-    const int32_t rows = min( PA_TRSM_RW, R - r0 ), ldx = nc + 1;
-    T            *X  = reinterpret_cast<T *>( pa_smem_raw );    /* [64][nc + 1]       */
-    T            *Cs = X + PA_TRSM_RW * ldx;                     /* [2][nc + 16][16]   */
-    T            *C  = ( MODE == 2 ? Uc : Lc );
-    const T      *D  = C + S.c_coef[k];
-    const int     t = threadIdx.x, lane = t & 63, wave = t >> 6, rl = lane & 15, g = lane >> 4;
-    const int32_t xr = 16 * wave + rl; /* this lane's slab row */
-
-    if ( t < PA_TRSM_RW ) {
-        if ( t < rows ) {
-            const int4 ri = S.r_info[S.r_base[k] + r0 + t];
-            xoff[t] = pa_rinfo_off( ri );
-            xld[t]  = ri.y;
-        }
-        else {
-            xoff[t] = 0;
-            xld[t]  = 0;
-        }
-    }
-    /* stage coefficient block 0 */
-    {
-        const int32_t kmax = min( nc, 16 );
-        for ( int32_t e = t; e < kmax * 16; e += 256 ) {
-            const int32_t a = MODE == 0 ? ( e & 15 ) : e / kmax, kk = MODE == 0 ? ( e >> 4 ) : e % kmax;
-            Cs[kk * 16 + a] = ( a < nc ) ? pa_trsm_coef<T, MODE>( D, nc, kk, a ) : T( 0 );
-        }
-    }
-    __syncthreads();
-    for ( int32_t e = t; e < PA_TRSM_RW * nc; e += 256 ) {
-        const int32_t row = e % PA_TRSM_RW, c = e / PA_TRSM_RW;
-        X[row * ldx + c] = ( row < rows ) ? C[xoff[row] + (int64_t)c * xld[row]] : T( 0 );
-    }
-    __syncthreads();
-
-    const int32_t nb = ( nc + 15 ) >> 4;
-    for ( int32_t j = 0; j < nb; j++ ) {
-        const int32_t cj = 16 * j;
-        T *Cb = Cs + ( j & 1 ) * ( nc + 16 ) * 16;
-        /* prefetch coefficient block j + 1 (rows 0 .. cj + 32) */
-        T            pf[PA_TRSM_PF];
-        const int32_t cn = cj + 16, kmaxn = min( nc, cn + 16 );
-        const bool    more = j + 1 < nb;
-#pragma unroll
-        for ( int u = 0; u < PA_TRSM_PF; u++ ) {
-            const int32_t e = t + 256 * u;
-            pf[u] = T( 0 );
-            if ( more && e < kmaxn * 16 ) {
-                const int32_t a = MODE == 0 ? ( e & 15 ) : e / kmaxn, kk = MODE == 0 ? ( e >> 4 ) : e % kmaxn;
-                if ( cn + a < nc ) pf[u] = pa_trsm_coef<T, MODE>( D, nc, kk, cn + a );
-            }
-        }
-        /* X[:, J]^T -= C[P, J]^T X[:, P]^T over P < J */
-        acc_t acc;
-#pragma unroll
-        for ( int r = 0; r < 4; r++ ) {
-            const int32_t a = cj + pa_mfma<T>::row( lane, r );
-            acc[r] = ( a < nc ) ? X[xr * ldx + a] : T( 0 );
-        }
-#pragma unroll 2
-        for ( int32_t p = 0; p < j; p++ ) {
-#pragma unroll
-            for ( int q = 0; q < 4; q++ ) {
-                const int32_t kk = 16 * p + 4 * q + g;
-                acc = pa_mfma<T>::run( -Cb[kk * 16 + rl], X[xr * ldx + kk], acc );
-            }
-        }
-#pragma unroll
-        for ( int r = 0; r < 4; r++ ) {
-            const int32_t a = cj + pa_mfma<T>::row( lane, r );
-            if ( a < nc ) X[xr * ldx + a] = acc[r];
-        }
-        __syncthreads();
-        /* 16 x 16 triangle, one lane per row */
-        if ( g == 0 ) {
-            const int32_t w = min( 16, nc - cj );
-            T x[16];
-#pragma unroll
-            for ( int c = 0; c < 16; c++ ) x[c] = ( c < w ) ? X[xr * ldx + cj + c] : T( 0 );
-#pragma unroll
-            for ( int c = 0; c < 16; c++ ) {
-                if ( c < w ) {
-                    if ( MODE != 2 ) x[c] *= T( 1 ) / Cb[( cj + c ) * 16 + c];
-#pragma unroll
-                    for ( int jj = c + 1; jj < 16; jj++ )
-                        if ( jj < w ) x[jj] -= x[c] * Cb[( cj + c ) * 16 + jj];
-                }
-            }
-#pragma unroll
-            for ( int c = 0; c < 16; c++ )
-                if ( c < w ) X[xr * ldx + cj + c] = x[c];
-        }
-        if ( more ) {
-            T *Cn = Cs + ( ( j + 1 ) & 1 ) * ( nc + 16 ) * 16;
-#pragma unroll
-            for ( int u = 0; u < PA_TRSM_PF; u++ ) {
-                const int32_t e = t + 256 * u;
-                if ( e < kmaxn * 16 ) {
-                    const int32_t a = MODE == 0 ? ( e & 15 ) : e / kmaxn, kk = MODE == 0 ? ( e >> 4 ) : e % kmaxn;
-                    Cn[kk * 16 + a] = pf[u];
-                }
-            }
-        }
-        __syncthreads();
-    }
-    for ( int32_t e = t; e < PA_TRSM_RW * nc; e += 256 ) {
-        const int32_t row = e % PA_TRSM_RW, c = e / PA_TRSM_RW;
-        if ( row < rows ) C[xoff[row] + (int64_t)c * xld[row]] = X[row * ldx + c];
-    }
-}
-
 /* Per-call description of a run of blocks (reference-signature entry points):
  * block i holds rows [pref[i], pref[i+1]) of the concatenated rows, starts at
  * off[i] with leading dimension ld[i]; for targets, toff/tld describe where
@@ -935,7 +787,7 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 /* scatter store is coalesced.                                               */
 /* ------------------------------------------------------------------------- */
 #define PA_GT  64
-#define PA_GKC 16
+#define PA_GKC 32
 
 template <typename T> struct pa_mfma;
 
@@ -944,7 +796,11 @@ template <typename T> struct pa_mfma;
  * Tile core shared by the level-batched kernel and the reference-signature
  * entry points.  TILE x TILE output (TILE = 64 or 128), 4 waves in a 2 x 2
  * arrangement, K in chunks of PA_GKC staged in LDS, the next chunk prefetched
- * into registers while the current one feeds the MFMAs.  Row maps (in LDS)
+ * into registers while the current one feeds the MFMAs (a chunk of 32 keeps
+ * the MFMAs busy for longer than the global load latency).  The loads are
+ * unconditional from clamped in-range addresses (empty rows are zeroed when
+ * the chunk is stored to LDS), so no per-load branch holds the prefetch back.
+ * Row maps (in LDS)
  * must be filled by the caller:
  *   a_off/a_ld: source of the TILE rho rows inside Asrc (a_ld = 0: empty row)
  *   b_off/b_ld: source of the TILE sigma rows inside Bsrc
@@ -958,7 +814,7 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[PA_GKC][TILE + pa_gpad<T>::value], T ( *Bs )[PA_GKC][TILE + pa_gpad<T>::value] )
+              T ( *As )[TILE + pa_gpad<T>::value], T ( *Bs )[TILE + pa_gpad<T>::value] )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int NB  = TILE / 32;        /* MFMA blocks per wave per dimension */
@@ -978,33 +834,32 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
             for ( int r = 0; r < 4; r++ ) acc[i][j][r] = T( 0 );
 
     const int lrow = t % TILE, lk = t / TILE;
-    const int64_t aof = a_off[lrow], bof = b_off[lrow];
     const int32_t ald = a_ld[lrow], bld = b_ld[lrow];
+    /* an empty row reads row 0 of its operand (in range) and stores zeros */
+    const int64_t aof = ald > 0 ? a_off[lrow] : 0, bof = bld > 0 ? b_off[lrow] : 0;
+    const int32_t alds = ald > 0 ? ald : 1, blds = bld > 0 ? bld : 1;
     T pa[NQ], pb[NQ];
 
 #pragma unroll
     for ( int q = 0; q < NQ; q++ ) {
-        int32_t kk = lk + KL * q;
-        pa[q] = ( ald > 0 && kk < K ) ? Asrc[aof + (int64_t)kk * ald] : T( 0 );
-        pb[q] = ( bld > 0 && kk < K ) ? Bsrc[bof + (int64_t)kk * bld] : T( 0 );
+        const int32_t kk = min( lk + KL * q, K - 1 );
+        pa[q] = Asrc[aof + (int64_t)kk * alds];
+        pb[q] = Bsrc[bof + (int64_t)kk * blds];
     }
-    /* double-buffered K loop: chunk i is computed from LDS buffer i & 1 while
-     * chunk i + 1 is loaded into registers and then stored into the other
-     * buffer; one barrier per chunk */
+    for ( int32_t k0 = 0; k0 < K; k0 += PA_GKC ) {
 #pragma unroll
-    for ( int q = 0; q < NQ; q++ ) {
-        As[0][lk + KL * q][lrow] = pa[q];
-        Bs[0][lk + KL * q][lrow] = pb[q];
-    }
-    __syncthreads();
-    for ( int32_t k0 = 0, buf = 0; k0 < K; k0 += PA_GKC, buf ^= 1 ) {
-        const bool more = k0 + PA_GKC < K;
-        if ( more ) {
+        for ( int q = 0; q < NQ; q++ ) {
+            const bool kin = k0 + lk + KL * q < K;
+            As[lk + KL * q][lrow] = ( ald > 0 && kin ) ? pa[q] : T( 0 );
+            Bs[lk + KL * q][lrow] = ( bld > 0 && kin ) ? pb[q] : T( 0 );
+        }
+        __syncthreads();
+        if ( k0 + PA_GKC < K ) {
 #pragma unroll
             for ( int q = 0; q < NQ; q++ ) {
-                int32_t kk = k0 + PA_GKC + lk + KL * q;
-                pa[q] = ( ald > 0 && kk < K ) ? Asrc[aof + (int64_t)kk * ald] : T( 0 );
-                pb[q] = ( bld > 0 && kk < K ) ? Bsrc[bof + (int64_t)kk * bld] : T( 0 );
+                const int32_t kk = min( k0 + PA_GKC + lk + KL * q, K - 1 );
+                pa[q] = Asrc[aof + (int64_t)kk * alds];
+                pb[q] = Bsrc[bof + (int64_t)kk * blds];
             }
         }
 #pragma unroll
@@ -1012,20 +867,13 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
             int kk = ks * 4 + ( lane >> 4 );
             T xa[NB], yb[NB];
 #pragma unroll
-            for ( int mi = 0; mi < NB; mi++ ) xa[mi] = Bs[buf][kk][wn * ( TILE / 2 ) + mi * 16 + ( lane & 15 )];
+            for ( int mi = 0; mi < NB; mi++ ) xa[mi] = Bs[kk][wn * ( TILE / 2 ) + mi * 16 + ( lane & 15 )];
 #pragma unroll
-            for ( int mj = 0; mj < NB; mj++ ) yb[mj] = As[buf][kk][wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 )];
+            for ( int mj = 0; mj < NB; mj++ ) yb[mj] = As[kk][wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 )];
 #pragma unroll
             for ( int mi = 0; mi < NB; mi++ )
 #pragma unroll
                 for ( int mj = 0; mj < NB; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
-        }
-        if ( more ) {
-#pragma unroll
-            for ( int q = 0; q < NQ; q++ ) {
-                As[buf ^ 1][lk + KL * q][lrow] = pa[q];
-                Bs[buf ^ 1][lk + KL * q][lrow] = pb[q];
-            }
         }
         __syncthreads();
     }
@@ -1058,8 +906,8 @@ __global__ __launch_bounds__( 256 ) void
 k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
 {
     constexpr int PAD = pa_gpad<T>::value;
-    __shared__ T As[2][PA_GKC][TILE + PAD];
-    __shared__ T Bs[2][PA_GKC][TILE + PAD];
+    __shared__ T As[PA_GKC][TILE + PAD];
+    __shared__ T Bs[PA_GKC][TILE + PAD];
     __shared__ int64_t a_off[TILE], b_off[TILE], c_off[TILE];
     __shared__ int32_t a_ld[TILE], b_ld[TILE], c_ld[TILE], c_col[TILE], a_blk[TILE], b_blk[TILE], grow_s[TILE];
     /* first rows / last rows / row offsets of the target cblk's blocks */
@@ -1152,8 +1000,8 @@ template <typename T>
 __global__ __launch_bounds__( 256 ) void
 k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_rows Br, int32_t ccol0 )
 {
-    __shared__ T As[2][PA_GKC][PA_GT + pa_gpad<T>::value];
-    __shared__ T Bs[2][PA_GKC][PA_GT + pa_gpad<T>::value];
+    __shared__ T As[PA_GKC][PA_GT + pa_gpad<T>::value];
+    __shared__ T Bs[PA_GKC][PA_GT + pa_gpad<T>::value];
     __shared__ int64_t a_off[PA_GT], b_off[PA_GT], c_off[PA_GT];
     __shared__ int32_t a_ld[PA_GT], b_ld[PA_GT], c_ld[PA_GT], c_col[PA_GT], a_blk[PA_GT], b_blk[PA_GT];
     const int t = threadIdx.x;
