@@ -634,11 +634,6 @@ pa_factorize_t( pa_device_s *dev, double crit )
 {
     T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
     const int lu = dev->lu;
-    if ( dev->profiling && dev->prof_ev.size() < 6 * dev->levels.size() ) {
-        size_t old = dev->prof_ev.size();
-        dev->prof_ev.resize( 6 * dev->levels.size() );
-        for ( size_t i = old; i < dev->prof_ev.size(); i++ ) PA_HIP( hipEventCreate( &dev->prof_ev[i] ) );
-    }
     for ( size_t li = 0; li < dev->levels.size(); li++ ) {
         const pa_level_t &lev = dev->levels[li];
         int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
@@ -742,16 +737,23 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     auto launch = [&]() {
         return ( dev->prec == PA_PREC_FLOAT ) ? pa_factorize_t<float>( dev, crit ) : pa_factorize_t<double>( dev, crit );
     };
+    if ( dev->profiling && dev->prof_ev.size() < 6 * dev->levels.size() ) {
+        /* the per-level timing events exist before any capture */
+        size_t old = dev->prof_ev.size();
+        dev->prof_ev.resize( 6 * dev->levels.size() );
+        for ( size_t i = old; i < dev->prof_ev.size(); i++ ) PA_HIP( hipEventCreate( &dev->prof_ev[i] ) );
+    }
     if ( dev->use_graphs && !dev->profiling ) {
         /* the launch sequence depends on the factors' addresses and the
-         * pivoting criterion only */
+         * pivoting criterion; the per-level timing events are recorded by the
+         * eager launches only (captured events cannot be timed) */
         pa_graph_entry &g = dev->fact_graph;
         bool hit = g.exec && g.key[0] == dev->d_L && g.key[1] == dev->d_U && g.key[2] == dev->d_DT &&
-                   g.ikey[0] == dev->prec && g.ikey[1] == dev->lu && g.dkey == crit;
+                   g.ikey[0] == dev->prec && g.ikey[1] == dev->lu && g.ikey[2] == dev->profiling && g.dkey == crit;
         rc = pa_graph_run( dev, g, hit, launch );
         if ( rc ) return rc;
         g.key[0] = dev->d_L; g.key[1] = dev->d_U; g.key[2] = dev->d_DT;
-        g.ikey[0] = dev->prec; g.ikey[1] = dev->lu; g.dkey = crit;
+        g.ikey[0] = dev->prec; g.ikey[1] = dev->lu; g.ikey[2] = dev->profiling; g.dkey = crit;
     }
     else {
         rc = launch();
