@@ -1,17 +1,26 @@
 """Benchmark of the factor+solve path on one MI355X.
 
-Workload (the single-GPU configuration of the metric): 3D 7-point Laplacian
-on a 100^3 grid (n = 1e6, FP64 SPD), LL^t with grid nested dissection, one RHS
-b = A x_true with x_true ~ U[-1,1] (numpy default_rng, seed 0).
+Default workload (the single-GPU configuration of the metric): 3D 7-point
+Laplacian on a 100^3 grid (n = 1e6, FP64 SPD), LL^t with grid nested
+dissection, one RHS b = A x_true with x_true ~ U[-1,1] (numpy default_rng,
+seed 0).
 
 A step is one numerical pass with the matrix and b already resident in HBM:
-coeftab fill from the device CSC + factorization + solve + refinement to
-||r||_2/||b||_2 <= 1e-12.  The timed steps run the FP32-factor + FP64-GMRES
-path (the headline); the plain FP64 path is timed the same way alongside.
-value = factorization flops of the reference model (THFLOPS, 1e9 scale) per
-second of a full FP32+IR step.
+coeftab fill from the device CSC + factorization + solve + FP64 GMRES
+refinement to ||r||_2/||b||_2 <= 1e-13.  The timed steps run the FP32-factor
++ FP64-GMRES path (the headline); the plain FP64 path is timed the same way
+alongside.  value = factorization flops of the reference model (THFLOPS, 1e9
+scale) per second of a full FP32+IR step.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 100]
+Other workloads (--workload):
+  convdiff  3D 27-point convection-diffusion 80^3, LU with static pivoting,
+            FP32+GMRES vs FP64 (same step and line shape);
+  batch     256 uneven 2D Laplacians (n_i ~ U{64..512}, seed 1) factored
+            and solved (16 RHS each, FP64) in one pass;
+  anisodiff forward + backward of the differentiable solve on the
+            anisotropic diffusion 64^3 (FP64 LL^t).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--grid N]
 """
 import argparse
 import ctypes as C
@@ -31,9 +40,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_instance(pa, A, grid, mixed, eps_refine):
+def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=True):
     ip, dp = pa.initParam()
-    ip[pa.iparm.factorization] = pa.factotype.LLT
+    ip[pa.iparm.factorization] = factotype
     ip[pa.iparm.verbose] = pa.verbose.Not
     ip[pa.iparm.refinement] = pa.refine.GMRES
     # GMRES stops on ||r||_2/||b||_2 (reference refinement/z_refine_gmres.c:169);
@@ -42,23 +51,27 @@ def make_instance(pa, A, grid, mixed, eps_refine):
     if mixed:
         ip[pa.iparm.mixed] = 1
     h = pa.init(ip, dp)
-    spm = pa.spmatrix(A, mtxtype=pa.mtxtype.Symmetric)
-    o = pa.order.grid(*grid)
+    mt = pa.mtxtype.General if factotype == pa.factotype.LU else pa.mtxtype.Symmetric
+    spm = pa.spmatrix(A, mtxtype=mt)
+    o = order if order is not None else pa.order.grid(*grid)
     t0 = time.perf_counter()
     pa.task_analyze(h, spm, o)
     t_an = time.perf_counter() - t0
     t0 = time.perf_counter()
     pa.task_numfact(h, spm)          # CSC permutation + upload + first factorization
     t_nf = time.perf_counter() - t0
-    return dict(h=h, ip=ip, dp=dp, spm=spm, order=o, t_analyze=t_an, t_first_numfact=t_nf)
+    return dict(h=h, ip=ip, dp=dp, spm=spm, order=o, t_analyze=t_an, t_first_numfact=t_nf, refine=refine)
 
 
 def step(pa, lib, inst, d_b, d_x, stages=None):
     """One numerical pass on HBM-resident data; returns refinement iterations.
+    d_b / d_x are (n) or column-major (nrhs, n) device tensors.
     With `stages` (a dict), the stages are separated by device syncs and
-    timed: fill, factor, solve, refine (ms)."""
+    timed one by one."""
     import torch
     h = inst["h"]
+    n = d_b.shape[-1]
+    nrhs = 1 if d_b.dim() == 1 else d_b.shape[0]
 
     def mark(name, t0):
         if stages is None:
@@ -75,13 +88,14 @@ def step(pa, lib, inst, d_b, d_x, stages=None):
     assert rc == 0, rc
     t0 = mark("factor", t0)
     d_x.copy_(d_b)
-    rc = lib.pastix_amd_solve_dev(h, pa.trans.NoTrans, 1, C.c_void_p(d_x.data_ptr()), d_x.numel())
+    rc = lib.pastix_amd_solve_dev(h, pa.trans.NoTrans, nrhs, C.c_void_p(d_x.data_ptr()), n)
     assert rc == 0, rc
     t0 = mark("solve", t0)
-    it = lib.pastix_amd_refine_dev(h, 1, C.c_void_p(d_b.data_ptr()), d_b.numel(),
-                                   C.c_void_p(d_x.data_ptr()), d_x.numel())
-    assert it >= 0, it
-    mark("refine", t0)
+    it = 0
+    if inst["refine"]:
+        it = lib.pastix_amd_refine_dev(h, nrhs, C.c_void_p(d_b.data_ptr()), n, C.c_void_p(d_x.data_ptr()), n)
+        assert it >= 0, it
+        mark("refine", t0)
     return it
 
 
@@ -132,21 +146,24 @@ def measured_traffic():
     return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, ROOT)}
 
 
-def cpu_baseline(pa, problems, seconds_target=20.0):
-    """The oracle's sequential LL^t (restated PaStiX CPU path, one core, its
-    own dense loops) on a bounded sample of the same workload family: the
-    7-point Laplacian on the largest cube that fits the time budget."""
+def cpu_baseline(pa, problems, family="laplacian", seconds_target=20.0):
+    """The oracle's sequential factorization (restated PaStiX CPU path, one
+    core, its own dense loops) on a bounded sample of the same workload
+    family: the largest cube of the family's stencil that fits the time
+    budget."""
     import numpy as np
     import scipy.sparse as sp
     from oracle import pyoracle as orc
+    lu = family == "convdiff"
+    sizes = (28, 36, 44) if lu else (30, 40, 50)
     res = None
-    for N in (30, 40, 50):
-        A = problems.laplacian(N, N, N)
+    for N in sizes:
+        A = problems.convection_diffusion_27(N, N, N) if lu else problems.laplacian(N, N, N)
         ip, dp = pa.initParam()
-        ip[pa.iparm.factorization] = pa.factotype.LLT
+        ip[pa.iparm.factorization] = pa.factotype.LU if lu else pa.factotype.LLT
         ip[pa.iparm.verbose] = pa.verbose.Not
         h = pa.init(ip, dp)
-        spm = pa.spmatrix(A, mtxtype=pa.mtxtype.Symmetric)
+        spm = pa.spmatrix(A, mtxtype=pa.mtxtype.General if lu else pa.mtxtype.Symmetric)
         pa.task_analyze(h, spm, pa.order.grid(N, N, N))
         cblk, blok, perm, invp, dims = pa.get_structure(h)
         th = float(dp[pa.dparm.fact_thflops])
@@ -154,15 +171,276 @@ def cpu_baseline(pa, problems, seconds_target=20.0):
         S = orc.Structure(cblk, blok)
         crit = np.finfo(np.float64).eps / 2 * sp.linalg.norm(A)
         t0 = time.perf_counter()
-        orc.factorize(S, A, perm, 0, crit)
+        orc.factorize(S, A, perm, 2 if lu else 0, crit)
         t = time.perf_counter() - t0
         res = {"value": th / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
-               "sample": "oracle sequential LL^t (fill+factor) of the 7-point Laplacian %d^3, "
-                         "n=%d, %.3e flops, %.2f s" % (N, N ** 3, th, t)}
+               "sample": "oracle sequential %s (fill+factor) of the %s %d^3, n=%d, %.3e flops, %.2f s" %
+                         ("LU" if lu else "LL^t", "27-point convection-diffusion" if lu else "7-point Laplacian",
+                          N, N ** 3, th, t)}
         log("cpu baseline", res["sample"])
         if t > seconds_target / 4:
             break
     return res
+
+
+def gemm_roofline(prof, steps, precision):
+    """Roofline figures of the level scatter-GEMM class from the per-level HIP
+    events (prof: the pastix_amd_profile accumulators)."""
+    gemm_ms = prof[2] / steps
+    gemm_launches = prof[5] / steps
+    gemm_flops = prof[8]
+    achieved = gemm_flops / (gemm_ms * 1e-3)
+    tname = "float" if precision == "fp32" else "double"
+    return {"bound": "mfma",
+            "kernel": "k_gemm_level<%s,*> (level scatter-GEMM, all tile sizes)" % tname,
+            "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK[precision], "traffic": None,
+            "launches_per_step": gemm_launches, "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
+            "algorithmic_flops_per_launch": gemm_flops / max(gemm_launches, 1),
+            "algorithmic_flops_per_step": gemm_flops}
+
+
+def kernel_ms(prof, steps):
+    return {"diag": prof[0] / steps, "trsm": prof[1] / steps, "gemm": prof[2] / steps}
+
+
+def run_pair(args, torch, pa, lib, A, grid, factotype, d_b, d_x, x_true=None):
+    """FP32+IR and plain FP64 (unless --no-fp64) on the same matrix and RHS."""
+    import numpy as np
+    from pastix_amd import problems
+    results = {}
+    b = d_b.cpu().numpy()
+    for mode in (["fp32ir"] if args.no_fp64 else ["fp32ir", "fp64"]):
+        log("mode", mode, ": analysis + first factorization")
+        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps, factotype=factotype)
+        th = float(inst["dp"][pa.dparm.fact_thflops])
+        t, iters, prof, stages, t_prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup)
+        x = d_x.cpu().numpy()
+        berr = problems.backward_error(A, x, b)
+        ferr = np.abs(x - x_true).max() / np.abs(x_true).max() if x_true is not None else None
+        results[mode] = dict(t=t, iters=iters, prof=prof, th=th, berr=berr, ferr=ferr, stages=stages, t_prof=t_prof,
+                             t_analyze=inst["t_analyze"], fact_s=float(inst["dp"][pa.dparm.fact_time]),
+                             x=x if mode == "fp64" else None)
+        log("mode %s: %.1f ms/step, berr %.2e, ferr %s, iters %s, fact %.1f ms" %
+            (mode, t * 1e3, berr, ferr, iters, results[mode]["fact_s"] * 1e3))
+        if mode == "fp32ir":
+            results[mode]["x32"] = x
+        pa.finalize(inst["h"])
+    return results
+
+
+def pair_line(args, pa, results, metric_config, family):
+    """The contract JSON line of an FP32+IR vs FP64 workload."""
+    import numpy as np
+    r = results["fp32ir"]
+    steps = args.steps
+    roofline = gemm_roofline(r["prof"], steps, "fp32")
+    roofline["events_timed_ms_per_step"] = r["t_prof"] * 1e3
+    if family == "laplacian":
+        roofline["kernel"] = "k_gemm_level<float,128> + k_gemm_level<float,64> (level scatter-GEMM)"
+        traffic = measured_traffic()
+        if traffic:
+            roofline["traffic"] = traffic["bytes_per_launch"]
+            roofline["traffic_source"] = traffic["source"]
+    line = {
+        "metric": "factor+solve time & GFLOP/s, FP32+IR vs FP64 at backward error <=1e-12",
+        "value": r["th"] / r["t"] / 1e9,
+        "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": r["t"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": metric_config,
+        "berr": r["berr"], "ferr": r["ferr"], "refine_iters": r["iters"][-1],
+        "kernel_ms_per_step": kernel_ms(r["prof"], steps),
+        "stage_ms": r["stages"],
+        "analysis_s": r["t_analyze"],
+        "roofline": roofline,
+    }
+    if "fp64" in results:
+        r64 = results["fp64"]
+        line["fp64_ms_per_step"] = r64["t"] * 1e3
+        line["fp64_gflops"] = r64["th"] / r64["t"] / 1e9
+        line["fp64_berr"] = r64["berr"]
+        line["fp64_ferr"] = r64["ferr"]
+        line["speedup_fp32ir_vs_fp64"] = r64["t"] / r["t"]
+        line["fp64_kernel_ms_per_step"] = kernel_ms(r64["prof"], steps)
+        line["fp64_roofline"] = gemm_roofline(r64["prof"], steps, "fp64")
+        line["fp64_stage_ms"] = r64["stages"]
+        # forward error of the FP32+IR solution against the FP64 factorization's
+        x64 = r64["x"]
+        line["ferr_vs_fp64_solution"] = float(np.abs(r["x32"] - x64).max() / np.abs(x64).max())
+    if not args.no_cpu:
+        from pastix_amd import problems
+        cb = cpu_baseline(pa, problems, family)
+        line["cpu_baseline"] = cb
+        line["speedup_vs_cpu_baseline"] = line["value"] / cb["value"]
+    return line
+
+
+def workload_laplacian(args, torch, pa, lib, dev):
+    from pastix_amd import problems
+    N = args.grid or 100
+    grid = (N, N, N)
+    log("generating 7-point Laplacian %d^3" % N)
+    A = problems.laplacian(*grid)
+    n = A.shape[0]
+    xt = problems.uniform(n, seed=0)
+    d_b = torch.from_numpy(A @ xt).to(dev)
+    d_x = torch.empty_like(d_b)
+    results = run_pair(args, torch, pa, lib, A, grid, pa.factotype.LLT, d_b, d_x, x_true=xt)
+    cfg = {"workload": "3D 7-point Laplacian %d^3 (n=%d), LL^t, grid nested dissection, "
+                       "1 RHS b=A*x_true, x_true~U[-1,1] seed 0" % (N, n),
+           "thflops": results["fp32ir"]["th"], "parallelism": "single GPU"}
+    return pair_line(args, pa, results, cfg, "laplacian")
+
+
+def workload_convdiff(args, torch, pa, lib, dev):
+    from pastix_amd import problems
+    N = args.grid or 80
+    grid = (N, N, N)
+    log("generating 27-point convection-diffusion %d^3" % N)
+    A = problems.convection_diffusion_27(*grid)
+    n = A.shape[0]
+    xt = problems.uniform(n, seed=0)
+    d_b = torch.from_numpy(A @ xt).to(dev)
+    d_x = torch.empty_like(d_b)
+    results = run_pair(args, torch, pa, lib, A, grid, pa.factotype.LU, d_b, d_x, x_true=xt)
+    cfg = {"workload": "3D 27-point convection-diffusion %d^3 (n=%d, nnz=%d), diffusion 1, upwind velocity "
+                       "(0.3,0.2,0.1), LU with static pivoting, grid nested dissection, GMRES, "
+                       "1 RHS b=A*x_true, x_true~U[-1,1] seed 0" % (N, n, A.nnz),
+           "thflops": results["fp32ir"]["th"], "parallelism": "single GPU"}
+    return pair_line(args, pa, results, cfg, "convdiff")
+
+
+def workload_batch(args, torch, pa, lib, dev):
+    """Many uneven SPD systems factored and solved in one pass (FP64, 16 RHS
+    each): the block-diagonal forest of 2D Laplacians n_i x n_i, n_i ~
+    U{64..512}, diagonal shift ~ U[0,1], seed 1."""
+    import numpy as np
+    from pastix_amd import batch, problems
+    count = args.count or 256
+    t0 = time.perf_counter()
+    mats, grids = batch.uneven_laplacians(count=count, low=64, high=512, seed=1)
+    log("generated %d systems, sum n = %d (%.1f s)" % (count, sum(m.shape[0] for m in mats),
+                                                       time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    s = batch.BatchedSolver(mats, grids, mixed=False)
+    t_an = time.perf_counter() - t0
+    log("analysis %.1f s" % t_an)
+    n = int(s.offsets[-1])
+    nrhs = 16
+    rng = np.random.default_rng(1)
+    # column-major (n x nrhs): row r of the (nrhs, n) tensor is right-hand side r
+    B = np.empty((nrhs, n))
+    for i, m in enumerate(mats):
+        B[:, s.offsets[i]:s.offsets[i + 1]] = rng.uniform(-1, 1, (mats[i].shape[0], nrhs)).T
+    d_b = torch.from_numpy(B).to(dev)
+    d_x = torch.empty_like(d_b)
+    del B
+    s.factorize()
+    inst = dict(h=s.h, refine=False)
+    t, iters, prof, stages, t_prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup)
+    th = float(s.dparm[pa.dparm.fact_thflops])
+    X = d_x.cpu().numpy()
+    Bh = d_b.cpu().numpy()
+    berr = 0.0
+    for i, m in enumerate(mats):
+        sl = slice(s.offsets[i], s.offsets[i + 1])
+        berr = max(berr, problems.backward_error(m, X[:, sl].T, Bh[:, sl].T))
+    log("batch: %.1f ms/pass, max berr %.2e" % (t * 1e3, berr))
+    steps = args.steps
+    line = {
+        "metric": "factor+solve time & GFLOP/s, FP64 batched pass of many uneven systems",
+        "value": th / t / 1e9, "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d 2D 5-point Laplacians n_i x n_i, n_i~U{64..512} + shift~U[0,1] seed 1 "
+                               "(sum n=%d), LL^t FP64, 16 RHS~U[-1,1] each, one batched pass" % (count, n),
+                   "thflops": th, "parallelism": "single GPU"},
+        "berr_max": berr, "systems": count, "nrhs": nrhs,
+        "kernel_ms_per_step": kernel_ms(prof, steps), "stage_ms": stages, "analysis_s": t_an,
+        "roofline": gemm_roofline(prof, steps, "fp64"),
+    }
+    s.close()
+    return line
+
+
+def workload_anisodiff(args, torch, pa, lib, dev):
+    """Differentiable solve: a step is one forward (device value update,
+    FP64 LL^t factorization, solve, refinement) and one backward (adjoint
+    solve, sampled outer product on the pattern) of loss = 0.5||x - x_t||^2
+    for the anisotropic diffusion 64^3, theta = exp(N(0,1)) seed 2, ratio 10
+    in z, b ~ U[-1,1] seed 3, x_t ~ U[-1,1] seed 4."""
+    import numpy as np
+    import scipy.sparse as sp
+    from pastix_amd import problems
+    from pastix_amd.autograd import SparseSolver, solve
+    N = args.grid or 64
+    grid = (N, N, N)
+    n = N ** 3
+    theta = problems.anisotropic_conductivity(*grid, seed=2)
+    P, J = problems.anisotropic_diffusion_jacobian(*grid, ratio_z=10.0)
+    b = problems.uniform(n, seed=3)
+    xt = problems.uniform(n, seed=4)
+    t0 = time.perf_counter()
+    s = SparseSolver(P, symmetric=True, grid=grid)
+    t_an = time.perf_counter() - t0
+    vals0 = torch.from_numpy(J @ theta).to(dev)
+    bt = torch.from_numpy(b).to(dev)
+    xtt = torch.from_numpy(xt).to(dev)
+    s.factor(vals0)                      # first factorization: builds the permuted CSC on the host
+
+    def one_step():
+        v = vals0.clone().requires_grad_(True)
+        bb = bt.clone().requires_grad_(True)
+        x = solve(s, v, bb)
+        loss = 0.5 * ((x - xtt) ** 2).sum()
+        loss.backward()
+        return x, v.grad, bb.grad
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, gv, gb = one_step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    th = float(s.dparm[pa.dparm.fact_thflops])
+    x0 = x.detach().cpu().numpy()
+    A = sp.csc_matrix((J @ theta, P.indices, P.indptr), shape=(n, n))
+    lam = gb.cpu().numpy()
+    adj_res = float(np.abs(A.T @ lam - (x0 - xt)).max() / np.abs(x0 - xt).max())
+    berr = problems.backward_error(A, x0, b)
+    # the pattern gradient is -lambda_i x_j on every stored entry
+    C = sp.csc_matrix(P)
+    C.sort_indices()
+    cols = np.repeat(np.arange(n), np.diff(C.indptr))
+    ref = -lam[C.indices] * x0[cols]
+    g = gv.cpu().numpy()
+    grad_err = float(np.abs(g - ref).max() / np.abs(ref).max())
+    log("anisodiff: %.1f ms/step, berr %.2e, adjoint residual %.2e, grad err %.2e" % (t * 1e3, berr, adj_res, grad_err))
+    s.close()
+    return {
+        "metric": "forward+backward time of the differentiable solve (FP64 LL^t)",
+        "value": th / t / 1e9, "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D 7-point anisotropic diffusion %d^3 (n=%d), theta=exp(N(0,1)) seed 2, ratio 10 in z, "
+                               "LL^t FP64; loss 0.5||x-x_t||^2, b~U[-1,1] seed 3, x_t~U[-1,1] seed 4; step = forward "
+                               "(device value update, factor, solve, refine) + backward (adjoint solve, sampled "
+                               "outer product)" % (N, n),
+                   "thflops": th, "parallelism": "single GPU"},
+        "berr": berr, "adjoint_residual": adj_res, "pattern_grad_rel_err": grad_err, "analysis_s": t_an,
+    }
+
+
+WORKLOADS = {"laplacian": workload_laplacian, "convdiff": workload_convdiff, "batch": workload_batch,
+             "anisodiff": workload_anisodiff}
 
 
 def main():
@@ -170,7 +448,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--grid", type=int, default=100)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="laplacian")
+    ap.add_argument("--grid", type=int, default=0, help="cube side (default: 100 laplacian, 80 convdiff)")
+    ap.add_argument("--count", type=int, default=0, help="systems of the batch workload (default 256)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--eps", type=float, default=1e-13)
@@ -181,91 +461,12 @@ def main():
         sys.exit(2)
 
     import torch  # torch's HIP runtime is loaded first and shared by the engine
-    import numpy as np
     import pastix_amd as pa
-    from pastix_amd import problems, _lib
+    from pastix_amd import _lib
     lib = _lib.load()
     assert torch.cuda.is_available()
     dev = torch.device("cuda:0")
-
-    N = args.grid
-    grid = (N, N, N)
-    log("generating 7-point Laplacian %d^3" % N)
-    A = problems.laplacian(*grid)
-    n = A.shape[0]
-    xt = problems.uniform(n, seed=0)
-    b = A @ xt
-    d_b = torch.from_numpy(b).to(dev)
-    d_x = torch.empty_like(d_b)
-
-    results = {}
-    for mode in (["fp32ir"] if args.no_fp64 else ["fp32ir", "fp64"]):
-        log("mode", mode, ": analysis + first factorization")
-        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps)
-        th = float(inst["dp"][pa.dparm.fact_thflops])
-        t, iters, prof, stages, t_prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup)
-        x = d_x.cpu().numpy()
-        berr = problems.backward_error(A, x, b)
-        ferr = np.abs(x - xt).max() / np.abs(xt).max()
-        results[mode] = dict(t=t, iters=iters, prof=prof, th=th, berr=berr, ferr=ferr, stages=stages, t_prof=t_prof,
-                             t_analyze=inst["t_analyze"], fact_s=float(inst["dp"][pa.dparm.fact_time]),
-                             refine_res=float(inst["dp"][pa.dparm.relative_error]))
-        log("mode %s: %.1f ms/step, berr %.2e, ferr %.2e, iters %s, fact %.1f ms" %
-            (mode, t * 1e3, berr, ferr, iters, results[mode]["fact_s"] * 1e3))
-        pa.finalize(inst["h"])
-
-    r = results["fp32ir"]
-    prof = r["prof"]
-    steps = args.steps
-    gemm_ms = prof[2] / steps
-    gemm_launches = prof[5] / steps
-    gemm_flops = prof[8]
-    achieved = gemm_flops / (gemm_ms * 1e-3)
-    traffic = measured_traffic()
-    roofline = {"bound": "mfma",
-                "kernel": "k_gemm_level<float,128> + k_gemm_level<float,64> (level scatter-GEMM)",
-                "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32"],
-                "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "launches_per_step": gemm_launches, "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
-                "algorithmic_flops_per_launch": gemm_flops / max(gemm_launches, 1),
-                "algorithmic_flops_per_step": gemm_flops,
-                "events_timed_ms_per_step": r["t_prof"] * 1e3}
-    if traffic:
-        roofline["traffic_source"] = traffic["source"]
-
-    line = {
-        "metric": "factor+solve time & GFLOP/s, FP32+IR vs FP64 at backward error <=1e-12",
-        "value": r["th"] / r["t"] / 1e9,
-        "unit": "GFLOP/s",
-        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": r["t"] * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": "3D 7-point Laplacian %d^3 (n=%d), LL^t, grid nested dissection, "
-                               "1 RHS b=A*x_true, x_true~U[-1,1] seed 0" % (N, n),
-                   "thflops": r["th"], "parallelism": "single GPU"},
-        "berr": r["berr"], "ferr": r["ferr"], "refine_iters": r["iters"][-1],
-        "kernel_ms_per_step": {"diag": prof[0] / steps, "trsm": prof[1] / steps, "gemm": prof[2] / steps},
-        "stage_ms": r["stages"],
-        "roofline": roofline,
-    }
-    if "fp64" in results:
-        r64 = results["fp64"]
-        line["fp64_ms_per_step"] = r64["t"] * 1e3
-        line["fp64_gflops"] = r64["th"] / r64["t"] / 1e9
-        line["fp64_berr"] = r64["berr"]
-        line["speedup_fp32ir_vs_fp64"] = r64["t"] / r["t"]
-        p64 = r64["prof"]
-        line["fp64_kernel_ms_per_step"] = {"diag": p64[0] / steps, "trsm": p64[1] / steps,
-                                           "gemm": p64[2] / steps}
-        line["fp64_gemm_tflops"] = p64[8] / (p64[2] / steps * 1e-3) / 1e12
-        line["fp64_stage_ms"] = r64["stages"]
-    if not args.no_cpu:
-        cb = cpu_baseline(pa, problems)
-        line["cpu_baseline"] = cb
-        line["speedup_vs_cpu_baseline"] = line["value"] / cb["value"]
+    line = WORKLOADS[args.workload](args, torch, pa, lib, dev)
     print(json.dumps(line))
 
 

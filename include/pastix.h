@@ -92,6 +92,19 @@ int pastix_amd_spmv_dev( pastix_data_t *pastix_data, pastix_trans_t trans, doubl
                          const void *dx, double beta, void *dy );
 
 /*
+ * New numerical values for the analysed pattern, from device memory: d_values
+ * is a device FP64 array with one value per stored entry of the spm last
+ * given to pastix_task_numfact / pastix_subtask_spm2bcsc, in that spm's
+ * order.  The permuted CSC is rebuilt on the device and ||A||_F
+ * (DPARM_A_NORM) recomputed there; then pastix_subtask_bcsc2ctab +
+ * pastix_subtask_sopalin factor the new values.  Replaces re-entering
+ * pastix_subtask_spm2bcsc with a host spm (reference
+ * sopalin/pastix_task_sopalin.c:93-210).  The first call builds the value
+ * map from that spm, which must still be alive then.
+ */
+int pastix_amd_values_dev( pastix_data_t *pastix_data, const void *d_values );
+
+/*
  * Adjoint helper of the differentiable solve: out[e] = alpha * sum_r
  * u[row(e), r] * v[col(e), r] for every stored entry e of the user's CSC
  * pattern (colptr[n+1], rows[nnz]: pastix_int_t device arrays; u, v: device

@@ -2,8 +2,11 @@
 
 ``SparseSolver`` holds one analysis (ordering, symbolic factorization and
 device structure) for a fixed CSC pattern; every forward call refactors with
-the current values and solves on HBM-resident tensors.  The backward pass
-reuses the factors:
+the current values and solves on HBM-resident tensors.  After the first
+factorization (which builds the permuted CSC on the host), new values go
+from the device tensor straight into the device CSC
+(``pastix_amd_values_dev``), so a forward pass moves no matrix data over
+PCIe.  The backward pass reuses the factors:
 
     lambda   = A^{-T} g          (transposed device solve, refined in FP64)
     dL/db    = lambda
@@ -51,15 +54,24 @@ class SparseSolver(object):
         dev = torch.device("cuda")
         self.d_colptr = torch.from_numpy(self.indptr).to(dev)
         self.d_rows = torch.from_numpy(self.indices).to(dev)
-        self._values = None
+        self._spm = None
 
     def factor(self, values):
         """Numerical factorization with `values` (nnz, CSC order)."""
-        v = values.detach().to("cpu", torch.float64).numpy()
-        A = sp.csc_matrix((v, self.indices, self.indptr), shape=(self.n, self.n))
-        self._spm = spmatrix(A, mtxtype=self._mt)
-        task_numfact(self.h, self._spm)
-        self._values = v
+        if values.shape[0] != self.nnz:
+            raise ValueError("expected %d values, got %d" % (self.nnz, values.shape[0]))
+        if self._spm is None or not values.is_cuda:
+            v = values.detach().to("cpu", torch.float64).numpy()
+            A = sp.csc_matrix((v, self.indices, self.indptr), shape=(self.n, self.n))
+            self._spm = spmatrix(A, mtxtype=self._mt)   # kept alive: the value map is built from it
+            task_numfact(self.h, self._spm)
+            return
+        lib = _lib.load()
+        v = values.detach().to(torch.float64).contiguous()
+        torch.cuda.current_stream(v.device).synchronize()
+        _check(lib.pastix_amd_values_dev(self.h, _dptr(v)), "pastix_amd_values_dev")
+        _check(lib.pastix_subtask_bcsc2ctab(self.h), "pastix_subtask_bcsc2ctab")
+        _check(lib.pastix_subtask_sopalin(self.h), "pastix_subtask_sopalin")
 
     def solve(self, b, transpose=False):
         """x = op(A)^{-1} b for a device FP64 tensor b (n or n x nrhs)."""
@@ -68,6 +80,7 @@ class SparseSolver(object):
         nrhs = 1 if b.dim() == 1 else b.shape[1]
         bcm = b.contiguous() if b.dim() == 1 else b.t().contiguous()   # column-major copy
         x = bcm.clone()
+        torch.cuda.current_stream(x.device).synchronize()
         tr = _tr.Trans if transpose else _tr.NoTrans
         _check(lib.pastix_amd_solve_dev(self.h, tr, nrhs, _dptr(x), self.n), "pastix_amd_solve_dev")
         # FP64 refinement of A x = b; A^T = A for the symmetric factorization
@@ -84,6 +97,7 @@ class SparseSolver(object):
         vv = v.t().contiguous() if v.dim() == 2 else v.contiguous()
         nrhs = 1 if u.dim() == 1 else u.shape[1]
         out = torch.empty(self.nnz, dtype=torch.float64, device=u.device)
+        torch.cuda.current_stream(out.device).synchronize()
         _check(lib.pastix_amd_sampled_outer(self.h, _dptr(self.d_colptr), _dptr(self.d_rows), nrhs, _dptr(uu),
                                             self.n, _dptr(vv), self.n, alpha, _dptr(out)),
                "pastix_amd_sampled_outer")

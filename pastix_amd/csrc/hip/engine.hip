@@ -57,10 +57,15 @@ struct pa_graph_entry {
     hipGraphExec_t exec  = nullptr;
     const void    *key[4] = { nullptr, nullptr, nullptr, nullptr };
     int64_t        ikey[4] = { 0, 0, 0, 0 };
-    double         dkey = 0.;
     uint64_t       stamp = 0;
 };
 #define PA_SOLVE_GRAPHS 6
+
+struct pa_wblock {
+    void  *p;
+    size_t bytes;
+    int    used;
+};
 
 struct pa_device_s {
     int         device;
@@ -88,6 +93,8 @@ struct pa_device_s {
     int64_t *d_colptr;
     int32_t *d_rowtab;
     double  *d_Lv, *d_Uv;
+    /* value map of the permuted CSC (device-resident value updates) */
+    int64_t *d_vm_Lptr, *d_vm_Lsrc, *d_vm_Uptr, *d_vm_Usrc;
     /* factors */
     int   prec, lu;
     void *d_L, *d_U;
@@ -99,6 +106,7 @@ struct pa_device_s {
     pa_graph_entry solve_graphs[PA_SOLVE_GRAPHS];
     uint64_t       graph_clock = 0;
     int32_t *d_counters;
+    double  *d_crit;     /* static pivoting criterion of the next factorization */
     /* work */
     double *d_partial;
     double *h_partial;
@@ -106,6 +114,7 @@ struct pa_device_s {
     int64_t xs_len;
     double *d_xtmp;    /* permuted rhs buffer */
     int64_t xtmp_len;
+    std::vector<pa_wblock> wpool;   /* pa_dev_wget / pa_dev_wput */
     hipEvent_t ev0, ev1;
     double  last_fact_event_s;
     /* per kernel class profiling (diag, trsm, gemm) with HIP events */
@@ -194,6 +203,7 @@ pa_dev_open( pastix_data_t *pd )
     dev->lds_limit = lds;
     pa_set_lds_limits( lds );
     PA_HIP( hipMalloc( (void **)&dev->d_counters, 16 * sizeof( int32_t ) ) );
+    PA_HIP( hipMalloc( (void **)&dev->d_crit, sizeof( double ) ) );
     PA_HIP( hipMalloc( (void **)&dev->d_partial, 4096 * sizeof( double ) ) );
     PA_HIP( hipHostMalloc( (void **)&dev->h_partial, 4096 * sizeof( double ) ) );
     pd->dev = dev;
@@ -249,6 +259,8 @@ pa_free_matrix( pa_device_s *dev )
 {
     hipFree( dev->d_colptr ); hipFree( dev->d_rowtab ); hipFree( dev->d_Lv ); hipFree( dev->d_Uv );
     dev->d_colptr = NULL; dev->d_rowtab = NULL; dev->d_Lv = NULL; dev->d_Uv = NULL;
+    hipFree( dev->d_vm_Lptr ); hipFree( dev->d_vm_Lsrc ); hipFree( dev->d_vm_Uptr ); hipFree( dev->d_vm_Usrc );
+    dev->d_vm_Lptr = dev->d_vm_Lsrc = dev->d_vm_Uptr = dev->d_vm_Usrc = NULL;
 }
 
 static void
@@ -292,7 +304,10 @@ pa_dev_close( pastix_data_t *pd )
     pa_free_matrix( dev );
     pa_free_factors( dev );
     pa_free_structure( dev );
+    for ( const pa_wblock &w : dev->wpool ) hipFree( w.p );
+    dev->wpool.clear();
     hipFree( dev->d_counters );
+    hipFree( dev->d_crit );
     hipFree( dev->d_partial );
     hipHostFree( dev->h_partial );
     hipFree( dev->d_xs );
@@ -573,6 +588,41 @@ pa_dev_upload_bcsc( pastix_data_t *pd )
     return PASTIX_SUCCESS;
 }
 
+extern "C" int
+pa_dev_upload_valmap( pastix_data_t *pd, const pastix_int_t *Lptr, const pastix_int_t *Lsrc,
+                      const pastix_int_t *Uptr, const pastix_int_t *Usrc, int64_t nnz )
+{
+    pa_device_s *dev = DEV( pd );
+    int rc;
+    PA_HIP( hipSetDevice( dev->device ) );
+    hipFree( dev->d_vm_Lptr ); hipFree( dev->d_vm_Lsrc ); hipFree( dev->d_vm_Uptr ); hipFree( dev->d_vm_Usrc );
+    dev->d_vm_Lptr = dev->d_vm_Lsrc = dev->d_vm_Uptr = dev->d_vm_Usrc = NULL;
+    std::vector<int64_t> lp( Lptr, Lptr + nnz + 1 ), up( Uptr, Uptr + nnz + 1 );
+    std::vector<int64_t> ls( Lsrc, Lsrc + Lptr[nnz] ), us( Usrc, Usrc + Uptr[nnz] );
+    if ( ( rc = pa_upload( &dev->d_vm_Lptr, lp ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_vm_Uptr, up ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_vm_Lsrc, ls ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_vm_Usrc, us ) ) ) return rc;
+    return PASTIX_SUCCESS;
+}
+
+extern "C" int
+pa_dev_set_values( pastix_data_t *pd, const double *d_values, double *normF )
+{
+    pa_device_s *dev = DEV( pd );
+    const int64_t nnz = pd->bcsc->nnz;
+    PA_HIP( hipSetDevice( dev->device ) );
+    if ( !dev->d_vm_Lptr || !dev->d_Lv ) return PASTIX_ERR_BADPARAMETER;
+    if ( nnz > 0 ) {
+        const unsigned grid = (unsigned)std::min<int64_t>( ( nnz + 255 ) / 256, 65536 );
+        k_gather_sum<<<grid, 256, 0, dev->stream>>>( nnz, dev->d_vm_Lptr, dev->d_vm_Lsrc, d_values, dev->d_Lv );
+        k_gather_sum<<<grid, 256, 0, dev->stream>>>( nnz, dev->d_vm_Uptr, dev->d_vm_Usrc, d_values, dev->d_Uv );
+        PA_HIP( hipGetLastError() );
+    }
+    *normF = pa_dev_nrm2( pd, nnz, dev->d_Lv );
+    return PASTIX_SUCCESS;
+}
+
 template <typename T>
 static int
 pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
@@ -630,8 +680,9 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
 
 template <typename T>
 static int
-pa_factorize_t( pa_device_s *dev, double crit )
+pa_factorize_t( pa_device_s *dev )
 {
+    const double *crit = dev->d_crit;
     T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
     const int lu = dev->lu;
     for ( size_t li = 0; li < dev->levels.size(); li++ ) {
@@ -731,11 +782,13 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     int rc;
     PA_HIP( hipSetDevice( dev->device ) );
     PA_HIP( hipMemsetAsync( dev->d_counters, 0, 2 * sizeof( int32_t ), dev->stream ) );
+    k_set_f64<<<1, 64, 0, dev->stream>>>( dev->d_crit, crit );
+    PA_HIP( hipGetLastError() );
     PA_HIP( hipStreamSynchronize( dev->stream ) );
     auto t0 = std::chrono::steady_clock::now();
     PA_HIP( hipEventRecord( dev->ev0, dev->stream ) );
     auto launch = [&]() {
-        return ( dev->prec == PA_PREC_FLOAT ) ? pa_factorize_t<float>( dev, crit ) : pa_factorize_t<double>( dev, crit );
+        return ( dev->prec == PA_PREC_FLOAT ) ? pa_factorize_t<float>( dev ) : pa_factorize_t<double>( dev );
     };
     if ( dev->profiling && dev->prof_ev.size() < 6 * dev->levels.size() ) {
         /* the per-level timing events exist before any capture */
@@ -744,16 +797,17 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
         for ( size_t i = old; i < dev->prof_ev.size(); i++ ) PA_HIP( hipEventCreate( &dev->prof_ev[i] ) );
     }
     if ( dev->use_graphs && !dev->profiling ) {
-        /* the launch sequence depends on the factors' addresses and the
-         * pivoting criterion; the per-level timing events are recorded by the
-         * eager launches only (captured events cannot be timed) */
+        /* the launch sequence depends on the factors' addresses only (the
+         * criterion is read from d_crit); the per-level timing events are
+         * recorded by the eager launches only (captured events cannot be
+         * timed) */
         pa_graph_entry &g = dev->fact_graph;
         bool hit = g.exec && g.key[0] == dev->d_L && g.key[1] == dev->d_U && g.key[2] == dev->d_DT &&
-                   g.ikey[0] == dev->prec && g.ikey[1] == dev->lu && g.ikey[2] == dev->profiling && g.dkey == crit;
+                   g.ikey[0] == dev->prec && g.ikey[1] == dev->lu && g.ikey[2] == dev->profiling;
         rc = pa_graph_run( dev, g, hit, launch );
         if ( rc ) return rc;
         g.key[0] = dev->d_L; g.key[1] = dev->d_U; g.key[2] = dev->d_DT;
-        g.ikey[0] = dev->prec; g.ikey[1] = dev->lu; g.ikey[2] = dev->profiling; g.dkey = crit;
+        g.ikey[0] = dev->prec; g.ikey[1] = dev->lu; g.ikey[2] = dev->profiling;
     }
     else {
         rc = launch();
@@ -975,6 +1029,56 @@ extern "C" void
 pa_dev_free( void *p )
 {
     if ( p ) hipFree( p );
+}
+extern "C" void *
+pa_dev_wget( pastix_data_t *pd, size_t bytes )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( !dev ) return NULL;
+    bytes = ( std::max<size_t>( bytes, 256 ) + 255 ) & ~(size_t)255;
+    int best = -1;
+    for ( size_t i = 0; i < dev->wpool.size(); i++ ) {
+        const pa_wblock &w = dev->wpool[i];
+        if ( !w.used && w.bytes >= bytes && ( best < 0 || w.bytes < dev->wpool[best].bytes ) ) best = (int)i;
+    }
+    if ( best < 0 ) {
+        void *p = NULL;
+        if ( hipMalloc( &p, bytes ) != hipSuccess ) {
+            /* release the free blocks (after the stream has drained) and retry once */
+            hipGetLastError();
+            hipStreamSynchronize( dev->stream );
+            for ( size_t i = 0; i < dev->wpool.size(); ) {
+                if ( !dev->wpool[i].used ) {
+                    hipFree( dev->wpool[i].p );
+                    dev->wpool.erase( dev->wpool.begin() + i );
+                }
+                else {
+                    i++;
+                }
+            }
+            pa_graphs_drop( dev );
+            if ( hipMalloc( &p, bytes ) != hipSuccess ) {
+                hipGetLastError();
+                return NULL;
+            }
+        }
+        dev->wpool.push_back( { p, bytes, 0 } );
+        best = (int)dev->wpool.size() - 1;
+    }
+    dev->wpool[best].used = 1;
+    return dev->wpool[best].p;
+}
+extern "C" void
+pa_dev_wput( pastix_data_t *pd, void *p )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( !dev || !p ) return;
+    for ( pa_wblock &w : dev->wpool ) {
+        if ( w.p == p ) {
+            w.used = 0;
+            return;
+        }
+    }
 }
 extern "C" int
 pa_dev_memcpy_h2d( pastix_data_t *pd, void *d, const void *h, size_t bytes )

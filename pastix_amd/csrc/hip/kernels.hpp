@@ -384,16 +384,36 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
     }
 }
 
-/* one workgroup per cblk of a level */
+/* one workgroup per cblk of a level; the pivoting criterion is read from
+ * device memory so that a recorded launch sequence stays valid when it
+ * changes with the matrix values */
 template <typename T, bool LU, int TH>
 __global__ __launch_bounds__( TH ) void
-k_diag( pa_dstruct S, const int32_t *cblks, T *Lc, T *Uc, double crit, int32_t *npiv,
+k_diag( pa_dstruct S, const int32_t *cblks, T *Lc, T *Uc, const double *critp, int32_t *npiv,
         int32_t *nneg )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     int32_t k = cblks[blockIdx.x];
-    pa_diag_factor<T, LU, TH>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], crit, npiv, nneg,
-                               reinterpret_cast<T *>( pa_smem_raw ) );
+    pa_diag_factor<T, LU, TH>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], *critp, npiv,
+                               nneg, reinterpret_cast<T *>( pa_smem_raw ) );
+}
+
+/* out[w] = sum of vals[src[k]] for k in [ptr[w], ptr[w+1]): the permuted
+ * CSC gathered from the user's stored values through the value map */
+__global__ __launch_bounds__( 256 ) void
+k_gather_sum( int64_t nnz, const int64_t *ptr, const int64_t *src, const double *vals, double *out )
+{
+    for ( int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nnz; w += (int64_t)gridDim.x * blockDim.x ) {
+        double s = 0.;
+        for ( int64_t k = ptr[w]; k < ptr[w + 1]; k++ ) s += vals[src[k]];
+        out[w] = s;
+    }
+}
+
+__global__ void
+k_set_f64( double *p, double v )
+{
+    if ( threadIdx.x == 0 && blockIdx.x == 0 ) *p = v;
 }
 
 /* single diagonal block given by pointers (reference-signature entry points) */
@@ -558,10 +578,11 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
 }
 
 __global__ __launch_bounds__( 1024 ) void
-k_diag_potrf_reg( pa_dstruct S, const int32_t *cblks, float *Lc, double crit, int32_t *npiv, int32_t *nneg )
+k_diag_potrf_reg( pa_dstruct S, const int32_t *cblks, float *Lc, const double *critp, int32_t *npiv,
+                  int32_t *nneg )
 {
     const int32_t k = cblks[blockIdx.x];
-    pa_diag_potrf_reg( Lc + S.c_coef[k], S.c_ncol[k], crit, npiv, nneg );
+    pa_diag_potrf_reg( Lc + S.c_coef[k], S.c_ncol[k], *critp, npiv, nneg );
 }
 
 __global__ __launch_bounds__( 1024 ) void

@@ -24,6 +24,12 @@ void   pa_dev_close( pastix_data_t *pastix_data );
 int    pa_dev_setup_structure( pastix_data_t *pastix_data );
 /* upload the permuted CSC (pattern + FP64 values) */
 int    pa_dev_upload_bcsc( pastix_data_t *pastix_data );
+/* upload the value map of the permuted CSC (see pa_valmap_t) */
+int    pa_dev_upload_valmap( pastix_data_t *pastix_data, const pastix_int_t *Lptr, const pastix_int_t *Lsrc,
+                             const pastix_int_t *Uptr, const pastix_int_t *Usrc, int64_t nnz );
+/* gather new FP64 user values (device, user CSC order) into the permuted
+ * device CSC through the value map; *normF gets ||A||_F */
+int    pa_dev_set_values( pastix_data_t *pastix_data, const double *d_values, double *normF );
 /* allocate the coeftab in HBM (prec) and fill it from the device CSC */
 int    pa_dev_coeftab_init( pastix_data_t *pastix_data, int prec, int lu, int32_t *overflow );
 /* level-batched numerical factorization; returns PASTIX_SUCCESS */
@@ -43,6 +49,13 @@ int    pa_dev_apply_perm( pastix_data_t *pastix_data, int forward, int32_t nrhs,
 /* FP64 vector helpers on the library stream */
 void  *pa_dev_malloc( size_t bytes );
 void   pa_dev_free( void *d_ptr );
+/* work buffers of the solve/refinement drivers: a block of at least `bytes`
+ * from the handle's pool (hipMalloc only when no free block is large
+ * enough), returned with pa_dev_wput; the pool is released at finalize.
+ * The same call sequence gets the same addresses, which keeps the cached
+ * solve graphs valid. */
+void  *pa_dev_wget( pastix_data_t *pastix_data, size_t bytes );
+void   pa_dev_wput( pastix_data_t *pastix_data, void *d_ptr );
 int    pa_dev_memcpy_h2d( pastix_data_t *pastix_data, void *d_dst, const void *h_src, size_t bytes );
 int    pa_dev_memcpy_d2h( pastix_data_t *pastix_data, void *h_dst, const void *d_src, size_t bytes );
 int    pa_dev_memcpy_d2d( pastix_data_t *pastix_data, void *d_dst, const void *d_src, size_t bytes );

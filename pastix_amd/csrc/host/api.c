@@ -391,8 +391,47 @@ pastix_subtask_spm2bcsc( pastix_data_t *pd, spmatrix_t *spm )
     pd->csc = spm;
     pd->dparm[DPARM_A_NORM] = pd->bcsc->normF;
     if ( ( rc = pa_dev_upload_bcsc( pd ) ) ) return rc;
+    pd->valmap_ready = 0;
     pd->steps &= ~( STEP_BCSC2CTAB | STEP_NUMFACT );
     pd->steps |= STEP_CSC2BCSC;
+    return PASTIX_SUCCESS;
+}
+
+/* Device-resident counterpart of re-entering pastix_subtask_spm2bcsc with new
+ * values on the same pattern (reference sopalin/pastix_task_sopalin.c:93-210):
+ * the permuted CSC is gathered on the device and ||A||_F recomputed there. */
+int
+pastix_amd_values_dev( pastix_data_t *pd, const void *d_values )
+{
+    int rc;
+    double normF = 0.;
+    if ( pd == NULL || d_values == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( !( pd->steps & STEP_CSC2BCSC ) || pd->csc == NULL || pd->bcsc == NULL ) {
+        pa_error( "pastix_amd_values_dev: pastix_task_numfact() or pastix_subtask_spm2bcsc() has to be called first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !pd->valmap_ready ) {
+        /* the map is built once from the spm of the last spm2bcsc */
+        pa_bcsc_t   tmp;
+        pa_valmap_t map;
+        memset( &tmp, 0, sizeof( tmp ) );
+        if ( ( rc = pa_bcsc_build( &tmp, pd->csc, pd->ordemesh, &map ) ) ) return rc;
+        if ( tmp.nnz != pd->bcsc->nnz ) {
+            pa_bcsc_exit( &tmp );
+            pa_valmap_exit( &map );
+            pa_error( "pastix_amd_values_dev: the matrix given to spm2bcsc has changed" );
+            return PASTIX_ERR_BADPARAMETER;
+        }
+        rc = pa_dev_upload_valmap( pd, map.Lptr, map.Lsrc, map.Uptr, map.Usrc, map.nnz );
+        pa_bcsc_exit( &tmp );
+        pa_valmap_exit( &map );
+        if ( rc ) return rc;
+        pd->valmap_ready = 1;
+    }
+    if ( ( rc = pa_dev_set_values( pd, (const double *)d_values, &normF ) ) ) return rc;
+    pd->bcsc->normF         = normF;
+    pd->dparm[DPARM_A_NORM] = normF;
+    pd->steps &= ~( STEP_BCSC2CTAB | STEP_NUMFACT );
     return PASTIX_SUCCESS;
 }
 
@@ -503,13 +542,13 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrh
     if ( nrhs < 1 || ldb < n ) return PASTIX_ERR_BADPARAMETER;
     if ( transA != PastixNoTrans && transA != PastixTrans ) return PASTIX_ERR_BADPARAMETER;
     t0  = pa_wtime();
-    tmp = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
+    tmp = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
     if ( !tmp ) return PASTIX_ERR_OUTOFMEMORY;
     rc = pa_dev_apply_perm( pd, 1, (int32_t)nrhs, dB, ldb, tmp, n );
     if ( !rc ) rc = pa_dev_solve_perm( pd, transA, (int32_t)nrhs, tmp, n );
     if ( !rc ) rc = pa_dev_apply_perm( pd, 0, (int32_t)nrhs, tmp, n, dB, ldb );
     if ( !rc ) rc = pa_dev_sync( pd );
-    pa_dev_free( tmp );
+    pa_dev_wput( pd, tmp );
     pd->dparm[DPARM_SOLV_TIME] = pa_wtime() - t0;
     if ( !rc ) pd->steps |= STEP_SOLVE;
     return rc;
@@ -529,12 +568,12 @@ pastix_task_solve( pastix_data_t *pd, pastix_int_t m, pastix_int_t nrhs, void *B
     }
     n = pd->solvmatr->nodenbr;
     if ( m != n || ldb < n || B == NULL ) return PASTIX_ERR_BADPARAMETER;
-    dB = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
+    dB = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
     if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
     for ( j = 0; j < nrhs; j++ ) pa_dev_memcpy_h2d( pd, dB + j * n, (double *)B + j * ldb, n * sizeof( double ) );
     rc = pastix_amd_solve_dev( pd, (pastix_trans_t)pd->iparm[IPARM_TRANSPOSE_SOLVE], nrhs, dB, n );
     for ( j = 0; !rc && j < nrhs; j++ ) pa_dev_memcpy_d2h( pd, (double *)B + j * ldb, dB + j * n, n * sizeof( double ) );
-    pa_dev_free( dB );
+    pa_dev_wput( pd, dB );
     return rc;
 }
 
@@ -543,7 +582,7 @@ pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t
 {
     if ( b == NULL || b->flttype != PastixDouble ) return PASTIX_ERR_BADPARAMETER;
     {
-        double *dB = pa_dev_malloc( (size_t)b->m * b->n * sizeof( double ) );
+        double *dB = pa_dev_wget( pd, (size_t)b->m * b->n * sizeof( double ) );
         int rc;
         pastix_int_t j;
         if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
@@ -551,7 +590,7 @@ pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t
         /* b is already in the factorization numbering */
         rc = pa_dev_solve_perm( pd, transA, (int32_t)b->n, dB, b->m );
         for ( j = 0; !rc && j < b->n; j++ ) pa_dev_memcpy_d2h( pd, (double *)b->b + j * b->ld, dB + j * b->m, b->m * sizeof( double ) );
-        pa_dev_free( dB );
+        pa_dev_wput( pd, dB );
         return rc;
     }
 }

@@ -11,7 +11,9 @@
 #include <math.h>
 #include "internal.h"
 
-typedef struct { pastix_int_t r; double v, vt; } pa_entry_t;
+/* one coefficient of the expanded pattern before the per-column sort: row,
+ * A(row,col), A(col,row), and the user entries they come from (-1: none) */
+typedef struct { pastix_int_t r; double v, vt; pastix_int_t sv, svt; } pa_entry_t;
 
 static int
 cmp_entry( const void *a, const void *b )
@@ -22,6 +24,21 @@ cmp_entry( const void *a, const void *b )
 
 int
 pa_bcsc_init( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *order )
+{
+    return pa_bcsc_build( bcsc, spm, order, NULL );
+}
+
+void
+pa_valmap_exit( pa_valmap_t *map )
+{
+    if ( map ) {
+        free( map->Lptr ); free( map->Lsrc ); free( map->Uptr ); free( map->Usrc );
+        memset( map, 0, sizeof( *map ) );
+    }
+}
+
+int
+pa_bcsc_build( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *order, pa_valmap_t *map )
 {
     pastix_int_t n = spm->n, b = spm->baseval, i, e, total, *cnt, *pos;
     const pastix_int_t *ptr = spm->colptr, *idx = spm->rowptr;
@@ -68,20 +85,26 @@ pa_bcsc_init( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *orde
             pastix_int_t row = csr ? i : r, col = csr ? r : i;
             pastix_int_t prow = order->permtab[row], pcol = order->permtab[col];
             pa_entry_t *x = ent + cnt[pcol]++;
-            x->r  = prow;
-            x->v  = v;   /* A(row, col)                    */
-            x->vt = 0.;  /* A(col, row) unknown from here  */
+            x->r   = prow;
+            x->v   = v;   /* A(row, col)                    */
+            x->vt  = 0.;  /* A(col, row) unknown from here  */
+            x->sv  = e;
+            x->svt = -1;
             if ( row != col ) {
                 pa_entry_t *y = ent + cnt[prow]++;
-                y->r  = pcol;
-                y->v  = sym ? v : 0.; /* A(col,row) = v if symmetric      */
-                y->vt = v;            /* transpose view of column prow    */
+                y->r   = pcol;
+                y->v   = sym ? v : 0.; /* A(col,row) = v if symmetric      */
+                y->vt  = v;            /* transpose view of column prow    */
+                y->sv  = sym ? e : -1;
+                y->svt = e;
             }
             else {
-                x->vt = v;
+                x->vt  = v;
+                x->svt = e;
             }
             if ( sym && row != col ) {
-                x->vt = v;
+                x->vt  = v;
+                x->svt = e;
             }
         }
     }
@@ -93,8 +116,16 @@ pa_bcsc_init( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *orde
     bcsc->rowtab  = malloc( ( total + 1 ) * sizeof( pastix_int_t ) );
     bcsc->Lvalues = malloc( ( total + 1 ) * sizeof( double ) );
     bcsc->Uvalues = malloc( ( total + 1 ) * sizeof( double ) );
+    if ( map ) {
+        memset( map, 0, sizeof( *map ) );
+        map->nuser = ptr[n] - b;
+        map->Lptr  = malloc( ( total + 1 ) * sizeof( pastix_int_t ) );
+        map->Uptr  = malloc( ( total + 1 ) * sizeof( pastix_int_t ) );
+        map->Lsrc  = malloc( ( total + 1 ) * sizeof( pastix_int_t ) );
+        map->Usrc  = malloc( ( total + 1 ) * sizeof( pastix_int_t ) );
+    }
     {
-        pastix_int_t w = 0;
+        pastix_int_t w = 0, nl = 0, nu = 0;
         double *rowsum = calloc( n + 1, sizeof( double ) );
         double  frob = 0.;
         for ( i = 0; i < n; i++ ) {
@@ -111,14 +142,28 @@ pa_bcsc_init( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *orde
                     bcsc->Uvalues[w - 1] += ent[e].vt;
                 }
                 else {
+                    if ( map ) {
+                        map->Lptr[w] = nl;
+                        map->Uptr[w] = nu;
+                    }
                     bcsc->rowtab[w]  = ent[e].r;
                     bcsc->Lvalues[w] = ent[e].v;
                     bcsc->Uvalues[w] = ent[e].vt;
                     w++;
                 }
+                if ( map ) {
+                    /* the entries summed into position w-1, by source */
+                    if ( ent[e].sv >= 0 ) map->Lsrc[nl++] = ent[e].sv;
+                    if ( ent[e].svt >= 0 ) map->Usrc[nu++] = ent[e].svt;
+                }
             }
         }
         bcsc->colptr[n] = w;
+        if ( map ) {
+            map->nnz     = w;
+            map->Lptr[w] = nl;
+            map->Uptr[w] = nu;
+        }
         bcsc->nnz       = w;
         for ( i = 0; i < n; i++ ) {
             for ( e = bcsc->colptr[i]; e < bcsc->colptr[i + 1]; e++ ) {

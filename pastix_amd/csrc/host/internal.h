@@ -114,8 +114,22 @@ typedef struct pa_bcsc_s {
     double        normInf;      /* ||A||_inf                                   */
 } pa_bcsc_t;
 
+/* Where the values of the expanded pattern come from: position w of
+ * Lvalues is the sum of the user's stored values vals[Lsrc[Lptr[w] ..
+ * Lptr[w+1]-1]], likewise for Uvalues (duplicates are summed, a symmetric
+ * matrix storing both triangles keeps one copy). */
+typedef struct pa_valmap_s {
+    pastix_int_t  nnz;          /* positions of the expanded pattern           */
+    pastix_int_t  nuser;        /* stored entries of the user matrix           */
+    pastix_int_t *Lptr, *Lsrc;
+    pastix_int_t *Uptr, *Usrc;
+} pa_valmap_t;
+
 int  pa_bcsc_init( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *order );
+/* pa_bcsc_init that also records the value map (map may be NULL) */
+int  pa_bcsc_build( pa_bcsc_t *bcsc, const spmatrix_t *spm, const pastix_order_t *order, pa_valmap_t *map );
 void pa_bcsc_exit( pa_bcsc_t *bcsc );
+void pa_valmap_exit( pa_valmap_t *map );
 
 /* -------------------------------------------------------------------------- */
 /* Main instance structure.                                                   */
@@ -144,6 +158,7 @@ struct pastix_data_s {
     pa_bcsc_t         *bcsc;
     SolverMatrix      *solvmatr;
     struct pa_device_s *dev;
+    int                valmap_ready;   /* the device holds the value map of csc */
 };
 
 #ifdef __cplusplus

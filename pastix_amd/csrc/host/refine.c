@@ -44,8 +44,8 @@ pa_gmres( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
     pastix_int_t i, j, iters = 0;
     int outflag = 1, rc = 0;
 
-    gmV = pa_dev_malloc( (size_t)n * im1 * sizeof( double ) );
-    gmW = pa_dev_malloc( (size_t)n * im * sizeof( double ) );
+    gmV = pa_dev_wget( pd, (size_t)n * im1 * sizeof( double ) );
+    gmW = pa_dev_wget( pd, (size_t)n * im * sizeof( double ) );
     H   = calloc( im * im1, sizeof( double ) );
     cs  = calloc( im, sizeof( double ) );
     sn  = calloc( im, sizeof( double ) );
@@ -118,8 +118,8 @@ pa_gmres( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
     }
 end:
     pa_dev_sync( pd );
-    pa_dev_free( gmV );
-    pa_dev_free( gmW );
+    pa_dev_wput( pd, gmV );
+    pa_dev_wput( pd, gmW );
     free( H ); free( cs ); free( sn ); free( G );
     out->resid_b = resid_b;
     out->iters   = iters;
@@ -132,8 +132,8 @@ pa_cg( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
 {
     pastix_int_t n = pd->solvmatr->nodenbr, itermax = pd->iparm[IPARM_ITERMAX], it = 0;
     double eps = pd->dparm[DPARM_EPSILON_REFINEMENT];
-    double *r = pa_dev_malloc( n * sizeof( double ) ), *p = pa_dev_malloc( n * sizeof( double ) );
-    double *z = pa_dev_malloc( n * sizeof( double ) ), *q = pa_dev_malloc( n * sizeof( double ) );
+    double *r = pa_dev_wget( pd, n * sizeof( double ) ), *p = pa_dev_wget( pd, n * sizeof( double ) );
+    double *z = pa_dev_wget( pd, n * sizeof( double ) ), *q = pa_dev_wget( pd, n * sizeof( double ) );
     double normb, normx, normr, resid_b, alpha, beta;
     int rc = 0;
     if ( !r || !p || !z || !q ) { rc = PASTIX_ERR_OUTOFMEMORY; goto end; }
@@ -169,7 +169,7 @@ pa_cg( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
     out->iters   = it;
 end:
     pa_dev_sync( pd );
-    pa_dev_free( r ); pa_dev_free( p ); pa_dev_free( z ); pa_dev_free( q );
+    pa_dev_wput( pd, r ); pa_dev_wput( pd, p ); pa_dev_wput( pd, z ); pa_dev_wput( pd, q );
     return rc;
 }
 
@@ -180,7 +180,7 @@ pa_sr( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
 {
     pastix_int_t n = pd->solvmatr->nodenbr, itermax = pd->iparm[IPARM_ITERMAX], iter = 0;
     double eps = pd->dparm[DPARM_EPSILON_REFINEMENT];
-    double *r = pa_dev_malloc( n * sizeof( double ) );
+    double *r = pa_dev_wget( pd, n * sizeof( double ) );
     double normb, berr = 0., last_berr = 0.;
     int rc = 0;
     if ( !r ) return PASTIX_ERR_OUTOFMEMORY;
@@ -202,7 +202,7 @@ pa_sr( pastix_data_t *pd, const double *b, double *x, pa_refine_out_t *out )
         iter++;
     }
     pa_dev_sync( pd );
-    pa_dev_free( r );
+    pa_dev_wput( pd, r );
     out->resid_b = berr;
     out->iters   = iter;
     return rc;
@@ -245,16 +245,16 @@ pastix_amd_refine_dev( pastix_data_t *pd, pastix_int_t nrhs, const void *dB, pas
     }
     n  = pd->solvmatr->nodenbr;
     t0 = pa_wtime();
-    bp = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
-    xp = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
-    if ( !bp || !xp ) { pa_dev_free( bp ); pa_dev_free( xp ); return -PASTIX_ERR_OUTOFMEMORY; }
+    bp = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    xp = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    if ( !bp || !xp ) { pa_dev_wput( pd, bp ); pa_dev_wput( pd, xp ); return -PASTIX_ERR_OUTOFMEMORY; }
     pa_dev_apply_perm( pd, 1, (int32_t)nrhs, dB, ldb, bp, n );
     pa_dev_apply_perm( pd, 1, (int32_t)nrhs, dX, ldx, xp, n );
     rc = pa_refine_perm( pd, nrhs, bp, n, xp, n );
     pa_dev_apply_perm( pd, 0, (int32_t)nrhs, xp, n, dX, ldx );
     pa_dev_sync( pd );
-    pa_dev_free( bp );
-    pa_dev_free( xp );
+    pa_dev_wput( pd, bp );
+    pa_dev_wput( pd, xp );
     pd->dparm[DPARM_REFINE_TIME] = pa_wtime() - t0;
     if ( rc ) return -rc;
     pd->steps |= STEP_REFINE;
@@ -273,9 +273,9 @@ pastix_task_refine( pastix_data_t *pd, pastix_int_t n, pastix_int_t nrhs, void *
         return PASTIX_ERR_BADPARAMETER;
     }
     if ( n != pd->solvmatr->nodenbr || B == NULL || X == NULL ) return PASTIX_ERR_BADPARAMETER;
-    dB = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
-    dX = pa_dev_malloc( (size_t)n * nrhs * sizeof( double ) );
-    if ( !dB || !dX ) { pa_dev_free( dB ); pa_dev_free( dX ); return PASTIX_ERR_OUTOFMEMORY; }
+    dB = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    dX = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    if ( !dB || !dX ) { pa_dev_wput( pd, dB ); pa_dev_wput( pd, dX ); return PASTIX_ERR_OUTOFMEMORY; }
     for ( j = 0; j < nrhs; j++ ) {
         pa_dev_memcpy_h2d( pd, dB + j * n, (double *)B + j * ldb, n * sizeof( double ) );
         pa_dev_memcpy_h2d( pd, dX + j * n, (double *)X + j * ldx, n * sizeof( double ) );
@@ -284,8 +284,8 @@ pastix_task_refine( pastix_data_t *pd, pastix_int_t n, pastix_int_t nrhs, void *
     for ( j = 0; rc >= 0 && j < nrhs; j++ ) {
         pa_dev_memcpy_d2h( pd, (double *)X + j * ldx, dX + j * n, n * sizeof( double ) );
     }
-    pa_dev_free( dB );
-    pa_dev_free( dX );
+    pa_dev_wput( pd, dB );
+    pa_dev_wput( pd, dX );
     return rc < 0 ? -rc : PASTIX_SUCCESS;
 }
 
@@ -308,15 +308,15 @@ pastix_amd_spmv_dev( pastix_data_t *pd, pastix_trans_t trans, double alpha, cons
     double *xp, *yp;
     if ( pd == NULL || !( pd->steps & STEP_CSC2BCSC ) ) return PASTIX_ERR_BADPARAMETER;
     n  = pd->solvmatr->nodenbr;
-    xp = pa_dev_malloc( n * sizeof( double ) );
-    yp = pa_dev_malloc( n * sizeof( double ) );
-    if ( !xp || !yp ) { pa_dev_free( xp ); pa_dev_free( yp ); return PASTIX_ERR_OUTOFMEMORY; }
+    xp = pa_dev_wget( pd, n * sizeof( double ) );
+    yp = pa_dev_wget( pd, n * sizeof( double ) );
+    if ( !xp || !yp ) { pa_dev_wput( pd, xp ); pa_dev_wput( pd, yp ); return PASTIX_ERR_OUTOFMEMORY; }
     pa_dev_apply_perm( pd, 1, 1, dx, n, xp, n );
     pa_dev_apply_perm( pd, 1, 1, dy, n, yp, n );
     pa_dev_spmv_perm( pd, trans, alpha, xp, beta, yp );
     pa_dev_apply_perm( pd, 0, 1, yp, n, dy, n );
     pa_dev_sync( pd );
-    pa_dev_free( xp );
-    pa_dev_free( yp );
+    pa_dev_wput( pd, xp );
+    pa_dev_wput( pd, yp );
     return PASTIX_SUCCESS;
 }
