@@ -42,12 +42,16 @@ double pa_flops_syrk( double k, double n );
 
 struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
+    int64_t fac_end;              /* [cblk_beg, fac_end): diag/TRSM/GEMM    */
+                                  /* kernels; [fac_end, cblk_end): narrow   */
+                                  /* cblks of the fused one-wave kernel     */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
     int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
     int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
     int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
     int32_t ncmax;                /* widest cblk of the level               */
+    int32_t ncmax_fac;            /* widest cblk of [cblk_beg, fac_end)     */
     double  flops[3];             /* diag/trsm/gemm flops (reference model)  */
 };
 
@@ -448,20 +452,39 @@ pa_dev_setup_structure( pastix_data_t *pd )
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
     dev->levels.resize( nlev );
 
-    dev->class_flops[0] = dev->class_flops[1] = dev->class_flops[2] = 0.;
-    for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
-        /* reference flop model (symbol/symbol_cost_flops.c, common/flops.h) */
+    /* narrow cblks (LL^t) are factored and applied by one wavefront each
+     * (k_small_llt); PASTIX_AMD_SMALL=0 sends them through the general
+     * kernels (same results up to rounding) */
+    const char *sm_env = getenv( "PASTIX_AMD_SMALL" );
+    const bool  use_small = !lu && !( sm_env && sm_env[0] == '0' );
+    auto is_small = [&]( pastix_int_t k ) {
+        return use_small && c_ncol[k] <= PA_SM_NC && c_stride[k] <= PA_SM_ROWS;
+    };
+    /* reference flop model (symbol/symbol_cost_flops.c, common/flops.h);
+     * the narrow cblks' flops all count in the diagonal class, whose
+     * launch slot runs their fused kernel */
+    auto cblk_flops = [&]( pastix_int_t k, double *f ) {
         double N = c_ncol[k], M = c_stride[k] - c_ncol[k];
         if ( lu ) {
-            dev->class_flops[0] += pa_flops_getrf( N );
-            dev->class_flops[1] += 2. * pa_flops_trsm( M, N );
-            dev->class_flops[2] += pa_flops_gemm( M, M, N );
+            f[0] = pa_flops_getrf( N );
+            f[1] = 2. * pa_flops_trsm( M, N );
+            f[2] = pa_flops_gemm( M, M, N );
         }
         else {
-            dev->class_flops[0] += pa_flops_potrf( N );
-            dev->class_flops[1] += pa_flops_trsm( M, N );
-            dev->class_flops[2] += pa_flops_syrk( N, M );
+            f[0] = pa_flops_potrf( N );
+            f[1] = pa_flops_trsm( M, N );
+            f[2] = pa_flops_syrk( N, M );
         }
+        if ( is_small( k ) ) {
+            f[0] += f[1] + f[2];
+            f[1] = f[2] = 0.;
+        }
+    };
+    dev->class_flops[0] = dev->class_flops[1] = dev->class_flops[2] = 0.;
+    for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
+        double f[3];
+        cblk_flops( k, f );
+        for ( int c = 0; c < 3; c++ ) dev->class_flops[c] += f[c];
     }
     for ( pastix_int_t l = 0; l < nlev; l++ ) {
         pa_level_t &L = dev->levels[l];
@@ -472,17 +495,25 @@ pa_dev_setup_structure( pastix_data_t *pd )
         L.solve_beg = solve_tasks.size();
         L.bsolve_beg = bsolve_tasks.size();
         L.ncmax     = 0;
+        L.ncmax_fac = 0;
         L.flops[0] = L.flops[1] = L.flops[2] = 0.;
+        /* the level's cblks: general ones first, then the narrow ones */
+        for ( int32_t k : bylev[l] )
+            if ( !is_small( k ) ) {
+                lev_cblks.push_back( k );
+                L.ncmax_fac = std::max( L.ncmax_fac, c_ncol[k] );
+            }
+        L.fac_end = lev_cblks.size();
+        for ( int32_t k : bylev[l] )
+            if ( is_small( k ) ) lev_cblks.push_back( k );
         /* number of distinct source cblks of this level per target cblk */
         std::vector<int32_t> touched;
         for ( int32_t k : bylev[l] ) {
-            lev_cblks.push_back( k );
             L.ncmax = std::max( L.ncmax, c_ncol[k] );
             {
-                double N = c_ncol[k], M = c_stride[k] - c_ncol[k];
-                L.flops[0] += lu ? pa_flops_getrf( N ) : pa_flops_potrf( N );
-                L.flops[1] += lu ? 2. * pa_flops_trsm( M, N ) : pa_flops_trsm( M, N );
-                L.flops[2] += lu ? pa_flops_gemm( M, M, N ) : pa_flops_syrk( N, M );
+                double f[3];
+                cblk_flops( k, f );
+                for ( int c = 0; c < 3; c++ ) L.flops[c] += f[c];
             }
             for ( int32_t b = c_fblok[k] + 1; b < c_fblok[k + 1]; b++ ) {
                 int32_t F = b_fcblk[b];
@@ -494,7 +525,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
         }
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
-            for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
             /* solve update tasks: forward 256 rows x 64 columns; backward
              * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
             for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
@@ -502,6 +532,8 @@ pa_dev_setup_structure( pastix_data_t *pd )
             const int32_t ncg = PA_SB_NCG( nc );
             for ( int32_t r = 0; r < R; r += 1024 / ncg )
                 for ( int32_t c = 0; c < nc; c += 16 * ncg ) bsolve_tasks.push_back( make_int4( k, r, c, 0 ) );
+            if ( is_small( k ) ) continue; /* factored and applied by k_small_llt */
+            for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
             /* column groups: consecutive blocks facing the same cblk */
             int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
             while ( b < lb ) {
@@ -687,17 +719,25 @@ pa_factorize_t( pa_device_s *dev )
     const int lu = dev->lu;
     for ( size_t li = 0; li < dev->levels.size(); li++ ) {
         const pa_level_t &lev = dev->levels[li];
-        int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
-        if ( ncb == 0 ) continue;
+        int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg ), nsm = (int32_t)( lev.cblk_end - lev.fac_end );
+        if ( ncb == 0 && nsm == 0 ) continue;
         PA_PROF_MARK( 0 );
-        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax * sizeof( T );
+        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax_fac * sizeof( T );
         const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
-        const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax > 64 && lev.ncmax <= 16 * PA_DREG_NT;
-        if ( regdiag ) {
+        if ( nsm > 0 ) {
+            k_small_llt<T><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm, L, crit,
+                                                                     c0, c1 );
+            PA_HIP( hipGetLastError() );
+        }
+        const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax_fac > 64 && lev.ncmax_fac <= 16 * PA_DREG_NT;
+        if ( ncb == 0 ) {
+            /* only narrow cblks at this level */
+        }
+        else if ( regdiag ) {
             k_diag_potrf_reg<<<ncb, 1024, 0, dev->stream>>>( dev->S, cb, (float *)L, crit, c0, c1 );
         }
-        else if ( lev.ncmax > 64 ) {
+        else if ( lev.ncmax_fac > 64 ) {
             if ( lu ) k_diag<T, true, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
             else      k_diag<T, false, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
         }

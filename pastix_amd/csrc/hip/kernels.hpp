@@ -592,6 +592,167 @@ k_diag_potrf_reg_one( float *D, int32_t nc, double crit, int32_t *npiv, int32_t 
 }
 
 /* ------------------------------------------------------------------------- */
+/* Narrow supernodes (LL^t): one wavefront factors a whole cblk of at most   */
+/* PA_SM_NC columns and PA_SM_ROWS rows and applies its update, fusing the   */
+/* work of core_zpotrfsp (kernels/core_zpotrfsp.c:68-107), the panel TRSM    */
+/* (kernels/core_ztrsmsp.c:149-180) and the scatter-GEMM of                  */
+/* core_zgemmsp_2d2d (kernels/core_zgemmsp.c:404-481) for the narrow cblks   */
+/* that dominate the lower levels of the elimination tree (2D problems,      */
+/* batched systems): one 256-thread workgroup per cblk there would leave     */
+/* most lanes idle.                                                          */
+/*                                                                           */
+/* Lane l holds panel rows l, l+64, l+128 (diagonal rows first) in           */
+/* registers; column j is eliminated right-looking over the whole panel:     */
+/* static pivot check and sqrt on the diagonal, the rows below divided by    */
+/* it, then the later columns updated with L[c][j] broadcast by readlane.    */
+/* The update C[a][b] = sum_c X[a][c] X[b][c] (a >= b, off-diagonal rows)    */
+/* is scattered into the facing cblks with atomic adds: the target row of    */
+/* each lane is found by a binary search over the facing cblk's blocks.      */
+/* ------------------------------------------------------------------------- */
+#define PA_SM_NC   16
+#define PA_SM_ROWS 192
+
+template <typename T>
+__global__ __launch_bounds__( 256 ) void
+k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const double *critp, int32_t *npiv,
+             int32_t *nneg )
+{
+    const int     lane = threadIdx.x & 63;
+    const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
+    if ( idx >= ncb ) return; /* whole waves only; no workgroup barrier below */
+    const int32_t k = cblks[idx];
+    const int32_t nc = S.c_ncol[k], stride = S.c_stride[k], R = stride - nc;
+    const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
+    const T       tcrit = (T)*critp;
+
+    T       P[3][PA_SM_NC];
+    int64_t rbase[3];
+    int32_t rld[3], grow[3];
+    bool    valid[3];
+#pragma unroll
+    for ( int q = 0; q < 3; q++ ) {
+        const int32_t r = lane + 64 * q;
+        valid[q] = r < stride;
+        rbase[q] = coef + r;
+        rld[q]   = nc;
+        grow[q]  = 0;
+        if ( valid[q] && r >= nc ) {
+            const int4 ri = S.r_info[rb0 + ( r - nc )];
+            rbase[q] = pa_rinfo_off( ri );
+            rld[q]   = ri.y;
+            grow[q]  = ri.x;
+        }
+#pragma unroll
+        for ( int c = 0; c < PA_SM_NC; c++ )
+            P[q][c] = ( valid[q] && c < nc ) ? Lc[rbase[q] + (int64_t)c * rld[q]] : T( 0 );
+    }
+
+    /* panel Cholesky, right-looking over the nc columns */
+    int32_t mypiv = 0, myneg = 0;
+#pragma unroll
+    for ( int j = 0; j < PA_SM_NC; j++ ) {
+        if ( j < nc ) {
+            T d = pa_readlane( P[0][j], j );
+            const bool rep = pa_abs( d ) < tcrit;
+            if ( rep ) d = tcrit;
+            mypiv += rep ? 1 : 0;
+            myneg += ( d < T( 0 ) ) ? 1 : 0;
+            d = sqrt( d );
+#pragma unroll
+            for ( int q = 0; q < 3; q++ ) {
+                const int32_t r = lane + 64 * q;
+                if ( r == j ) P[q][j] = d;
+                else if ( r > j ) P[q][j] = P[q][j] / d;
+            }
+#pragma unroll
+            for ( int c = 1; c < PA_SM_NC; c++ ) {
+                if ( c > j && c < nc ) {
+                    const T f = pa_readlane( P[0][j], c ); /* L[c][j] */
+#pragma unroll
+                    for ( int q = 0; q < 3; q++ )
+                        if ( lane + 64 * q > j ) P[q][c] -= P[q][j] * f;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for ( int q = 0; q < 3; q++ ) {
+        const int32_t r = lane + 64 * q;
+        if ( valid[q] ) {
+#pragma unroll
+            for ( int c = 0; c < PA_SM_NC; c++ )
+                if ( c < nc && ( r >= nc || c <= r ) ) Lc[rbase[q] + (int64_t)c * rld[q]] = P[q][c];
+        }
+    }
+    if ( lane == 0 ) {
+        if ( mypiv ) atomicAdd( npiv, mypiv );
+        if ( myneg ) atomicAdd( nneg, myneg );
+    }
+
+    /* scatter update into the facing cblks */
+    int32_t F = -1, fcolF = 0;
+    int64_t tbase[3] = { 0, 0, 0 };
+    int32_t tld[3]   = { 0, 0, 0 };
+    bool    tok[3]   = { false, false, false };
+    for ( int32_t b = 0; b < R; b++ ) {
+        const int32_t rbrow = nc + b, lb = rbrow & 63, qb = rbrow >> 6;
+        const int32_t Fb    = S.b_fcblk[S.r_blk[rb0 + b]];
+        const int32_t growb = S.r_info[rb0 + b].x;
+        if ( Fb != F ) {
+            /* rows of this lane in the new facing cblk */
+            F     = Fb;
+            fcolF = S.c_fcol[F];
+            const int32_t fb0 = S.c_fblok[F], fb1 = S.c_fblok[F + 1], ncF = S.c_ncol[F];
+            const int64_t cF = S.c_coef[F];
+#pragma unroll
+            for ( int q = 0; q < 3; q++ ) {
+                tok[q] = false;
+                const int32_t r = lane + 64 * q;
+                if ( valid[q] && r >= rbrow ) {
+                    const int32_t g = grow[q];
+                    int32_t lo = fb0, hi = fb1 - 1;
+                    while ( lo < hi ) { /* last block with frow <= g */
+                        const int32_t mid = ( lo + hi + 1 ) >> 1;
+                        if ( S.b_frow[mid] <= g ) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    if ( S.b_frow[lo] <= g && g <= S.b_lrow[lo] ) {
+                        tok[q]   = true;
+                        tld[q]   = S.b_lrow[lo] - S.b_frow[lo] + 1;
+                        tbase[q] = cF + (int64_t)S.b_rowoff[lo] * ncF + ( g - S.b_frow[lo] );
+                    }
+                }
+            }
+        }
+        /* row b broadcast from its lane (uniform branch on its register slot) */
+        T xb[PA_SM_NC];
+        if ( qb == 0 ) {
+#pragma unroll
+            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[0][c], lb );
+        }
+        else if ( qb == 1 ) {
+#pragma unroll
+            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[1][c], lb );
+        }
+        else {
+#pragma unroll
+            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[2][c], lb );
+        }
+        const int32_t col = growb - fcolF;
+#pragma unroll
+        for ( int q = 0; q < 3; q++ ) {
+            const int32_t r = lane + 64 * q;
+            if ( tok[q] && r >= rbrow ) {
+                T acc = T( 0 );
+#pragma unroll
+                for ( int c = 0; c < PA_SM_NC; c++ ) acc += P[q][c] * xb[c];
+                pa_atomic_add( Lc + tbase[q] + (int64_t)col * tld[q], -acc );
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Panel TRSM on the off-diagonal rows of a cblk (reference                   */
 /* kernels/core_ztrsmsp.c:149-180):                                          */
 /*   MODE 0 (POTRF):  X <- X L^{-T}, L lower non-unit diagonal tile          */
