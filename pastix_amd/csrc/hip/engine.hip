@@ -525,6 +525,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         }
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
+            if ( is_small( k ) ) continue; /* k_small_llt / k_small_solve */
             /* solve update tasks: forward 256 rows x 64 columns; backward
              * 16 ncg columns x 1024 / ncg rows (4 waves of 16 x 256) */
             for ( int32_t r = 0; r < R; r += PA_SF_ROWS )
@@ -532,7 +533,6 @@ pa_dev_setup_structure( pastix_data_t *pd )
             const int32_t ncg = PA_SB_NCG( nc );
             for ( int32_t r = 0; r < R; r += 1024 / ncg )
                 for ( int32_t c = 0; c < nc; c += 16 * ncg ) bsolve_tasks.push_back( make_int4( k, r, c, 0 ) );
-            if ( is_small( k ) ) continue; /* factored and applied by k_small_llt */
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
             /* column groups: consecutive blocks facing the same cblk */
             int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
@@ -899,20 +899,23 @@ static int
 pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off, T *x,
                      int32_t nrhs, int64_t ld )
 {
-    const int32_t ncb = (int32_t)( lev.cblk_end - lev.cblk_beg );
+    /* the general cblks of the level; the narrow ones go to k_small_solve */
+    const int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg );
     const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
-    if ( lev.ncmax <= PA_SD_MAXCOL ) {
+    const int32_t ncmax = lev.ncmax_fac;
+    if ( ncb == 0 ) return PASTIX_SUCCESS;
+    if ( ncmax <= PA_SD_MAXCOL ) {
         const size_t  avail  = (size_t)dev->lds_limit - 4096;
-        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)lev.ncmax * sizeof( T ) ) ) );
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)ncmax * sizeof( T ) ) ) );
         dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
-        size_t        lds = (size_t)lev.ncmax * rchunk * sizeof( T );
-        if ( lev.ncmax <= 64 )       pa_solve_diag_launch<T, BWD, UNIT, 64>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else if ( lev.ncmax <= 128 ) pa_solve_diag_launch<T, BWD, UNIT, 128>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else if ( lev.ncmax <= 256 ) pa_solve_diag_launch<T, BWD, UNIT, 256>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else                         pa_solve_diag_launch<T, BWD, UNIT, 512>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        size_t        lds = (size_t)ncmax * rchunk * sizeof( T );
+        if ( ncmax <= 64 )       pa_solve_diag_launch<T, BWD, UNIT, 64>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else if ( ncmax <= 128 ) pa_solve_diag_launch<T, BWD, UNIT, 128>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else if ( ncmax <= 256 ) pa_solve_diag_launch<T, BWD, UNIT, 256>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        else                     pa_solve_diag_launch<T, BWD, UNIT, 512>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
     }
     else {
-        size_t lds = (size_t)lev.ncmax * nrhs * sizeof( T );
+        size_t lds = (size_t)ncmax * nrhs * sizeof( T );
         if ( lds > (size_t)dev->lds_limit - 4096 ) return PASTIX_ERR_BADPARAMETER;
         k_solve_diag<T, BWD, UNIT><<<ncb, 256, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, ld );
     }
@@ -938,6 +941,10 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         int64_t ns = lev.solve_end - lev.solve_beg;
         if ( ns > 0 )
             k_solve_fwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x, nrhs, ld );
+        const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
+        if ( nsm > 0 )
+            k_small_solve<T, false><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm,
+                                                                              Cf, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
     }
     for ( size_t li = nl; li-- > 0; ) {
@@ -946,6 +953,10 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         int64_t ns = lev.bsolve_end - lev.bsolve_beg;
         if ( ns > 0 )
             k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb, x, nrhs, ld );
+        const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
+        if ( nsm > 0 )
+            k_small_solve<T, true><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm,
+                                                                             Cb, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
         rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld )
                     : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld );

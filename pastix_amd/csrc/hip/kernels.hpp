@@ -1672,6 +1672,108 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 }
 
 /* ------------------------------------------------------------------------- */
+/* Supernodal solve of the narrow LL^t cblks (those of k_small_llt), one     */
+/* wavefront per cblk for all right-hand sides: the panel is read once into  */
+/* registers (lane l: rows l, l+64, l+128; lanes < nc also the columns of    */
+/* the diagonal block) and reused for every RHS.  Forward (reference          */
+/* kernels/solve_ztrsmsp.c:302-466): x_k = L_kk^{-1} x_k, then the rows      */
+/* below are updated with atomic adds.  Backward (:468-661): x_k -= L_bk^T   */
+/* x_b gathered from the final rows below (transposing wave reduction), then */
+/* x_k = L_kk^{-T} x_k.                                                      */
+/* ------------------------------------------------------------------------- */
+template <typename T, bool BWD>
+__global__ __launch_bounds__( 256 ) void
+k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx )
+{
+    const int     lane = threadIdx.x & 63;
+    const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
+    if ( idx >= ncb ) return; /* whole waves only; no workgroup barrier below */
+    const int32_t k = cblks[idx];
+    const int32_t nc = S.c_ncol[k], stride = S.c_stride[k], fcol = S.c_fcol[k];
+    const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
+
+    /* off-diagonal rows: slots q with row lane + 64 q in [nc, stride) */
+    T       P[3][PA_SM_NC];
+    int32_t grow[3];
+    bool    offd[3];
+#pragma unroll
+    for ( int q = 0; q < 3; q++ ) {
+        const int32_t r = lane + 64 * q;
+        offd[q]          = r >= nc && r < stride;
+        int64_t base     = 0;
+        int32_t ld       = 1;
+        grow[q]          = 0;
+        if ( offd[q] ) {
+            const int4 ri = S.r_info[rb0 + ( r - nc )];
+            base    = pa_rinfo_off( ri );
+            ld      = ri.y;
+            grow[q] = ri.x;
+        }
+#pragma unroll
+        for ( int c = 0; c < PA_SM_NC; c++ ) P[q][c] = ( offd[q] && c < nc ) ? Lc[base + (int64_t)c * ld] : T( 0 );
+    }
+    /* diagonal block: row `lane` (forward) or column `lane` (backward) */
+    T D[PA_SM_NC];
+#pragma unroll
+    for ( int c = 0; c < PA_SM_NC; c++ ) {
+        const bool in = lane < nc && c < nc;
+        D[c] = in ? ( BWD ? Lc[coef + (int64_t)lane * nc + c] : Lc[coef + (int64_t)c * nc + lane] ) : T( 0 );
+    }
+
+    for ( int32_t r = 0; r < nrhs; r++ ) {
+        T *xr = x + (int64_t)r * ldx;
+        T  xv = ( lane < nc ) ? xr[fcol + lane] : T( 0 );
+        if ( !BWD ) {
+            T xs[PA_SM_NC];
+#pragma unroll
+            for ( int j = 0; j < PA_SM_NC; j++ ) {
+                xs[j] = T( 0 );
+                if ( j < nc ) {
+                    const T xj = pa_readlane( xv, j ) / pa_readlane( D[j], j );
+                    xs[j]      = xj;
+                    if ( lane == j ) xv = xj;
+                    else if ( lane > j ) xv -= D[j] * xj; /* L[lane][j] x_j */
+                }
+            }
+            if ( lane < nc ) xr[fcol + lane] = xv;
+#pragma unroll
+            for ( int q = 0; q < 3; q++ ) {
+                if ( offd[q] ) {
+                    T acc = T( 0 );
+#pragma unroll
+                    for ( int c = 0; c < PA_SM_NC; c++ ) acc += P[q][c] * xs[c];
+                    pa_atomic_add( xr + grow[q], -acc );
+                }
+            }
+        }
+        else {
+            T v[PA_SM_NC];
+#pragma unroll
+            for ( int c = 0; c < PA_SM_NC; c++ ) v[c] = T( 0 );
+#pragma unroll
+            for ( int q = 0; q < 3; q++ ) {
+                const T y = offd[q] ? xr[grow[q]] : T( 0 );
+#pragma unroll
+                for ( int c = 0; c < PA_SM_NC; c++ ) v[c] += P[q][c] * y;
+            }
+            const T s  = pa_wave_sum16<T>( v, lane );     /* lane 4c: column c */
+            const T sc = __shfl( s, ( lane & 15 ) << 2 );
+            xv -= sc;
+            /* x_k = L_kk^{-T} x_k: lane c holds column c of L_kk in D */
+#pragma unroll
+            for ( int j = PA_SM_NC - 1; j >= 0; j-- ) {
+                if ( j < nc ) {
+                    const T xj = pa_readlane( xv, j ) / pa_readlane( D[j], j );
+                    if ( lane == j ) xv = xj;
+                    else if ( lane < j ) xv -= D[j] * xj; /* L[j][lane] x_j */
+                }
+            }
+            if ( lane < nc ) xr[fcol + lane] = xv;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* SpMV on the permuted CSC (reference bcsc/bcsc_zspmv.c:57-200):            */
 /* y = alpha op(A) x + beta y.  With vals = Uvalues (A^T by columns) a       */
 /* column of the CSC is a row of A, so each output is a gathered dot product */

@@ -1,7 +1,8 @@
 """The fused one-wave kernel of the narrow cblks (k_small_llt) gives the same
 factors as the general diagonal / TRSM / scatter-GEMM kernels (switched off
 with PASTIX_AMD_SMALL=0), in FP64 and FP32, on 2D and 3D grids where most
-cblks are narrow; both agree with the CPU oracle."""
+cblks are narrow; both agree with the CPU oracle, and so do the multi-RHS
+solves of the narrow cblks (k_small_solve) and of the general kernels."""
 import os
 
 import numpy as np
@@ -67,10 +68,13 @@ def test_narrow_cblk_kernel_matches_general_kernels(grid, mixed):
             Lo, _, _ = orc.factorize(S, A, perm, 0, crit)
             assert np.abs(L1 - Lo[keep]).max() <= 1e-12 * np.abs(Lo).max()
         n = A.shape[0]
-        b = problems.uniform(n, seed=2)
-        x = b.copy()
-        pa.task_solve(h1, s1, x)
-        assert problems.backward_error(A, x, b) <= (1e-6 if mixed else 1e-14)
+        B = np.asfortranarray(np.random.default_rng(2).uniform(-1, 1, (n, 5)))
+        X1, X0 = B.copy(order="F"), B.copy(order="F")
+        pa.task_solve(h1, s1, X1)                  # narrow cblks: k_small_solve, 5 RHS at once
+        pa.task_solve(h0, s0, X0)
+        assert np.abs(X1 - X0).max() <= (1e-4 if mixed else 1e-12) * np.abs(X0).max()
+        for j in range(5):
+            assert problems.backward_error(A, X1[:, j], B[:, j]) <= (1e-6 if mixed else 1e-14)
     finally:
         pa.finalize(h1)
         pa.finalize(h0)
