@@ -44,7 +44,9 @@ struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
     int64_t fac_end;              /* [cblk_beg, fac_end): diag/TRSM/GEMM    */
                                   /* kernels; [fac_end, cblk_end): narrow   */
-                                  /* cblks of the fused one-wave kernel     */
+                                  /* cblks of the fused one-wave kernels,   */
+    int64_t sm_end[3];            /* by row slots: [fac_end, sm_end[0]) of  */
+                                  /* <= 64 rows, then <= 128, <= 192        */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
     int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
     int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
@@ -504,8 +506,11 @@ pa_dev_setup_structure( pastix_data_t *pd )
                 L.ncmax_fac = std::max( L.ncmax_fac, c_ncol[k] );
             }
         L.fac_end = lev_cblks.size();
-        for ( int32_t k : bylev[l] )
-            if ( is_small( k ) ) lev_cblks.push_back( k );
+        for ( int ns = 1; ns <= 3; ns++ ) {
+            for ( int32_t k : bylev[l] )
+                if ( is_small( k ) && ( c_stride[k] + 63 ) / 64 == ns ) lev_cblks.push_back( k );
+            L.sm_end[ns - 1] = lev_cblks.size();
+        }
         /* number of distinct source cblks of this level per target cblk */
         std::vector<int32_t> touched;
         for ( int32_t k : bylev[l] ) {
@@ -710,6 +715,43 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
         if ( dev->profiling ) PA_HIP( hipEventRecord( dev->prof_ev[6 * li + ( slot )], dev->stream ) ); \
     } while ( 0 )
 
+/* the narrow cblks of a level, one launch per row-slot count */
+template <typename T>
+static void
+pa_small_llt_launch( pa_device_s *dev, const pa_level_t &lev, T *L, const double *crit, int32_t *c0, int32_t *c1 )
+{
+    int64_t beg = lev.fac_end;
+    for ( int ns = 1; ns <= 3; ns++ ) {
+        const int32_t n = (int32_t)( lev.sm_end[ns - 1] - beg );
+        const int32_t *cb = dev->d_lev_cblks + beg;
+        const unsigned g  = (unsigned)( ( n + 3 ) / 4 );
+        if ( n > 0 ) {
+            if ( ns == 1 )      k_small_llt<T, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
+            else if ( ns == 2 ) k_small_llt<T, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
+            else                k_small_llt<T, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
+        }
+        beg = lev.sm_end[ns - 1];
+    }
+}
+
+template <typename T, bool BWD>
+static void
+pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x, int32_t nrhs, int64_t ld )
+{
+    int64_t beg = lev.fac_end;
+    for ( int ns = 1; ns <= 3; ns++ ) {
+        const int32_t n = (int32_t)( lev.sm_end[ns - 1] - beg );
+        const int32_t *cb = dev->d_lev_cblks + beg;
+        const unsigned g  = (unsigned)( ( n + 3 ) / 4 );
+        if ( n > 0 ) {
+            if ( ns == 1 )      k_small_solve<T, BWD, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
+            else if ( ns == 2 ) k_small_solve<T, BWD, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
+            else                k_small_solve<T, BWD, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
+        }
+        beg = lev.sm_end[ns - 1];
+    }
+}
+
 template <typename T>
 static int
 pa_factorize_t( pa_device_s *dev )
@@ -726,8 +768,7 @@ pa_factorize_t( pa_device_s *dev )
         const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
         if ( nsm > 0 ) {
-            k_small_llt<T><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm, L, crit,
-                                                                     c0, c1 );
+            pa_small_llt_launch<T>( dev, lev, L, crit, c0, c1 );
             PA_HIP( hipGetLastError() );
         }
         const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax_fac > 64 && lev.ncmax_fac <= 16 * PA_DREG_NT;
@@ -942,9 +983,7 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         if ( ns > 0 )
             k_solve_fwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x, nrhs, ld );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
-        if ( nsm > 0 )
-            k_small_solve<T, false><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm,
-                                                                              Cf, x, nrhs, ld );
+        if ( nsm > 0 ) pa_small_solve_launch<T, false>( dev, lev, Cf, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
     }
     for ( size_t li = nl; li-- > 0; ) {
@@ -954,9 +993,7 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         if ( ns > 0 )
             k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb, x, nrhs, ld );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
-        if ( nsm > 0 )
-            k_small_solve<T, true><<<( nsm + 3 ) / 4, 256, 0, dev->stream>>>( dev->S, dev->d_lev_cblks + lev.fac_end, nsm,
-                                                                             Cb, x, nrhs, ld );
+        if ( nsm > 0 ) pa_small_solve_launch<T, true>( dev, lev, Cb, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
         rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld )
                     : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld );

@@ -612,7 +612,7 @@ k_diag_potrf_reg_one( float *D, int32_t nc, double crit, int32_t *npiv, int32_t 
 #define PA_SM_NC   16
 #define PA_SM_ROWS 192
 
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__( 256 ) void
 k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const double *critp, int32_t *npiv,
              int32_t *nneg )
@@ -625,12 +625,12 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
     const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
     const T       tcrit = (T)*critp;
 
-    T       P[3][PA_SM_NC];
-    int64_t rbase[3];
-    int32_t rld[3], grow[3];
-    bool    valid[3];
+    T       P[NS][PA_SM_NC];
+    int64_t rbase[NS];
+    int32_t rld[NS], grow[NS];
+    bool    valid[NS];
 #pragma unroll
-    for ( int q = 0; q < 3; q++ ) {
+    for ( int q = 0; q < NS; q++ ) {
         const int32_t r = lane + 64 * q;
         valid[q] = r < stride;
         rbase[q] = coef + r;
@@ -659,7 +659,7 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
             myneg += ( d < T( 0 ) ) ? 1 : 0;
             d = sqrt( d );
 #pragma unroll
-            for ( int q = 0; q < 3; q++ ) {
+            for ( int q = 0; q < NS; q++ ) {
                 const int32_t r = lane + 64 * q;
                 if ( r == j ) P[q][j] = d;
                 else if ( r > j ) P[q][j] = P[q][j] / d;
@@ -669,14 +669,14 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
                 if ( c > j && c < nc ) {
                     const T f = pa_readlane( P[0][j], c ); /* L[c][j] */
 #pragma unroll
-                    for ( int q = 0; q < 3; q++ )
+                    for ( int q = 0; q < NS; q++ )
                         if ( lane + 64 * q > j ) P[q][c] -= P[q][j] * f;
                 }
             }
         }
     }
 #pragma unroll
-    for ( int q = 0; q < 3; q++ ) {
+    for ( int q = 0; q < NS; q++ ) {
         const int32_t r = lane + 64 * q;
         if ( valid[q] ) {
 #pragma unroll
@@ -689,66 +689,84 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
         if ( myneg ) atomicAdd( nneg, myneg );
     }
 
-    /* scatter update into the facing cblks */
-    int32_t F = -1, fcolF = 0;
-    int64_t tbase[3] = { 0, 0, 0 };
-    int32_t tld[3]   = { 0, 0, 0 };
-    bool    tok[3]   = { false, false, false };
-    for ( int32_t b = 0; b < R; b++ ) {
-        const int32_t rbrow = nc + b, lb = rbrow & 63, qb = rbrow >> 6;
-        const int32_t Fb    = S.b_fcblk[S.r_blk[rb0 + b]];
-        const int32_t growb = S.r_info[rb0 + b].x;
-        if ( Fb != F ) {
-            /* rows of this lane in the new facing cblk */
-            F     = Fb;
-            fcolF = S.c_fcol[F];
-            const int32_t fb0 = S.c_fblok[F], fb1 = S.c_fblok[F + 1], ncF = S.c_ncol[F];
-            const int64_t cF = S.c_coef[F];
+    /* scatter update into the facing cblks, one group of consecutive blocks
+     * facing the same cblk F at a time.  Every block of the cblk lies inside
+     * one block of F, found by walking F's blocks in row order (the blocks
+     * of both are sorted); the lanes of the rows of block i then take that
+     * target block. */
+    int32_t bidx[NS];
 #pragma unroll
-            for ( int q = 0; q < 3; q++ ) {
-                tok[q] = false;
-                const int32_t r = lane + 64 * q;
-                if ( valid[q] && r >= rbrow ) {
-                    const int32_t g = grow[q];
-                    int32_t lo = fb0, hi = fb1 - 1;
-                    while ( lo < hi ) { /* last block with frow <= g */
-                        const int32_t mid = ( lo + hi + 1 ) >> 1;
-                        if ( S.b_frow[mid] <= g ) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    if ( S.b_frow[lo] <= g && g <= S.b_lrow[lo] ) {
-                        tok[q]   = true;
-                        tld[q]   = S.b_lrow[lo] - S.b_frow[lo] + 1;
-                        tbase[q] = cF + (int64_t)S.b_rowoff[lo] * ncF + ( g - S.b_frow[lo] );
-                    }
+    for ( int q = 0; q < NS; q++ ) {
+        const int32_t r = lane + 64 * q;
+        bidx[q] = ( valid[q] && r >= nc ) ? S.r_blk[rb0 + ( r - nc )] : -1;
+    }
+    const int32_t lblk = S.c_fblok[k + 1];
+    int32_t       gb   = S.c_fblok[k] + 1;
+    while ( gb < lblk ) {
+        const int32_t F = S.b_fcblk[gb];
+        int32_t       ge = gb;
+        while ( ge + 1 < lblk && S.b_fcblk[ge + 1] == F ) ge++;
+        const int32_t fcolF = S.c_fcol[F], ncF = S.c_ncol[F], flast = S.c_fblok[F + 1] - 1;
+        const int64_t cF = S.c_coef[F];
+        int64_t tbase[NS];
+        int32_t tld[NS];
+        bool    tok[NS];
+#pragma unroll
+        for ( int q = 0; q < NS; q++ ) {
+            tbase[q] = 0;
+            tld[q]   = 0;
+            tok[q]   = false;
+        }
+        int32_t f        = S.c_fblok[F];
+        for ( int32_t i = gb; i < lblk; i++ ) {
+            const int32_t fr = S.b_frow[i];
+            while ( f < flast && S.b_lrow[f] < fr ) f++;
+            const int32_t ffr = S.b_frow[f], flr = S.b_lrow[f];
+            const int64_t fbase = cF + (int64_t)S.b_rowoff[f] * ncF - ffr;
+#pragma unroll
+            for ( int q = 0; q < NS; q++ ) {
+                if ( bidx[q] == i && grow[q] >= ffr && grow[q] <= flr ) {
+                    tok[q]   = true;
+                    tbase[q] = fbase + grow[q];
+                    tld[q]   = flr - ffr + 1;
                 }
             }
         }
-        /* row b broadcast from its lane (uniform branch on its register slot) */
-        T xb[PA_SM_NC];
-        if ( qb == 0 ) {
+        /* the columns of F updated by this group: the rows of blocks gb..ge */
+        const int32_t b0 = S.b_rowoff[gb], b1 = S.b_rowoff[ge] + ( S.b_lrow[ge] - S.b_frow[ge] + 1 );
+        for ( int32_t rbrow = b0; rbrow < b1; rbrow++ ) {
+            const int32_t lb = rbrow & 63, qb = rbrow >> 6;
+            /* row rbrow broadcast from its lane (uniform branch on its register slot) */
+            T       xb[PA_SM_NC];
+            int32_t growb;
+            if ( qb == 0 ) {
 #pragma unroll
-            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[0][c], lb );
-        }
-        else if ( qb == 1 ) {
+                for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[0][c], lb );
+                growb = __builtin_amdgcn_readlane( grow[0], lb );
+            }
+            else if ( NS > 1 && qb == 1 ) {
 #pragma unroll
-            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[1][c], lb );
-        }
-        else {
+                for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[NS > 1 ? 1 : 0][c], lb );
+                growb = __builtin_amdgcn_readlane( grow[NS > 1 ? 1 : 0], lb );
+            }
+            else {
 #pragma unroll
-            for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[2][c], lb );
-        }
-        const int32_t col = growb - fcolF;
+                for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[NS - 1][c], lb );
+                growb = __builtin_amdgcn_readlane( grow[NS - 1], lb );
+            }
+            const int32_t col = growb - fcolF;
 #pragma unroll
-        for ( int q = 0; q < 3; q++ ) {
-            const int32_t r = lane + 64 * q;
-            if ( tok[q] && r >= rbrow ) {
-                T acc = T( 0 );
+            for ( int q = 0; q < NS; q++ ) {
+                const int32_t r = lane + 64 * q;
+                if ( tok[q] && r >= rbrow ) {
+                    T acc = T( 0 );
 #pragma unroll
-                for ( int c = 0; c < PA_SM_NC; c++ ) acc += P[q][c] * xb[c];
-                pa_atomic_add( Lc + tbase[q] + (int64_t)col * tld[q], -acc );
+                    for ( int c = 0; c < PA_SM_NC; c++ ) acc += P[q][c] * xb[c];
+                    pa_atomic_add( Lc + tbase[q] + (int64_t)col * tld[q], -acc );
+                }
             }
         }
+        gb = ge + 1;
     }
 }
 
@@ -1681,7 +1699,9 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 /* x_b gathered from the final rows below (transposing wave reduction), then */
 /* x_k = L_kk^{-T} x_k.                                                      */
 /* ------------------------------------------------------------------------- */
-template <typename T, bool BWD>
+#define PA_SM_RC 4
+
+template <typename T, bool BWD, int NS>
 __global__ __launch_bounds__( 256 ) void
 k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx )
 {
@@ -1693,11 +1713,11 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
     const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
 
     /* off-diagonal rows: slots q with row lane + 64 q in [nc, stride) */
-    T       P[3][PA_SM_NC];
-    int32_t grow[3];
-    bool    offd[3];
+    T       P[NS][PA_SM_NC];
+    int32_t grow[NS];
+    bool    offd[NS];
 #pragma unroll
-    for ( int q = 0; q < 3; q++ ) {
+    for ( int q = 0; q < NS; q++ ) {
         const int32_t r = lane + 64 * q;
         offd[q]          = r >= nc && r < stride;
         int64_t base     = 0;
@@ -1720,55 +1740,87 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
         D[c] = in ? ( BWD ? Lc[coef + (int64_t)lane * nc + c] : Lc[coef + (int64_t)c * nc + lane] ) : T( 0 );
     }
 
-    for ( int32_t r = 0; r < nrhs; r++ ) {
-        T *xr = x + (int64_t)r * ldx;
-        T  xv = ( lane < nc ) ? xr[fcol + lane] : T( 0 );
+    /* reciprocals of the diagonal, uniform across the wave */
+    T dinv[PA_SM_NC];
+#pragma unroll
+    for ( int j = 0; j < PA_SM_NC; j++ ) dinv[j] = ( j < nc ) ? T( 1 ) / pa_readlane( D[j], j ) : T( 0 );
+
+    /* right-hand sides PA_SM_RC at a time: independent chains interleave */
+    for ( int32_t r0 = 0; r0 < nrhs; r0 += PA_SM_RC ) {
+        T    xv[PA_SM_RC];
+        bool rok[PA_SM_RC];
+#pragma unroll
+        for ( int u = 0; u < PA_SM_RC; u++ ) {
+            rok[u] = r0 + u < nrhs;
+            xv[u]  = ( rok[u] && lane < nc ) ? x[(int64_t)( r0 + u ) * ldx + fcol + lane] : T( 0 );
+        }
         if ( !BWD ) {
-            T xs[PA_SM_NC];
 #pragma unroll
             for ( int j = 0; j < PA_SM_NC; j++ ) {
-                xs[j] = T( 0 );
                 if ( j < nc ) {
-                    const T xj = pa_readlane( xv, j ) / pa_readlane( D[j], j );
-                    xs[j]      = xj;
-                    if ( lane == j ) xv = xj;
-                    else if ( lane > j ) xv -= D[j] * xj; /* L[lane][j] x_j */
+#pragma unroll
+                    for ( int u = 0; u < PA_SM_RC; u++ ) {
+                        const T xj = pa_readlane( xv[u], j ) * dinv[j];
+                        if ( lane == j ) xv[u] = xj;
+                        else if ( lane > j ) xv[u] -= D[j] * xj; /* L[lane][j] x_j */
+                    }
                 }
             }
-            if ( lane < nc ) xr[fcol + lane] = xv;
 #pragma unroll
-            for ( int q = 0; q < 3; q++ ) {
-                if ( offd[q] ) {
-                    T acc = T( 0 );
+            for ( int u = 0; u < PA_SM_RC; u++ ) {
+                if ( !rok[u] ) continue;
+                T *xr = x + (int64_t)( r0 + u ) * ldx;
+                if ( lane < nc ) xr[fcol + lane] = xv[u];
+                T acc[NS];
 #pragma unroll
-                    for ( int c = 0; c < PA_SM_NC; c++ ) acc += P[q][c] * xs[c];
-                    pa_atomic_add( xr + grow[q], -acc );
+                for ( int q = 0; q < NS; q++ ) acc[q] = T( 0 );
+#pragma unroll
+                for ( int c = 0; c < PA_SM_NC; c++ ) {
+                    if ( c < nc ) {
+                        const T xc = pa_readlane( xv[u], c );
+#pragma unroll
+                        for ( int q = 0; q < NS; q++ ) acc[q] += P[q][c] * xc;
+                    }
                 }
+#pragma unroll
+                for ( int q = 0; q < NS; q++ )
+                    if ( offd[q] ) pa_atomic_add( xr + grow[q], -acc[q] );
             }
         }
         else {
-            T v[PA_SM_NC];
+            T y[PA_SM_RC][NS];
 #pragma unroll
-            for ( int c = 0; c < PA_SM_NC; c++ ) v[c] = T( 0 );
+            for ( int u = 0; u < PA_SM_RC; u++ )
 #pragma unroll
-            for ( int q = 0; q < 3; q++ ) {
-                const T y = offd[q] ? xr[grow[q]] : T( 0 );
+                for ( int q = 0; q < NS; q++ )
+                    y[u][q] = ( rok[u] && offd[q] ) ? x[(int64_t)( r0 + u ) * ldx + grow[q]] : T( 0 );
 #pragma unroll
-                for ( int c = 0; c < PA_SM_NC; c++ ) v[c] += P[q][c] * y;
+            for ( int u = 0; u < PA_SM_RC; u++ ) {
+                T v[PA_SM_NC];
+#pragma unroll
+                for ( int c = 0; c < PA_SM_NC; c++ ) {
+                    v[c] = T( 0 );
+#pragma unroll
+                    for ( int q = 0; q < NS; q++ ) v[c] += P[q][c] * y[u][q];
+                }
+                const T s = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c */
+                xv[u] -= __shfl( s, ( lane & 15 ) << 2 );
             }
-            const T s  = pa_wave_sum16<T>( v, lane );     /* lane 4c: column c */
-            const T sc = __shfl( s, ( lane & 15 ) << 2 );
-            xv -= sc;
             /* x_k = L_kk^{-T} x_k: lane c holds column c of L_kk in D */
 #pragma unroll
             for ( int j = PA_SM_NC - 1; j >= 0; j-- ) {
                 if ( j < nc ) {
-                    const T xj = pa_readlane( xv, j ) / pa_readlane( D[j], j );
-                    if ( lane == j ) xv = xj;
-                    else if ( lane < j ) xv -= D[j] * xj; /* L[j][lane] x_j */
+#pragma unroll
+                    for ( int u = 0; u < PA_SM_RC; u++ ) {
+                        const T xj = pa_readlane( xv[u], j ) * dinv[j];
+                        if ( lane == j ) xv[u] = xj;
+                        else if ( lane < j ) xv[u] -= D[j] * xj; /* L[j][lane] x_j */
+                    }
                 }
             }
-            if ( lane < nc ) xr[fcol + lane] = xv;
+#pragma unroll
+            for ( int u = 0; u < PA_SM_RC; u++ )
+                if ( rok[u] && lane < nc ) x[(int64_t)( r0 + u ) * ldx + fcol + lane] = xv[u];
         }
     }
 }
