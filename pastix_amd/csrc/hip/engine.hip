@@ -980,8 +980,12 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
                     : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld );
         if ( rc ) return rc;
         int64_t ns = lev.solve_end - lev.solve_beg;
-        if ( ns > 0 )
-            k_solve_fwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x, nrhs, ld );
+        if ( ns > 0 && nrhs == 1 )
+            k_solve_fwd_update<T, 1><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x,
+                                                                            nrhs, ld );
+        else if ( ns > 0 )
+            k_solve_fwd_update<T, 4><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x,
+                                                                            nrhs, ld );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
         if ( nsm > 0 ) pa_small_solve_launch<T, false>( dev, lev, Cf, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
@@ -990,8 +994,12 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         const pa_level_t &lev = dev->levels[li];
         if ( lev.cblk_end == lev.cblk_beg ) continue;
         int64_t ns = lev.bsolve_end - lev.bsolve_beg;
-        if ( ns > 0 )
-            k_solve_bwd_update<T><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb, x, nrhs, ld );
+        if ( ns > 0 && nrhs == 1 )
+            k_solve_bwd_update<T, false><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb,
+                                                                                x, nrhs, ld );
+        else if ( ns > 0 )
+            k_solve_bwd_update<T, true><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb,
+                                                                               x, nrhs, ld );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
         if ( nsm > 0 ) pa_small_solve_launch<T, true>( dev, lev, Cb, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
@@ -1094,11 +1102,11 @@ pa_dev_apply_perm( pastix_data_t *pd, int forward, int32_t nrhs, const double *d
     pa_device_s *dev = DEV( pd );
     int64_t n = dev->n;
     if ( n == 0 ) return PASTIX_SUCCESS;
-    /* forward: x_perm[i] = x[peritab[i]]  (P b); backward: x[peritab[i]] = x_perm[i] */
-    if ( forward )
-        k_gather<double, double><<<pa_grid( n * nrhs ), 256, 0, dev->stream>>>( n, nrhs, dev->d_invp, d_src, lds, d_dst, ldd );
-    else
-        k_scatter<double, double><<<pa_grid( n * nrhs ), 256, 0, dev->stream>>>( n, nrhs, dev->d_invp, d_src, lds, d_dst, ldd );
+    /* forward: x_perm[i] = x[peritab[i]]  (P b); backward: x[j] = x_perm[permtab[j]]
+     * (both as gathers: coalesced writes) */
+    dim3 grid( std::min<unsigned>( pa_grid( n ), 4096 ), (unsigned)std::min<int32_t>( nrhs, 64 ) );
+    k_gather<double, double><<<grid, 256, 0, dev->stream>>>( n, nrhs, forward ? dev->d_invp : dev->d_perm, d_src, lds,
+                                                             d_dst, ldd );
     PA_HIP( hipGetLastError() );
     return PASTIX_SUCCESS;
 }

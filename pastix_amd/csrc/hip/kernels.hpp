@@ -1615,11 +1615,11 @@ k_diag_transpose( pa_dstruct S, const T *Coef, T *DTab )
 #define PA_SF_COLS 64
 #define PA_SB_NCG( nc ) ( ( nc ) <= 16 ? 1 : ( ( nc ) <= 32 ? 2 : 4 ) )
 
-template <typename T>
+template <typename T, int RC>
 __global__ __launch_bounds__( 256 ) void
 k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
-    __shared__ T xk[PA_SF_COLS];
+    __shared__ T xk[RC][PA_SF_COLS];
     const int4    tk = tasks[blockIdx.x];
     const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
     const int32_t R = S.c_stride[k] - nc, c0 = tk.z, ncols = min( PA_SF_COLS, nc - c0 );
@@ -1629,27 +1629,42 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     int4 ri = make_int4( 0, 1, 0, 0 );
     if ( valid ) ri = S.r_info[S.r_base[k] + v];
     const T *A = Coef + pa_rinfo_off( ri ) + (int64_t)c0 * ri.y;
-    for ( int32_t r = 0; r < nrhs; r++ ) {
-        if ( t < ncols ) xk[t] = x[fcol + c0 + t + r * ldx];
+    /* right-hand sides RC at a time: each coefficient is read once per chunk */
+    for ( int32_t r0 = 0; r0 < nrhs; r0 += RC ) {
+        const int32_t nr = min( RC, nrhs - r0 );
+        for ( int32_t idx = t; idx < ncols * nr; idx += 256 ) {
+            const int32_t c = idx % ncols, u = idx / ncols;
+            xk[u][c] = x[fcol + c0 + c + (int64_t)( r0 + u ) * ldx];
+        }
         __syncthreads();
         if ( valid ) {
-            T acc = T( 0 );
+            T acc[RC];
+#pragma unroll
+            for ( int u = 0; u < RC; u++ ) acc[u] = T( 0 );
             int32_t c = 0;
             for ( ; c + 16 <= ncols; c += 16 ) {
                 T a[16];
 #pragma unroll
-                for ( int u = 0; u < 16; u++ ) a[u] = A[(int64_t)( c + u ) * ri.y];
+                for ( int w = 0; w < 16; w++ ) a[w] = A[(int64_t)( c + w ) * ri.y];
 #pragma unroll
-                for ( int u = 0; u < 16; u++ ) acc += a[u] * xk[c + u];
+                for ( int w = 0; w < 16; w++ )
+#pragma unroll
+                    for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xk[u][c + w];
             }
-            for ( ; c < ncols; c++ ) acc += A[(int64_t)c * ri.y] * xk[c];
-            pa_atomic_add( x + ri.x + r * ldx, -acc );
+            for ( ; c < ncols; c++ ) {
+                const T aw = A[(int64_t)c * ri.y];
+#pragma unroll
+                for ( int u = 0; u < RC; u++ ) acc[u] += aw * xk[u][c];
+            }
+#pragma unroll
+            for ( int u = 0; u < RC; u++ )
+                if ( u < nr ) pa_atomic_add( x + ri.x + (int64_t)( r0 + u ) * ldx, -acc[u] );
         }
         __syncthreads();
     }
 }
 
-template <typename T>
+template <typename T, bool MR>
 __global__ __launch_bounds__( 256 ) void
 k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
 {
@@ -1669,14 +1684,15 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
         ok[q] = ( row < R ) && ( ncols > 0 );
         ri[q] = ok[q] ? info[row] : make_int4( 0, 1, 0, 0 );
     }
-    for ( int32_t r = 0; r < nrhs; r++ ) {
+    const int col = lane >> 2;
+    if ( !MR ) { /* one right-hand side */
         T v[16];
 #pragma unroll
         for ( int cc = 0; cc < 16; cc++ ) v[cc] = T( 0 );
 #pragma unroll
         for ( int q = 0; q < 4; q++ ) {
             if ( ok[q] ) {
-                const T  xv = x[ri[q].x + r * ldx];
+                const T  xv = x[ri[q].x];
                 const T *A  = Coef + pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y;
 #pragma unroll
                 for ( int cc = 0; cc < 16; cc++ )
@@ -1684,7 +1700,25 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
             }
         }
         T s = pa_wave_sum16<T>( v, lane );
-        const int col = lane >> 2;
+        if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col, -s );
+        return;
+    }
+    /* several right-hand sides: the tile is read once into registers */
+    T a[4][16];
+#pragma unroll
+    for ( int q = 0; q < 4; q++ ) {
+        const T *A = Coef + pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y;
+#pragma unroll
+        for ( int cc = 0; cc < 16; cc++ ) a[q][cc] = ( ok[q] && cc < ncols ) ? A[(int64_t)cc * ri[q].y] : T( 0 );
+    }
+    for ( int32_t r = 0; r < nrhs; r++ ) {
+        T xv[4];
+#pragma unroll
+        for ( int q = 0; q < 4; q++ ) xv[q] = ok[q] ? x[ri[q].x + r * ldx] : T( 0 );
+        T v[16];
+#pragma unroll
+        for ( int cc = 0; cc < 16; cc++ ) v[cc] = a[0][cc] * xv[0] + a[1][cc] * xv[1] + a[2][cc] * xv[2] + a[3][cc] * xv[3];
+        T s = pa_wave_sum16<T>( v, lane );
         if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col + r * ldx, -s );
     }
 }
@@ -1893,24 +1927,16 @@ k_axpby( int64_t n, double a, const double *x, double b, double *y )
         y[i] = a * x[i] + ( b == 0. ? 0. : b * y[i] );
 }
 
-/* dst[i] = src[perm[i]] (gather) for nrhs columns */
+/* dst[i] = src[perm[i]] (gather) for nrhs columns; blockIdx.y walks the
+ * columns (no 64-bit division per element) */
 template <typename TD, typename TS>
-__global__ void
+__global__ __launch_bounds__( 256 ) void
 k_gather( int64_t n, int32_t nrhs, const int32_t *perm, const TS *src, int64_t lds, TD *dst, int64_t ldd )
 {
-    for ( int64_t i = blockIdx.x * 256 + threadIdx.x; i < n * nrhs; i += (int64_t)gridDim.x * 256 ) {
-        int64_t r = i / n, j = i % n;
-        dst[j + r * ldd] = (TD)src[perm[j] + r * lds];
-    }
-}
-/* dst[perm[i]] = src[i] (scatter) */
-template <typename TD, typename TS>
-__global__ void
-k_scatter( int64_t n, int32_t nrhs, const int32_t *perm, const TS *src, int64_t lds, TD *dst, int64_t ldd )
-{
-    for ( int64_t i = blockIdx.x * 256 + threadIdx.x; i < n * nrhs; i += (int64_t)gridDim.x * 256 ) {
-        int64_t r = i / n, j = i % n;
-        dst[perm[j] + r * ldd] = (TD)src[j + r * lds];
+    for ( int32_t r = blockIdx.y; r < nrhs; r += gridDim.y ) {
+        const TS *s = src + (int64_t)r * lds;
+        TD       *d = dst + (int64_t)r * ldd;
+        for ( int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256 ) d[j] = (TD)s[perm[j]];
     }
 }
 /* Sampled outer product on a CSC pattern (the adjoint of a sparse solve):
