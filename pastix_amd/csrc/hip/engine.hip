@@ -40,6 +40,13 @@ double pa_flops_syrk( double k, double n );
         }                                                                                    \
     } while ( 0 )
 
+/* Scatter-GEMM tile classes (rho rows x sigma columns of the output tile):
+ * a column group narrower than the tile would pad the MFMA work, so the
+ * sigma extent follows the group width */
+#define PA_NGC 5
+static const int pa_gc_tm[PA_NGC] = { 64, 64, 128, 128, 128 };
+static const int pa_gc_tn[PA_NGC] = { 64, 32, 128, 64, 32 };
+
 struct pa_level_t {
     int64_t cblk_beg, cblk_end;   /* into d_lev_cblks                       */
     int64_t fac_end;              /* [cblk_beg, fac_end): diag/TRSM/GEMM    */
@@ -48,8 +55,7 @@ struct pa_level_t {
     int64_t sm_end[3];            /* by row slots: [fac_end, sm_end[0]) of  */
                                   /* <= 64 rows, then <= 128, <= 192        */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
-    int64_t gemm_beg, gemm_end;   /* into d_gemm_tasks (64 x 64 tiles)      */
-    int64_t big_beg, big_end;     /* into d_gemm_tasks (128 x 128 tiles)    */
+    int64_t gc_beg[PA_NGC], gc_end[PA_NGC]; /* into d_gemm_tasks, per tile class */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
     int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
     int32_t ncmax;                /* widest cblk of the level               */
@@ -450,7 +456,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
     std::vector<int32_t>      lev_cblks;
     std::vector<int2>         trsm_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
-    std::vector<pa_gemm_task> gemm_tasks, big_tasks;
+    std::vector<pa_gemm_task> gtasks[PA_NGC];
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
     dev->levels.resize( nlev );
 
@@ -492,8 +498,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
         pa_level_t &L = dev->levels[l];
         L.cblk_beg  = lev_cblks.size();
         L.trsm_beg  = trsm_tasks.size();
-        L.gemm_beg  = gemm_tasks.size();
-        L.big_beg   = big_tasks.size();
+        for ( int c = 0; c < PA_NGC; c++ ) L.gc_beg[c] = gtasks[c].size();
         L.solve_beg = solve_tasks.size();
         L.bsolve_beg = bsolve_tasks.size();
         L.ncmax     = 0;
@@ -547,21 +552,33 @@ pa_dev_setup_structure( pastix_data_t *pd )
                 int32_t g0 = b_rowoff[b] - nc;
                 int32_t g1 = b_rowoff[e] - nc + ( b_lrow[e] - b_frow[e] + 1 );
                 int32_t flags = ( srccount[F] > 1 ) ? PA_TASK_ATOMIC : 0;
-                /* wide updates use 128 x 128 output tiles, the others 64 x 64 */
-                const int32_t TS = ( big_tiles && R - g0 >= 384 ) ? 128 : 64;
-                std::vector<pa_gemm_task> &dst = ( TS == 128 ) ? big_tasks : gemm_tasks;
+                /* tall updates use 128-row output tiles (FP32 only: in FP64
+                 * their accumulators leave one wave per SIMD), the others 64;
+                 * the tile width follows the remaining columns of the group */
+                const int32_t TM = ( big_tiles && R - g0 >= 384 ) ? 128 : 64;
                 for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
                     /* U side needs rows of strictly later blocks only */
                     if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
-                    for ( int32_t s = g0; s < g1; s += TS ) {
-                        int32_t slen = std::min( TS, g1 - s );
-                        for ( int32_t r = g0; r < R; r += TS ) {
+                    for ( int32_t s = g0; s < g1; ) {
+                        const int32_t rem = g1 - s;
+                        int32_t TN, cls;
+                        if ( TM == 128 ) {
+                            TN  = rem > 64 ? 128 : ( rem > 32 ? 64 : 32 );
+                            cls = TN == 128 ? 2 : ( TN == 64 ? 3 : 4 );
+                        }
+                        else {
+                            TN  = rem > 32 ? 64 : 32;
+                            cls = TN == 64 ? 0 : 1;
+                        }
+                        const int32_t slen = std::min( TN, rem );
+                        for ( int32_t r = g0; r < R; r += TM ) {
                             pa_gemm_task t;
                             t.k = k; t.F = F; t.s0 = s; t.r0 = r; t.slen = slen;
                             t.flags = flags | ( side ? PA_TASK_USIDE : 0 );
                             t.pad0 = t.pad1 = 0;
-                            dst.push_back( t );
+                            gtasks[cls].push_back( t );
                         }
+                        s += TN;
                     }
                 }
                 b = e + 1;
@@ -570,21 +587,23 @@ pa_dev_setup_structure( pastix_data_t *pd )
         for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
         L.cblk_end  = lev_cblks.size();
         L.trsm_end  = trsm_tasks.size();
-        L.gemm_end  = gemm_tasks.size();
-        L.big_end   = big_tasks.size();
+        for ( int c = 0; c < PA_NGC; c++ ) L.gc_end[c] = gtasks[c].size();
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
     }
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
-    /* both task lists in one array: the 128-tile tasks follow the 64-tile ones */
-    const int64_t nsmall = gemm_tasks.size();
-    for ( pa_level_t &L : dev->levels ) {
-        L.big_beg += nsmall;
-        L.big_end += nsmall;
+    /* the task lists of all classes in one array, class after class */
+    std::vector<pa_gemm_task> gemm_tasks;
+    for ( int c = 0; c < PA_NGC; c++ ) {
+        const int64_t base = gemm_tasks.size();
+        for ( pa_level_t &L : dev->levels ) {
+            L.gc_beg[c] += base;
+            L.gc_end[c] += base;
+        }
+        gemm_tasks.insert( gemm_tasks.end(), gtasks[c].begin(), gtasks[c].end() );
     }
-    gemm_tasks.insert( gemm_tasks.end(), big_tasks.begin(), big_tasks.end() );
     if ( ( rc = pa_upload( &dev->d_gemm_tasks, gemm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_solve_tasks, solve_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
@@ -802,16 +821,27 @@ pa_factorize_t( pa_device_s *dev )
             PA_HIP( hipGetLastError() );
         }
         PA_PROF_MARK( 3 );
-        int64_t ng = lev.gemm_end - lev.gemm_beg, nbig = lev.big_end - lev.big_beg;
         PA_PROF_MARK( 4 );
-        if ( nbig > 0 ) {
-            k_gemm_level<T, 128><<<(unsigned)nbig, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.big_beg, L,
-                                                                          lu ? U : nullptr );
-            PA_HIP( hipGetLastError() );
-        }
-        if ( ng > 0 ) {
-            k_gemm_level<T, 64><<<(unsigned)ng, 256, 0, dev->stream>>>( dev->S, dev->d_gemm_tasks + lev.gemm_beg, L,
-                                                                       lu ? U : nullptr );
+        /* the widest tiles first: they hold most of the level's work */
+        for ( int c = PA_NGC - 1; c >= 0; c-- ) {
+            const int64_t ng = lev.gc_end[c] - lev.gc_beg[c];
+            if ( ng <= 0 ) continue;
+            const pa_gemm_task *tk = dev->d_gemm_tasks + lev.gc_beg[c];
+            T *Up = lu ? U : nullptr;
+            const unsigned g = (unsigned)ng;
+            switch ( c ) {
+            case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
+            case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
+            default:
+                if constexpr ( sizeof( T ) == 4 ) {
+                    if ( c == 2 )      k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
+                    else if ( c == 3 ) k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
+                    else               k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
+                }
+                else {
+                    return PASTIX_ERR_INTERNAL; /* 128-row tiles are FP32 only */
+                }
+            }
             PA_HIP( hipGetLastError() );
         }
         PA_PROF_MARK( 5 );
@@ -843,15 +873,17 @@ pa_collect_profile( pa_device_s *dev )
             float t[3];
             for ( int c = 0; c < 3; c++ )
                 PA_HIP( hipEventElapsedTime( &t[c], dev->prof_ev[6 * li + 2 * c], dev->prof_ev[6 * li + 2 * c + 1] ) );
-            fprintf( stderr, "level %3zu cblks %6d ncmax %4d trsm %7d gemm64 %7d gemm128 %6d | diag %8.3f trsm %8.3f gemm %8.3f ms"
+            fprintf( stderr, "level %3zu cblks %6d ncmax %4d trsm %7d gemm %7d/%7d/%7d/%7d/%7d | diag %8.3f trsm %8.3f gemm %8.3f ms"
                              " | gemm %6.1f TF trsm %6.1f TF\n",
                      li, (int)( lev.cblk_end - lev.cblk_beg ), lev.ncmax, (int)( lev.trsm_end - lev.trsm_beg ),
-                     (int)( lev.gemm_end - lev.gemm_beg ), (int)( lev.big_end - lev.big_beg ), t[0], t[1], t[2],
+                     (int)( lev.gc_end[0] - lev.gc_beg[0] ), (int)( lev.gc_end[1] - lev.gc_beg[1] ),
+                     (int)( lev.gc_end[2] - lev.gc_beg[2] ), (int)( lev.gc_end[3] - lev.gc_beg[3] ),
+                     (int)( lev.gc_end[4] - lev.gc_beg[4] ), t[0], t[1], t[2],
                      lev.flops[2] / ( t[2] * 1e9 ), lev.flops[1] / ( t[1] * 1e9 ) );
         }
         dev->prof_launch[0] += 1;
         dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lu_launches( dev ) ) : 0;
-        dev->prof_launch[2] += ( lev.gemm_end > lev.gemm_beg ? 1 : 0 ) + ( lev.big_end > lev.big_beg ? 1 : 0 );
+        for ( int c = 0; c < PA_NGC; c++ ) dev->prof_launch[2] += ( lev.gc_end[c] > lev.gc_beg[c] ) ? 1 : 0;
     }
     return PASTIX_SUCCESS;
 }

@@ -994,102 +994,101 @@ template <typename T> struct pa_mfma;
 
 /*
  * Tile core shared by the level-batched kernel and the reference-signature
- * entry points.  TILE x TILE output (TILE = 64 or 128), 4 waves in a 2 x 2
- * arrangement, K in chunks of PA_GKC staged in LDS, the next chunk prefetched
+ * entry points.  TM x TN output (rho x sigma: 128 x 128, 128 x 64, 128 x 32,
+ * 64 x 64, 64 x 32), 4 waves in a 2 x 2 arrangement, K in chunks of PA_GKC staged in LDS, the next chunk prefetched
  * into registers while the current one feeds the MFMAs (a chunk of 32 keeps
  * the MFMAs busy for longer than the global load latency).  The loads are
  * unconditional from clamped in-range addresses (empty rows are zeroed when
  * the chunk is stored to LDS), so no per-load branch holds the prefetch back.
  * Row maps (in LDS)
  * must be filled by the caller:
- *   a_off/a_ld: source of the TILE rho rows inside Asrc (a_ld = 0: empty row)
- *   b_off/b_ld: source of the TILE sigma rows inside Bsrc
+ *   a_off/a_ld: source of the TM rho rows inside Asrc (a_ld = 0: empty row)
+ *   b_off/b_ld: source of the TN sigma rows inside Bsrc
  *   c_off/c_ld: target row base and leading dimension for each rho row
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T, int TILE>
+template <typename T, int TM, int TN>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[TILE + pa_gpad<T>::value], T ( *Bs )[TILE + pa_gpad<T>::value] )
+              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value] )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
-    constexpr int NB  = TILE / 32;        /* MFMA blocks per wave per dimension */
-    constexpr int KL  = 256 / TILE;       /* k lanes of the loader              */
-    constexpr int NQ  = PA_GKC / KL;      /* loads per operand per chunk        */
+    constexpr int NBM = TM / 32;          /* MFMA blocks per wave, rho          */
+    constexpr int NBN = TN / 32;          /* MFMA blocks per wave, sigma        */
+    constexpr int KLA = 256 / TM;         /* k lanes of the A loader            */
+    constexpr int KLB = 256 / TN;         /* k lanes of the B loader            */
+    constexpr int NQA = PA_GKC / KLA;     /* A loads per thread per chunk       */
+    constexpr int NQB = PA_GKC / KLB;     /* B loads per thread per chunk       */
     const int t    = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
     const int wm   = wave & 1;  /* rho half   */
     const int wn   = wave >> 1; /* sigma half */
-    acc_t acc[NB][NB];
+    acc_t acc[NBN][NBM];
 #pragma unroll
-    for ( int i = 0; i < NB; i++ )
+    for ( int i = 0; i < NBN; i++ )
 #pragma unroll
-        for ( int j = 0; j < NB; j++ )
+        for ( int j = 0; j < NBM; j++ )
 #pragma unroll
             for ( int r = 0; r < 4; r++ ) acc[i][j][r] = T( 0 );
 
-    const int lrow = t % TILE, lk = t / TILE;
-    const int32_t ald = a_ld[lrow], bld = b_ld[lrow];
+    const int arow = t % TM, ak = t / TM, brow = t % TN, bk = t / TN;
+    const int32_t ald = a_ld[arow], bld = b_ld[brow];
     /* an empty row reads row 0 of its operand (in range) and stores zeros */
-    const int64_t aof = ald > 0 ? a_off[lrow] : 0, bof = bld > 0 ? b_off[lrow] : 0;
+    const int64_t aof = ald > 0 ? a_off[arow] : 0, bof = bld > 0 ? b_off[brow] : 0;
     const int32_t alds = ald > 0 ? ald : 1, blds = bld > 0 ? bld : 1;
-    T pa[NQ], pb[NQ];
+    T pa[NQA], pb[NQB];
 
 #pragma unroll
-    for ( int q = 0; q < NQ; q++ ) {
-        const int32_t kk = min( lk + KL * q, K - 1 );
-        pa[q] = Asrc[aof + (int64_t)kk * alds];
-        pb[q] = Bsrc[bof + (int64_t)kk * blds];
-    }
+    for ( int q = 0; q < NQA; q++ ) pa[q] = Asrc[aof + (int64_t)min( ak + KLA * q, K - 1 ) * alds];
+#pragma unroll
+    for ( int q = 0; q < NQB; q++ ) pb[q] = Bsrc[bof + (int64_t)min( bk + KLB * q, K - 1 ) * blds];
     for ( int32_t k0 = 0; k0 < K; k0 += PA_GKC ) {
 #pragma unroll
-        for ( int q = 0; q < NQ; q++ ) {
-            const bool kin = k0 + lk + KL * q < K;
-            As[lk + KL * q][lrow] = ( ald > 0 && kin ) ? pa[q] : T( 0 );
-            Bs[lk + KL * q][lrow] = ( bld > 0 && kin ) ? pb[q] : T( 0 );
-        }
+        for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = ( ald > 0 && k0 + ak + KLA * q < K ) ? pa[q] : T( 0 );
+#pragma unroll
+        for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = ( bld > 0 && k0 + bk + KLB * q < K ) ? pb[q] : T( 0 );
         __syncthreads();
         if ( k0 + PA_GKC < K ) {
 #pragma unroll
-            for ( int q = 0; q < NQ; q++ ) {
-                const int32_t kk = min( k0 + PA_GKC + lk + KL * q, K - 1 );
-                pa[q] = Asrc[aof + (int64_t)kk * alds];
-                pb[q] = Bsrc[bof + (int64_t)kk * blds];
-            }
+            for ( int q = 0; q < NQA; q++ )
+                pa[q] = Asrc[aof + (int64_t)min( k0 + PA_GKC + ak + KLA * q, K - 1 ) * alds];
+#pragma unroll
+            for ( int q = 0; q < NQB; q++ )
+                pb[q] = Bsrc[bof + (int64_t)min( k0 + PA_GKC + bk + KLB * q, K - 1 ) * blds];
         }
 #pragma unroll
         for ( int ks = 0; ks < PA_GKC / 4; ks++ ) {
             int kk = ks * 4 + ( lane >> 4 );
-            T xa[NB], yb[NB];
+            T xa[NBN], yb[NBM];
 #pragma unroll
-            for ( int mi = 0; mi < NB; mi++ ) xa[mi] = Bs[kk][wn * ( TILE / 2 ) + mi * 16 + ( lane & 15 )];
+            for ( int mi = 0; mi < NBN; mi++ ) xa[mi] = Bs[kk][wn * ( TN / 2 ) + mi * 16 + ( lane & 15 )];
 #pragma unroll
-            for ( int mj = 0; mj < NB; mj++ ) yb[mj] = As[kk][wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 )];
+            for ( int mj = 0; mj < NBM; mj++ ) yb[mj] = As[kk][wm * ( TM / 2 ) + mj * 16 + ( lane & 15 )];
 #pragma unroll
-            for ( int mi = 0; mi < NB; mi++ )
+            for ( int mi = 0; mi < NBN; mi++ )
 #pragma unroll
-                for ( int mj = 0; mj < NB; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
+                for ( int mj = 0; mj < NBM; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
         }
         __syncthreads();
     }
 
 #pragma unroll
-    for ( int mj = 0; mj < NB; mj++ ) {
-        int rl = wm * ( TILE / 2 ) + mj * 16 + ( lane & 15 );
+    for ( int mj = 0; mj < NBM; mj++ ) {
+        int rl = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
         if ( a_ld[rl] == 0 ) continue;
         int64_t cbase = c_off[rl];
         int32_t cld   = c_ld[rl];
         int32_t ablk  = a_blk[rl];
 #pragma unroll
-        for ( int mi = 0; mi < NB; mi++ ) {
+        for ( int mi = 0; mi < NBN; mi++ ) {
 #pragma unroll
             for ( int r = 0; r < 4; r++ ) {
-                int sl = wn * ( TILE / 2 ) + mi * 16 + pa_mfma<T>::row( lane, r );
+                int sl = wn * ( TN / 2 ) + mi * 16 + pa_mfma<T>::row( lane, r );
                 if ( b_ld[sl] == 0 || ablk < b_blk[sl] + shift ) continue;
                 T *dst = C + cbase + (int64_t)c_col[sl] * cld;
                 T  v   = acc[mi][mj][r];
@@ -1100,16 +1099,16 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     }
 }
 
-/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task of TILE rows. */
-template <typename T, int TILE>
+/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task (TM x TN tile). */
+template <typename T, int TM, int TN>
 __global__ __launch_bounds__( 256 ) void
 k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
 {
     constexpr int PAD = pa_gpad<T>::value;
-    __shared__ T As[PA_GKC][TILE + PAD];
-    __shared__ T Bs[PA_GKC][TILE + PAD];
-    __shared__ int64_t a_off[TILE], b_off[TILE], c_off[TILE];
-    __shared__ int32_t a_ld[TILE], b_ld[TILE], c_ld[TILE], c_col[TILE], a_blk[TILE], b_blk[TILE], grow_s[TILE];
+    __shared__ T As[PA_GKC][TM + PAD];
+    __shared__ T Bs[PA_GKC][TN + PAD];
+    __shared__ int64_t a_off[TM], b_off[TN], c_off[TM];
+    __shared__ int32_t a_ld[TM], b_ld[TN], c_ld[TM], c_col[TN], a_blk[TM], b_blk[TN], grow_s[TM];
     /* first rows / last rows / row offsets of the target cblk's blocks */
     __shared__ int32_t f_frow[PA_GF_STAGE], f_lrow[PA_GF_STAGE], f_roff[PA_GF_STAGE];
 
@@ -1122,10 +1121,10 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
     const int32_t fb0 = S.c_fblok[F], nfb = S.c_fblok[F + 1] - fb0;
     const bool staged = nfb <= PA_GF_STAGE;
 
-    /* row descriptors from S.r_info (rho rows: u < TILE, sigma rows: u >= TILE) */
-    for ( int u = t; u < 2 * TILE; u += 256 ) {
-        const bool rho = u < TILE;
-        const int32_t tt = rho ? u : u - TILE;
+    /* row descriptors from S.r_info (rho rows: u < TM, sigma rows: u >= TM) */
+    for ( int u = t; u < TM + TN; u += 256 ) {
+        const bool rho = u < TM;
+        const int32_t tt = rho ? u : u - TM;
         const int32_t v  = ( rho ? tk.r0 : tk.s0 ) + tt;
         const bool ok = ( v < R ) && ( rho || tt < tk.slen );
         int4 ri = make_int4( 0, 0, 0, 0 );
@@ -1156,7 +1155,7 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
     /* target block of each rho row in F */
     const int32_t ncF = S.c_ncol[F];
     const int64_t coefF = S.c_coef[F];
-    for ( int tt = t; tt < TILE; tt += 256 ) {
+    for ( int tt = t; tt < TM; tt += 256 ) {
         const int32_t grow = grow_s[tt];
         if ( grow < 0 ) {
             a_ld[tt] = 0; c_off[tt] = 0; c_ld[tt] = 0;
@@ -1186,7 +1185,7 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T, TILE>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+    pa_gemm_tile<T, TM, TN>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
                            uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
 }
 
@@ -1234,7 +1233,7 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
         b_blk[s] = 0;
     }
     __syncthreads();
-    pa_gemm_tile<T, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
+    pa_gemm_tile<T, PA_GT, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
 }
 
 /* ------------------------------------------------------------------------- */

@@ -62,9 +62,12 @@ def test_getrf_fp64_matches_oracle(name, grid, bs):
         P.close()
 
 
-@pytest.mark.parametrize("ft", [pa.factotype.LLT, pa.factotype.LU], ids=["llt", "lu"])
-def test_fp32_factor_matches_oracle(ft):
-    grid = (10, 9, 8)
+@pytest.mark.parametrize("ft,grid", [(pa.factotype.LLT, (10, 9, 8)), (pa.factotype.LU, (10, 9, 8)),
+                                     (pa.factotype.LLT, (26, 25, 24)), (pa.factotype.LU, (19, 18, 17))],
+                         ids=["llt", "lu", "llt-128row-tiles", "lu-128row-tiles"])
+def test_fp32_factor_matches_oracle(ft, grid):
+    """The larger grids have updates of >= 384 rows, which take the 128-row
+    GEMM tiles (128 x 128, 128 x 64 and 128 x 32)."""
     A = problems.laplacian(*grid) if ft == pa.factotype.LLT else problems.convection_diffusion_27(*grid)
     P = Problem(A, grid, ft, mixed=True)
     try:
