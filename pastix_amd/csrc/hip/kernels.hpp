@@ -793,6 +793,7 @@ template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = 
  * where C(l, j) = L[j][l] (MODE 0) or D[l][j] (MODE 1, 2); the coefficients of
  * the block column are staged in LDS first.  LDS: SR*nc + 16*nc + 4*SR*16 T.
  */
+#define PA_TRSM_U 8
 template <typename T, int MODE>
 __device__ void
 pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xoff, const int32_t *xld, T *smem )
@@ -806,21 +807,43 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
     T *red = Ls + 16 * nc;         /* [KS][16][SR] partial sums  */
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 
-    for ( int32_t idx = t; idx < nc * SR; idx += 256 ) {
-        int32_t r = idx % SR, j = idx / SR;
-        Xs[j * SR + r] = ( r < rows ) ? panel[xoff[r] + (int64_t)j * xld[r]] : T( 0 );
+    /* global <-> LDS staging loops issue PA_TRSM_U independent loads before
+     * their stores, so the load latency is paid once per batch */
+    for ( int32_t i0 = t; i0 < nc * SR; i0 += 256 * PA_TRSM_U ) {
+        T v[PA_TRSM_U];
+#pragma unroll
+        for ( int u = 0; u < PA_TRSM_U; u++ ) {
+            const int32_t idx = i0 + 256 * u, r = idx % SR, j = idx / SR;
+            v[u] = ( idx < nc * SR && r < rows ) ? panel[xoff[r] + (int64_t)j * xld[r]] : T( 0 );
+        }
+#pragma unroll
+        for ( int u = 0; u < PA_TRSM_U; u++ ) {
+            const int32_t idx = i0 + 256 * u;
+            if ( idx < nc * SR ) Xs[idx] = v[u]; /* idx = j * SR + r */
+        }
     }
 
     for ( int32_t jb = 0; jb < nc; jb += 16 ) {
         const int32_t cw = nc - jb < 16 ? nc - jb : 16;
         const int32_t kend = jb + cw; /* coefficients needed: l < jb + cw */
         /* stage C(l, jb+jj) for l < kend */
-        for ( int32_t idx = t; idx < 16 * kend; idx += 256 ) {
-            int32_t jj, l;
-            if ( MODE == 0 ) { jj = idx % 16; l = idx / 16; }  /* L[jb+jj][l]: contiguous in jj */
-            else             { l = idx % kend; jj = idx / kend; } /* D[l][jb+jj]: contiguous in l  */
-            Ls[jj * nc + l] = ( jj < cw ) ? ( MODE == 0 ? D[( jb + jj ) + (int64_t)l * nc] : D[l + (int64_t)( jb + jj ) * nc] )
-                                          : T( 0 );
+        for ( int32_t i0 = t; i0 < 16 * kend; i0 += 256 * PA_TRSM_U ) {
+            T       v[PA_TRSM_U];
+            int32_t dst[PA_TRSM_U];
+#pragma unroll
+            for ( int u = 0; u < PA_TRSM_U; u++ ) {
+                const int32_t idx = i0 + 256 * u;
+                int32_t jj, l;
+                if ( MODE == 0 ) { jj = idx % 16; l = idx / 16; }  /* L[jb+jj][l]: contiguous in jj */
+                else             { l = idx % kend; jj = idx / kend; } /* D[l][jb+jj]: contiguous in l  */
+                const bool in = idx < 16 * kend;
+                dst[u] = in ? jj * nc + l : -1;
+                v[u]   = ( in && jj < cw ) ? ( MODE == 0 ? D[( jb + jj ) + (int64_t)l * nc] : D[l + (int64_t)( jb + jj ) * nc] )
+                                           : T( 0 );
+            }
+#pragma unroll
+            for ( int u = 0; u < PA_TRSM_U; u++ )
+                if ( dst[u] >= 0 ) Ls[dst[u]] = v[u];
         }
         __syncthreads();
         if ( jb > 0 ) {
@@ -874,6 +897,7 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
         if ( r < rows ) panel[xoff[r] + (int64_t)j * xld[r]] = Xs[j * SR + r];
     }
 }
+#undef PA_TRSM_U
 
 /* LDS bytes of pa_trsm_rows for nc columns */
 template <typename T>
