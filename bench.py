@@ -40,7 +40,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=True):
+def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=True, tune=True):
     ip, dp = pa.initParam()
     ip[pa.iparm.factorization] = factotype
     ip[pa.iparm.verbose] = pa.verbose.Not
@@ -60,7 +60,15 @@ def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=
     t0 = time.perf_counter()
     pa.task_numfact(h, spm)          # CSC permutation + upload + first factorization
     t_nf = time.perf_counter() - t0
-    return dict(h=h, ip=ip, dp=dp, spm=spm, order=o, t_analyze=t_an, t_first_numfact=t_nf, refine=refine)
+    tuning = None
+    if tune:
+        # the schedule autotuner: candidate tile classes / kernels timed on this matrix
+        t0 = time.perf_counter()
+        tuning = pa.tune(h)
+        tuning["seconds"] = time.perf_counter() - t0
+        log("tuned schedule", tuning)
+    return dict(h=h, ip=ip, dp=dp, spm=spm, order=o, t_analyze=t_an, t_first_numfact=t_nf, refine=refine,
+                tuning=tuning)
 
 
 def step(pa, lib, inst, d_b, d_x, stages=None):
@@ -212,7 +220,8 @@ def run_pair(args, torch, pa, lib, A, grid, factotype, d_b, d_x, x_true=None):
     b = d_b.cpu().numpy()
     for mode in (["fp32ir"] if args.no_fp64 else ["fp32ir", "fp64"]):
         log("mode", mode, ": analysis + first factorization")
-        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps, factotype=factotype)
+        inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps, factotype=factotype,
+                             tune=not args.no_tune)
         th = float(inst["dp"][pa.dparm.fact_thflops])
         t, iters, prof, stages, t_prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup)
         x = d_x.cpu().numpy()
@@ -220,6 +229,7 @@ def run_pair(args, torch, pa, lib, A, grid, factotype, d_b, d_x, x_true=None):
         ferr = np.abs(x - x_true).max() / np.abs(x_true).max() if x_true is not None else None
         results[mode] = dict(t=t, iters=iters, prof=prof, th=th, berr=berr, ferr=ferr, stages=stages, t_prof=t_prof,
                              t_analyze=inst["t_analyze"], fact_s=float(inst["dp"][pa.dparm.fact_time]),
+                             tuning=inst["tuning"],
                              x=x if mode == "fp64" else None)
         log("mode %s: %.1f ms/step, berr %.2e, ferr %s, iters %s, fact %.1f ms" %
             (mode, t * 1e3, berr, ferr, iters, results[mode]["fact_s"] * 1e3))
@@ -256,6 +266,7 @@ def pair_line(args, pa, results, metric_config, family):
         "kernel_ms_per_step": kernel_ms(r["prof"], steps),
         "stage_ms": r["stages"],
         "analysis_s": r["t_analyze"],
+        "tuning": r["tuning"],
         "roofline": roofline,
     }
     if "fp64" in results:
@@ -268,6 +279,7 @@ def pair_line(args, pa, results, metric_config, family):
         line["fp64_kernel_ms_per_step"] = kernel_ms(r64["prof"], steps)
         line["fp64_roofline"] = gemm_roofline(r64["prof"], steps, "fp64")
         line["fp64_stage_ms"] = r64["stages"]
+        line["fp64_tuning"] = r64["tuning"]
         # forward error of the FP32+IR solution against the FP64 factorization's
         x64 = r64["x"]
         line["ferr_vs_fp64_solution"] = float(np.abs(r["x32"] - x64).max() / np.abs(x64).max())
@@ -453,6 +465,7 @@ def main():
     ap.add_argument("--count", type=int, default=0, help="systems of the batch workload (default 256)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
     ap.add_argument("--eps", type=float, default=1e-13)
     args = ap.parse_args()
     if args.gpus != 1:

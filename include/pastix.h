@@ -116,6 +116,24 @@ int pastix_amd_sampled_outer( pastix_data_t *pastix_data, const void *d_colptr, 
                               pastix_int_t nrhs, const void *d_u, pastix_int_t ldu, const void *d_v,
                               pastix_int_t ldv, double alpha, void *d_out );
 
+/*
+ * Autotuner of the launch schedule: the GEMM tile classes (128-row tiles
+ * from how many update rows, tile width following the column-group width)
+ * and the fused narrow-cblk kernels.  Each of the PASTIX_AMD_TUNE_NCAND
+ * candidates that differs for this problem factors the current matrix
+ * (after pastix_task_numfact) and is timed on the device; the fastest is
+ * kept and its factors left in place.  times (may be NULL) receives each
+ * candidate's factorization seconds (0: not applicable), params[4] (may be
+ * NULL) the chosen {tm_min, narrow, small, 0}.  Returns the chosen
+ * candidate's index, or -PASTIX_ERR_*.  No reference counterpart (the
+ * reference leaves these choices to its StarPU/PaRSEC runtime).
+ */
+#define PASTIX_AMD_TUNE_NCAND 6
+int pastix_amd_tune( pastix_data_t *pastix_data, double *times, pastix_int_t *params );
+/* Select a schedule chosen earlier (params from pastix_amd_tune) for the
+ * next factorizations. */
+int pastix_amd_set_tuning( pastix_data_t *pastix_data, const pastix_int_t *params );
+
 /* Synchronise the library stream and return the elapsed seconds of the last
  * factorization measured with device events (0 if none). */
 double pastix_amd_last_fact_seconds( pastix_data_t *pastix_data );

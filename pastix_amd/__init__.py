@@ -11,10 +11,10 @@ from . import enum
 from .enum import iparm, dparm, factotype, refine, verbose, trans, mtxtype, ordering
 from .solver import (solver, spmatrix, order, initParam, init, finalize, task_analyze,
                      task_numfact, task_solve, task_refine, get_structure, get_coeftab,
-                     PastixError)
+                     tune, set_tuning, PastixError)
 from . import problems
 
 __all__ = ["enum", "iparm", "dparm", "factotype", "refine", "verbose", "trans", "mtxtype", "ordering",
            "solver", "spmatrix", "order", "initParam", "init", "finalize", "task_analyze",
            "task_numfact", "task_solve", "task_refine", "get_structure", "get_coeftab",
-           "PastixError", "problems"]
+           "tune", "set_tuning", "PastixError", "problems"]

@@ -105,6 +105,8 @@ PROTOTYPES = {
     "pastix_amd_refine_dev": (C.c_int, [_P, _I, _P, _I, _P, _I]),
     "pastix_amd_sampled_outer": (C.c_int, [_P, _P, _P, _I, _P, _I, _P, _I, C.c_double, _P]),
     "pastix_amd_values_dev": (C.c_int, [_P, _P]),
+    "pastix_amd_tune": (C.c_int, [_P, _P, _P]),
+    "pastix_amd_set_tuning": (C.c_int, [_P, _P]),
     "pastix_amd_spmv_dev": (C.c_int, [_P, C.c_int, _D, _P, _D, _P]),
     "pastix_amd_last_fact_seconds": (_D, [_P]),
     "pastix_amd_profile": (C.c_int, [_P, C.c_int, _P]),

@@ -73,6 +73,17 @@ struct pa_graph_entry {
 };
 #define PA_SOLVE_GRAPHS 6
 
+/* Tunable choices of the launch schedule (pastix_amd_tune): 128-row GEMM
+ * tiles for updates of at least tm_min rows (FP32), sigma width following
+ * the column-group width (narrow), the fused narrow-cblk kernels (small) */
+struct pa_tune_t {
+    int32_t tm_min;
+    int32_t narrow;
+    int32_t small;
+    int32_t reserved;
+};
+static const pa_tune_t pa_tune_default = { 384, 1, 1, 0 };
+
 struct pa_wblock {
     void  *p;
     size_t bytes;
@@ -127,6 +138,9 @@ struct pa_device_s {
     double *d_xtmp;    /* permuted rhs buffer */
     int64_t xtmp_len;
     std::vector<pa_wblock> wpool;   /* pa_dev_wget / pa_dev_wput */
+    /* host copy of the structure the schedule is built from */
+    std::vector<int32_t> h_c_ncol, h_c_stride, h_c_fblok, h_b_frow, h_b_lrow, h_b_fcblk, h_b_rowoff;
+    pa_tune_t tune = pa_tune_default;
     hipEvent_t ev0, ev1;
     double  last_fact_event_s;
     /* per kernel class profiling (diag, trsm, gemm) with HIP events */
@@ -334,6 +348,8 @@ pa_dev_close( pastix_data_t *pd )
 /* ------------------------------------------------------------------------- */
 /* structure + schedule                                                      */
 /* ------------------------------------------------------------------------- */
+static int pa_build_schedule( pastix_data_t *pd );
+
 extern "C" int
 pa_dev_setup_structure( pastix_data_t *pd )
 {
@@ -443,6 +459,38 @@ pa_dev_setup_structure( pastix_data_t *pd )
     S.r_info   = dev->d_r_info;
     S.r_blk    = dev->d_r_blk;
 
+    dev->h_c_ncol   = std::move( c_ncol );
+    dev->h_c_stride = std::move( c_stride );
+    dev->h_c_fblok  = std::move( c_fblok );
+    dev->h_b_frow   = std::move( b_frow );
+    dev->h_b_lrow   = std::move( b_lrow );
+    dev->h_b_fcblk  = std::move( b_fcblk );
+    dev->h_b_rowoff = std::move( b_rowoff );
+    return pa_build_schedule( pd );
+}
+
+/* The level schedule and the task lists of every kernel class, from the
+ * host copy of the structure and the tuning choices in dev->tune. */
+static int
+pa_build_schedule( pastix_data_t *pd )
+{
+    pa_device_s        *dev  = DEV( pd );
+    const SolverMatrix *solv = pd->solvmatr;
+    const pastix_int_t  cblknbr = solv->cblknbr;
+    const int32_t       colmax  = dev->colmax;
+    int rc;
+    const std::vector<int32_t> &c_ncol = dev->h_c_ncol, &c_stride = dev->h_c_stride, &c_fblok = dev->h_c_fblok;
+    const std::vector<int32_t> &b_frow = dev->h_b_frow, &b_lrow = dev->h_b_lrow, &b_fcblk = dev->h_b_fcblk;
+    const std::vector<int32_t> &b_rowoff = dev->h_b_rowoff;
+    pa_graphs_drop( dev );
+    {
+        void *ptrs[] = { dev->d_lev_cblks, dev->d_trsm_tasks, dev->d_gemm_tasks, dev->d_solve_tasks, dev->d_bsolve_tasks };
+        for ( void *p : ptrs ) hipFree( p );
+        dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
+        dev->d_bsolve_tasks = NULL;
+        dev->levels.clear();
+    }
+
     /* ---- level schedule ---- */
     std::vector<pastix_int_t> level( cblknbr + 1 );
     pastix_int_t nlev = cblknbr ? pa_solver_levels( solv, level.data() ) : 0;
@@ -464,7 +512,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
      * (k_small_llt); PASTIX_AMD_SMALL=0 sends them through the general
      * kernels (same results up to rounding) */
     const char *sm_env = getenv( "PASTIX_AMD_SMALL" );
-    const bool  use_small = !lu && !( sm_env && sm_env[0] == '0' );
+    const bool  use_small = !lu && dev->tune.small && !( sm_env && sm_env[0] == '0' );
     auto is_small = [&]( pastix_int_t k ) {
         return use_small && c_ncol[k] <= PA_SM_NC && c_stride[k] <= PA_SM_ROWS;
     };
@@ -555,7 +603,7 @@ pa_dev_setup_structure( pastix_data_t *pd )
                 /* tall updates use 128-row output tiles (FP32 only: in FP64
                  * their accumulators leave one wave per SIMD), the others 64;
                  * the tile width follows the remaining columns of the group */
-                const int32_t TM = ( big_tiles && R - g0 >= 384 ) ? 128 : 64;
+                const int32_t TM = ( big_tiles && R - g0 >= dev->tune.tm_min ) ? 128 : 64;
                 for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
                     /* U side needs rows of strictly later blocks only */
                     if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
@@ -563,11 +611,11 @@ pa_dev_setup_structure( pastix_data_t *pd )
                         const int32_t rem = g1 - s;
                         int32_t TN, cls;
                         if ( TM == 128 ) {
-                            TN  = rem > 64 ? 128 : ( rem > 32 ? 64 : 32 );
+                            TN  = ( rem > 64 || !dev->tune.narrow ) ? 128 : ( rem > 32 ? 64 : 32 );
                             cls = TN == 128 ? 2 : ( TN == 64 ? 3 : 4 );
                         }
                         else {
-                            TN  = rem > 32 ? 64 : 32;
+                            TN  = ( rem > 32 || !dev->tune.narrow ) ? 64 : 32;
                             cls = TN == 64 ? 0 : 1;
                         }
                         const int32_t slen = std::min( TN, rem );
@@ -616,6 +664,65 @@ pa_dev_setup_structure( pastix_data_t *pd )
         return PASTIX_ERR_BADPARAMETER;
     }
     return PASTIX_SUCCESS;
+}
+
+/* Candidate schedules of the autotuner (pastix_amd_tune), the default first */
+static const pa_tune_t pa_tune_cands[] = {
+    { 384, 1, 1, 0 }, { 256, 1, 1, 0 }, { 512, 1, 1, 0 }, { 384, 0, 1, 0 }, { 1 << 30, 1, 1, 0 }, { 384, 1, 0, 0 },
+};
+#define PA_NTUNE ( (int)( sizeof( pa_tune_cands ) / sizeof( pa_tune_cands[0] ) ) )
+
+extern "C" int
+pa_dev_tune_ncand( void )
+{
+    return PA_NTUNE;
+}
+
+/* 1 if candidate c gives a schedule of its own for this problem (the
+ * 128-row tile threshold matters in FP32 only, the narrow-cblk kernels in
+ * LL^t only) */
+extern "C" int
+pa_dev_tune_applicable( const pastix_data_t *pd, int c )
+{
+    if ( c < 0 || c >= PA_NTUNE ) return 0;
+    const pa_tune_t &t = pa_tune_cands[c], &d = pa_tune_default;
+    const int big = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
+    const int lu  = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    if ( !big && t.tm_min != d.tm_min ) return 0;
+    if ( lu && t.small != d.small ) return 0;
+    return 1;
+}
+
+/* set the schedule choices (candidate c, or params when c < 0) and rebuild
+ * the task lists; the factors have to be recomputed afterwards */
+extern "C" int
+pa_dev_set_tune( pastix_data_t *pd, int c, const int32_t *params )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( !dev ) return PASTIX_ERR_BADPARAMETER;
+    pa_tune_t t;
+    if ( c >= 0 ) {
+        if ( c >= PA_NTUNE ) return PASTIX_ERR_BADPARAMETER;
+        t = pa_tune_cands[c];
+    }
+    else {
+        if ( !params || params[0] < 64 || params[1] < 0 || params[1] > 1 || params[2] < 0 || params[2] > 1 )
+            return PASTIX_ERR_BADPARAMETER;
+        t.tm_min = params[0]; t.narrow = params[1]; t.small = params[2]; t.reserved = 0;
+    }
+    dev->tune = t;
+    if ( dev->h_c_ncol.empty() ) return PASTIX_SUCCESS; /* applied when the structure is uploaded */
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    return pa_build_schedule( pd );
+}
+
+extern "C" void
+pa_dev_get_tune( const pastix_data_t *pd, int32_t *params )
+{
+    const pa_device_s *dev = DEV( pd );
+    const pa_tune_t   &t   = dev ? dev->tune : pa_tune_default;
+    params[0] = t.tm_min; params[1] = t.narrow; params[2] = t.small; params[3] = 0;
 }
 
 extern "C" pastix_int_t

@@ -22,6 +22,14 @@ int    pa_dev_open( pastix_data_t *pastix_data );
 void   pa_dev_close( pastix_data_t *pastix_data );
 /* upload the SolverMatrix, permutation and the level schedule */
 int    pa_dev_setup_structure( pastix_data_t *pastix_data );
+/* schedule autotuning: number of candidates, whether candidate c differs
+ * from the default for this problem, select candidate c (or explicit
+ * params {tm_min, narrow, small} when c < 0) and rebuild the task lists,
+ * read the current choice (params[4]) */
+int    pa_dev_tune_ncand( void );
+int    pa_dev_tune_applicable( const pastix_data_t *pastix_data, int c );
+int    pa_dev_set_tune( pastix_data_t *pastix_data, int c, const int32_t *params );
+void   pa_dev_get_tune( const pastix_data_t *pastix_data, int32_t *params );
 /* upload the permuted CSC (pattern + FP64 values) */
 int    pa_dev_upload_bcsc( pastix_data_t *pastix_data );
 /* upload the value map of the permuted CSC (see pa_valmap_t) */

@@ -511,6 +511,60 @@ pastix_task_numfact( pastix_data_t *pd, spmatrix_t *spm )
     return PASTIX_SUCCESS;
 }
 
+/* Autotuner of the launch schedule (no reference counterpart: it replaces
+ * the StarPU/PaRSEC runtime's choices for one GPU).  Every applicable
+ * candidate factors the current matrix twice (the second, graph-replayed
+ * run is timed with device events); the fastest is kept and its factors are
+ * left in place. */
+int
+pastix_amd_tune( pastix_data_t *pd, double *times, pastix_int_t *params )
+{
+    int c, best = -1, ncand = pa_dev_tune_ncand(), rc, rep;
+    double tbest = 0.;
+    if ( pd == NULL || pd->dev == NULL || !( pd->steps & STEP_CSC2BCSC ) ) {
+        pa_error( "pastix_amd_tune: pastix_task_numfact() has to be called first" );
+        return -PASTIX_ERR_BADPARAMETER;
+    }
+    for ( c = 0; c < ncand; c++ ) {
+        double t = 0.;
+        if ( times ) times[c] = 0.;
+        if ( !pa_dev_tune_applicable( pd, c ) ) continue;
+        if ( ( rc = pa_dev_set_tune( pd, c, NULL ) ) ) return -rc;
+        for ( rep = 0; rep < 2; rep++ ) {
+            if ( ( rc = pastix_subtask_bcsc2ctab( pd ) ) ) return -rc;
+            if ( ( rc = pastix_subtask_sopalin( pd ) ) ) return -rc;
+            t = pa_dev_last_fact_event_seconds( pd );
+        }
+        if ( times ) times[c] = t;
+        if ( best < 0 || t < tbest ) {
+            best  = c;
+            tbest = t;
+        }
+    }
+    if ( ( rc = pa_dev_set_tune( pd, best, NULL ) ) ) return -rc;
+    if ( ( rc = pastix_subtask_bcsc2ctab( pd ) ) ) return -rc;
+    if ( ( rc = pastix_subtask_sopalin( pd ) ) ) return -rc;
+    if ( params ) {
+        int32_t p[4];
+        pa_dev_get_tune( pd, p );
+        for ( c = 0; c < 4; c++ ) params[c] = p[c];
+    }
+    return best;
+}
+
+int
+pastix_amd_set_tuning( pastix_data_t *pd, const pastix_int_t *params )
+{
+    int32_t p[4];
+    int     c, rc;
+    if ( pd == NULL || pd->dev == NULL || params == NULL ) return PASTIX_ERR_BADPARAMETER;
+    for ( c = 0; c < 4; c++ ) p[c] = (int32_t)params[c];
+    if ( ( rc = pa_dev_set_tune( pd, -1, p ) ) ) return rc;
+    /* the factors of the previous schedule stay valid; a new factorization
+     * uses the new task lists */
+    return PASTIX_SUCCESS;
+}
+
 int
 pastixRhsInit( pastix_rhs_t *rhs )
 {

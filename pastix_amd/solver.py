@@ -191,6 +191,32 @@ def get_structure(handle):
     return cblk, blok, perm, invp, dims
 
 
+TUNE_NCAND = 6   # PASTIX_AMD_TUNE_NCAND
+TUNE_CANDIDATES = [  # {tm_min, narrow, small}: the order of the library's candidate list
+    (384, 1, 1), (256, 1, 1), (512, 1, 1), (384, 0, 1), (1 << 30, 1, 1), (384, 1, 0)]
+
+
+def tune(handle):
+    """Autotune the launch schedule on the matrix of the last numfact
+    (``pastix_amd_tune``): returns {"best": index, "params": [tm_min,
+    narrow, small, 0], "times": seconds per candidate (0: not applicable)}.
+    The handle keeps the chosen schedule and its factors."""
+    lib = _lib.load()
+    times = np.zeros(TUNE_NCAND)
+    params = np.zeros(4, dtype=np.int64)
+    best = lib.pastix_amd_tune(handle, _ptr(times), _ptr(params))
+    if best < 0:
+        raise PastixError("pastix_amd_tune failed with error code %d" % -best)
+    return {"best": int(best), "params": [int(v) for v in params], "times": [float(t) for t in times]}
+
+
+def set_tuning(handle, params):
+    """Select a schedule (params as returned by ``tune``) for the next
+    factorizations."""
+    p = np.ascontiguousarray(list(params) + [0] * (4 - len(params)), dtype=np.int64)
+    _check(_lib.load().pastix_amd_set_tuning(handle, _ptr(p)), "pastix_amd_set_tuning")
+
+
 def get_coeftab(handle, side=0):
     lib = _lib.load()
     dims = np.zeros(8, dtype=np.int64)

@@ -125,3 +125,12 @@ def test_numfact_without_gpu_fails_loudly():
             pa.task_numfact(h, spm)
     finally:
         pa.finalize(h)
+
+
+def test_tuning_candidate_count_matches_the_header():
+    import re
+    from pastix_amd.solver import TUNE_CANDIDATES, TUNE_NCAND
+    txt = open(os.path.join(ROOT, "include", "pastix.h")).read()
+    m = re.search(r"#define\s+PASTIX_AMD_TUNE_NCAND\s+(\d+)", txt)
+    assert m and int(m.group(1)) == TUNE_NCAND == len(TUNE_CANDIDATES)
+    assert TUNE_CANDIDATES[0] == (384, 1, 1)   # the default schedule is candidate 0
