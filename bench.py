@@ -61,7 +61,12 @@ def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=
     pa.task_numfact(h, spm)          # CSC permutation + upload + first factorization
     t_nf = time.perf_counter() - t0
     tuning = None
-    if tune:
+    if isinstance(tune, (list, tuple)):
+        # a schedule chosen by an earlier tuning run
+        pa.set_tuning(h, tune)
+        pa.task_numfact(h, spm)
+        tuning = {"params": list(tune) + [0] * (4 - len(tune)), "given": True}
+    elif tune:
         # the schedule autotuner: candidate tile classes / kernels timed on this matrix
         t0 = time.perf_counter()
         tuning = pa.tune(h)
@@ -141,16 +146,19 @@ def timed(torch, pa, lib, inst, d_b, d_x, steps, warmup):
     return t, iters, prof, stages, t_prof
 
 
-def measured_traffic():
+def measured_traffic(schedule):
     """HBM bytes per launch of the level GEMM class, measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE passes of this workload and scaled by the same
     counters on a plain copy of known size; the figures and the commands
-    that produced them are committed beside the file."""
+    that produced them are committed beside the file.  Used only when the
+    measurement was taken with the same launch schedule."""
     path = os.path.join(ROOT, "profiles", "gemm_level_fp32_laplacian100_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if schedule is None or list(d.get("schedule", []))[:3] != list(schedule)[:3]:
+        return None
     return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, ROOT)}
 
 
@@ -200,7 +208,7 @@ def gemm_roofline(prof, steps, precision):
     achieved = gemm_flops / (gemm_ms * 1e-3)
     tname = "float" if precision == "fp32" else "double"
     return {"bound": "mfma",
-            "kernel": "k_gemm_level<%s,*> (level scatter-GEMM, all tile sizes)" % tname,
+            "kernel": "k_gemm_level<%s,TM,TN> (level scatter-GEMM, all tile classes of the schedule)" % tname,
             "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK[precision], "traffic": None,
             "launches_per_step": gemm_launches, "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
@@ -220,8 +228,9 @@ def run_pair(args, torch, pa, lib, A, grid, factotype, d_b, d_x, x_true=None):
     b = d_b.cpu().numpy()
     for mode in (["fp32ir"] if args.no_fp64 else ["fp32ir", "fp64"]):
         log("mode", mode, ": analysis + first factorization")
+        sched = args.schedule if mode == "fp32ir" else args.schedule64
         inst = make_instance(pa, A, grid, mixed=(mode == "fp32ir"), eps_refine=args.eps, factotype=factotype,
-                             tune=not args.no_tune)
+                             tune=sched if sched else not args.no_tune)
         th = float(inst["dp"][pa.dparm.fact_thflops])
         t, iters, prof, stages, t_prof = timed(torch, pa, lib, inst, d_b, d_x, args.steps, args.warmup)
         x = d_x.cpu().numpy()
@@ -246,9 +255,9 @@ def pair_line(args, pa, results, metric_config, family):
     steps = args.steps
     roofline = gemm_roofline(r["prof"], steps, "fp32")
     roofline["events_timed_ms_per_step"] = r["t_prof"] * 1e3
+    schedule = r["tuning"]["params"] if r["tuning"] else None
     if family == "laplacian":
-        roofline["kernel"] = "k_gemm_level<float,128> + k_gemm_level<float,64> (level scatter-GEMM)"
-        traffic = measured_traffic()
+        traffic = measured_traffic(schedule)
         if traffic:
             roofline["traffic"] = traffic["bytes_per_launch"]
             roofline["traffic_source"] = traffic["source"]
@@ -466,6 +475,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
+    sched = lambda v: [int(x) for x in v.split(",")]
+    ap.add_argument("--schedule", type=sched, default=None,
+                    help="tm_min,narrow,small of the FP32 run instead of autotuning (as printed by a tuning run)")
+    ap.add_argument("--schedule64", type=sched, default=None, help="the same for the FP64 run")
     ap.add_argument("--eps", type=float, default=1e-13)
     args = ap.parse_args()
     if args.gpus != 1:
