@@ -19,6 +19,13 @@
  *                                            and sopalin/pastix_task_solve.c:543-649
  * It is compiled twice: FP64 (oracle_d*) and, with ORACLE_SINGLE, FP32 (oracle_s*).
  *
+ * For the CPU-baseline leg only, the caller may route the dense GEMM/TRSM
+ * through a BLAS library (set_blas: Fortran-convention pointers, as the
+ * reference's per-task BLAS calls) and run the level-parallel driver
+ * potrf_par on several threads (OpenMP; updates into a shared facing cblk
+ * serialised by a per-cblk lock, as the reference's pastix_cblk_lock).  The
+ * tests use neither.
+ *
  * The structure is passed as the arrays exported by pastix_amd_get_structure:
  *   cblk[5*k]: fcolnum, lcolnum, first block, stride, coefficient offset
  *   blok[5*b]: frownum, lrownum, lcblknm, fcblknm, coefind
@@ -27,6 +34,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #if defined( ORACLE_SINGLE )
 #define REAL float
@@ -53,6 +61,22 @@ typedef int64_t oint;
 
 #define MAXSIZEOFBLOCKS 64
 
+/* ---- optional BLAS (CPU-baseline leg only) ------------------------------ */
+typedef void ( *blas_gemm_t )( const char *, const char *, const int *, const int *, const int *, const REAL *,
+                               const REAL *, const int *, const REAL *, const int *, const REAL *, REAL *,
+                               const int * );
+typedef void ( *blas_trsm_t )( const char *, const char *, const char *, const char *, const int *, const int *,
+                               const REAL *, const REAL *, const int *, REAL *, const int * );
+static blas_gemm_t blas_gemm;
+static blas_trsm_t blas_trsm;
+
+void
+PFX( set_blas )( void *gemm, void *trsm )
+{
+    blas_gemm = (blas_gemm_t)gemm;
+    blas_trsm = (blas_trsm_t)trsm;
+}
+
 /* ---- dense helpers (column major) --------------------------------------- */
 
 /* C(m x n) -= A(m x k) * op(B),  op(B) = B^T with B (n x k) */
@@ -60,6 +84,12 @@ static void
 gemm_nt_minus( oint m, oint n, oint k, const REAL *A, oint lda, const REAL *B, oint ldb, REAL *C, oint ldc )
 {
     oint i, j, l;
+    if ( blas_gemm && m > 0 && n > 0 && k > 0 ) {
+        const int  im = (int)m, in = (int)n, ik = (int)k, ia = (int)lda, ib = (int)ldb, ic = (int)ldc;
+        const REAL mone = (REAL)-1, one = (REAL)1;
+        blas_gemm( "N", "T", &im, &in, &ik, &mone, A, &ia, B, &ib, &one, C, &ic );
+        return;
+    }
     for ( j = 0; j < n; j++ ) {
         REAL *c = C + j * ldc;
         for ( l = 0; l < k; l++ ) {
@@ -76,6 +106,12 @@ static void
 gemm_nn_minus( oint m, oint n, oint k, const REAL *A, oint lda, const REAL *B, oint ldb, REAL *C, oint ldc )
 {
     oint i, j, l;
+    if ( blas_gemm && m > 0 && n > 0 && k > 0 ) {
+        const int  im = (int)m, in = (int)n, ik = (int)k, ia = (int)lda, ib = (int)ldb, ic = (int)ldc;
+        const REAL mone = (REAL)-1, one = (REAL)1;
+        blas_gemm( "N", "N", &im, &in, &ik, &mone, A, &ia, B, &ib, &one, C, &ic );
+        return;
+    }
     for ( j = 0; j < n; j++ ) {
         REAL *c = C + j * ldc;
         for ( l = 0; l < k; l++ ) {
@@ -92,6 +128,12 @@ static void
 trsm_rlt( oint m, oint n, const REAL *L, oint lda, REAL *X, oint ldx )
 {
     oint i, j, l;
+    if ( blas_trsm && m > 0 && n > 0 ) {
+        const int  im = (int)m, in = (int)n, ia = (int)lda, ix = (int)ldx;
+        const REAL one = (REAL)1;
+        blas_trsm( "R", "L", "T", "N", &im, &in, &one, L, &ia, X, &ix );
+        return;
+    }
     for ( j = 0; j < n; j++ ) {
         REAL *xj = X + j * ldx;
         for ( l = 0; l < j; l++ ) {
@@ -111,6 +153,12 @@ static void
 trsm_run( oint m, oint n, const REAL *U, oint lda, int unit, REAL *X, oint ldx )
 {
     oint i, j, l;
+    if ( blas_trsm && m > 0 && n > 0 ) {
+        const int  im = (int)m, in = (int)n, ia = (int)lda, ix = (int)ldx;
+        const REAL one = (REAL)1;
+        blas_trsm( "R", "U", "N", unit ? "U" : "N", &im, &in, &one, U, &ia, X, &ix );
+        return;
+    }
     for ( j = 0; j < n; j++ ) {
         REAL *xj = X + j * ldx;
         for ( l = 0; l < j; l++ ) {
@@ -130,6 +178,12 @@ static void
 trsm_llnu( oint m, oint n, const REAL *L, oint lda, REAL *X, oint ldx )
 {
     oint i, j, l;
+    if ( blas_trsm && m > 0 && n > 0 ) {
+        const int  im = (int)m, in = (int)n, ia = (int)lda, ix = (int)ldx;
+        const REAL one = (REAL)1;
+        blas_trsm( "L", "L", "N", "U", &im, &in, &one, L, &ia, X, &ix );
+        return;
+    }
     for ( j = 0; j < n; j++ ) {
         REAL *x = X + j * ldx;
         for ( l = 0; l < m; l++ ) {
@@ -180,10 +234,15 @@ potrfsp( oint n, REAL *A, oint lda, oint *npiv, oint *nneg, double crit )
             oint  ms = n - ( k * MAXSIZEOFBLOCKS + bs ), i, j, l;
             trsm_rlt( ms, bs, tmp, lda, tmp1, lda );
             /* syrk lower: tmp2 -= tmp1 tmp1^T */
-            for ( j = 0; j < ms; j++ ) {
-                for ( l = 0; l < bs; l++ ) {
-                    REAL b = tmp1[j + l * lda];
-                    for ( i = j; i < ms; i++ ) tmp2[i + j * lda] -= tmp1[i + l * lda] * b;
+            if ( blas_gemm ) {
+                gemm_nt_minus( ms, ms, bs, tmp1, lda, tmp1, lda, tmp2, lda );
+            }
+            else {
+                for ( j = 0; j < ms; j++ ) {
+                    for ( l = 0; l < bs; l++ ) {
+                        REAL b = tmp1[j + l * lda];
+                        for ( i = j; i < ms; i++ ) tmp2[i + j * lda] -= tmp1[i + l * lda] * b;
+                    }
                 }
             }
         }
@@ -302,6 +361,76 @@ PFX( potrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, double 
     return 0;
 }
 
+/* Level-parallel LL^t on nthreads threads (CPU-baseline leg): the cblks of
+ * one elimination-tree level are factored concurrently (diagonal + panel
+ * TRSM), then all (cblk, block) scatter-GEMMs of the level run concurrently,
+ * each under the lock of its facing cblk.  Same kernels and results (up to
+ * the order of the updates) as the sequential driver. */
+int
+PFX( potrf_par )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, double crit, oint *npiv, oint *nneg,
+                  int nthreads )
+{
+    oint       k, l, nlev = 0, *lev, *order, *lptr, *pairs, npairs = 0;
+    omp_lock_t *locks;
+    oint        piv = 0, neg = 0;
+    lev   = calloc( cblknbr + 1, sizeof( oint ) );
+    order = malloc( ( cblknbr + 1 ) * sizeof( oint ) );
+    locks = malloc( ( cblknbr + 1 ) * sizeof( omp_lock_t ) );
+    /* height of each cblk in the tree (children have smaller indices) */
+    for ( k = 0; k < cblknbr; k++ ) {
+        omp_init_lock( locks + k );
+        if ( C_FBLK( k ) + 1 < C_FBLK( k + 1 ) ) {
+            oint f = B_FCBK( C_FBLK( k ) + 1 );
+            if ( lev[f] < lev[k] + 1 ) lev[f] = lev[k] + 1;
+        }
+        if ( lev[k] + 1 > nlev ) nlev = lev[k] + 1;
+    }
+    lptr = calloc( nlev + 2, sizeof( oint ) );
+    for ( k = 0; k < cblknbr; k++ ) lptr[lev[k] + 1]++;
+    for ( l = 0; l < nlev; l++ ) lptr[l + 1] += lptr[l];
+    {
+        oint *pos = malloc( ( nlev + 1 ) * sizeof( oint ) );
+        memcpy( pos, lptr, ( nlev + 1 ) * sizeof( oint ) );
+        for ( k = 0; k < cblknbr; k++ ) order[pos[lev[k]]++] = k;
+        free( pos );
+    }
+    for ( k = 0; k < cblknbr; k++ ) npairs += C_FBLK( k + 1 ) - C_FBLK( k ) - 1;
+    pairs = malloc( ( npairs + 1 ) * 2 * sizeof( oint ) );
+    for ( l = 0; l < nlev; l++ ) {
+        oint b0 = lptr[l], b1 = lptr[l + 1], np = 0, i;
+#pragma omp parallel for num_threads( nthreads ) schedule( dynamic, 1 ) reduction( + : piv, neg )
+        for ( i = b0; i < b1; i++ ) {
+            oint kk = order[i], b, nc = C_NCOL( kk ), p = 0, q = 0;
+            REAL *D = L + C_COEF( kk );
+            potrfsp( nc, D, nc, &p, &q, crit );
+            piv += p;
+            neg += q;
+            for ( b = C_FBLK( kk ) + 1; b < C_FBLK( kk + 1 ); b++ )
+                trsm_rlt( B_NROW( b ), nc, D, nc, L + C_COEF( kk ) + B_COEF( b ), B_NROW( b ) );
+        }
+        for ( i = b0; i < b1; i++ ) {
+            oint kk = order[i], b;
+            for ( b = C_FBLK( kk ) + 1; b < C_FBLK( kk + 1 ); b++ ) {
+                pairs[2 * np]     = kk;
+                pairs[2 * np + 1] = b;
+                np++;
+            }
+        }
+#pragma omp parallel for num_threads( nthreads ) schedule( dynamic, 1 )
+        for ( i = 0; i < np; i++ ) {
+            oint kk = pairs[2 * i], b = pairs[2 * i + 1], f = B_FCBK( b );
+            omp_set_lock( locks + f );
+            gemmsp_2d2d( cblk, blok, kk, b, 0, L, L, L );
+            omp_unset_lock( locks + f );
+        }
+    }
+    for ( k = 0; k < cblknbr; k++ ) omp_destroy_lock( locks + k );
+    *npiv = piv;
+    *nneg = neg;
+    free( lev ); free( order ); free( locks ); free( lptr ); free( pairs );
+    return 0;
+}
+
 /* sequential LU with static pivoting (reference sopalin/sequential_zgetrf.c,
  * cpucblk_zgetrfsp1d, kernels/core_zgetrfsp.c:217-395) */
 int
@@ -328,6 +457,76 @@ PFX( getrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *U
             if ( b + 1 < lb ) gemmsp_2d2d( cblk, blok, k, b, 1, U, L, U );
         }
     }
+    return 0;
+}
+
+/* Level-parallel LU (CPU-baseline leg), the scheme of potrf_par with the
+ * LU kernels of the sequential driver. */
+int
+PFX( getrf_par )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *U, double crit, oint *npiv,
+                  int nthreads )
+{
+    oint        k, l, nlev = 0, *lev, *order, *lptr, *pairs, npairs = 0, piv = 0;
+    omp_lock_t *locks;
+    lev   = calloc( cblknbr + 1, sizeof( oint ) );
+    order = malloc( ( cblknbr + 1 ) * sizeof( oint ) );
+    locks = malloc( ( cblknbr + 1 ) * sizeof( omp_lock_t ) );
+    for ( k = 0; k < cblknbr; k++ ) {
+        omp_init_lock( locks + k );
+        if ( C_FBLK( k ) + 1 < C_FBLK( k + 1 ) ) {
+            oint f = B_FCBK( C_FBLK( k ) + 1 );
+            if ( lev[f] < lev[k] + 1 ) lev[f] = lev[k] + 1;
+        }
+        if ( lev[k] + 1 > nlev ) nlev = lev[k] + 1;
+    }
+    lptr = calloc( nlev + 2, sizeof( oint ) );
+    for ( k = 0; k < cblknbr; k++ ) lptr[lev[k] + 1]++;
+    for ( l = 0; l < nlev; l++ ) lptr[l + 1] += lptr[l];
+    {
+        oint *pos = malloc( ( nlev + 1 ) * sizeof( oint ) );
+        memcpy( pos, lptr, ( nlev + 1 ) * sizeof( oint ) );
+        for ( k = 0; k < cblknbr; k++ ) order[pos[lev[k]]++] = k;
+        free( pos );
+    }
+    for ( k = 0; k < cblknbr; k++ ) npairs += C_FBLK( k + 1 ) - C_FBLK( k ) - 1;
+    pairs = malloc( ( npairs + 1 ) * 2 * sizeof( oint ) );
+    for ( l = 0; l < nlev; l++ ) {
+        oint b0 = lptr[l], b1 = lptr[l + 1], np = 0, i;
+#pragma omp parallel for num_threads( nthreads ) schedule( dynamic, 1 ) reduction( + : piv )
+        for ( i = b0; i < b1; i++ ) {
+            oint kk = order[i], b, ii, jj, nc = C_NCOL( kk ), p = 0;
+            REAL *D = L + C_COEF( kk ), *Du = U + C_COEF( kk );
+            for ( jj = 0; jj < nc; jj++ )
+                for ( ii = 0; ii < nc; ii++ ) D[ii + jj * nc] += Du[jj + ii * nc];
+            getrfsp( nc, D, nc, &p, crit );
+            piv += p;
+            for ( jj = 0; jj < nc; jj++ )
+                for ( ii = 0; ii < nc; ii++ ) Du[jj + ii * nc] = D[ii + jj * nc];
+            for ( b = C_FBLK( kk ) + 1; b < C_FBLK( kk + 1 ); b++ ) {
+                trsm_run( B_NROW( b ), nc, D, nc, 0, L + C_COEF( kk ) + B_COEF( b ), B_NROW( b ) );
+                trsm_run( B_NROW( b ), nc, Du, nc, 1, U + C_COEF( kk ) + B_COEF( b ), B_NROW( b ) );
+            }
+        }
+        for ( i = b0; i < b1; i++ ) {
+            oint kk = order[i], b;
+            for ( b = C_FBLK( kk ) + 1; b < C_FBLK( kk + 1 ); b++ ) {
+                pairs[2 * np]     = kk;
+                pairs[2 * np + 1] = b;
+                np++;
+            }
+        }
+#pragma omp parallel for num_threads( nthreads ) schedule( dynamic, 1 )
+        for ( i = 0; i < np; i++ ) {
+            oint kk = pairs[2 * i], b = pairs[2 * i + 1], f = B_FCBK( b );
+            omp_set_lock( locks + f );
+            gemmsp_2d2d( cblk, blok, kk, b, 0, L, U, L );
+            if ( b + 1 < C_FBLK( kk + 1 ) ) gemmsp_2d2d( cblk, blok, kk, b, 1, U, L, U );
+            omp_unset_lock( locks + f );
+        }
+    }
+    for ( k = 0; k < cblknbr; k++ ) omp_destroy_lock( locks + k );
+    *npiv = piv;
+    free( lev ); free( order ); free( locks ); free( lptr ); free( pairs );
     return 0;
 }
 

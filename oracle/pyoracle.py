@@ -34,7 +34,27 @@ def load():
             getattr(_lib, "oracle_%spotrf_diag" % pfx).argtypes = [I, P, D, P, P]
             getattr(_lib, "oracle_%sgetrf_diag" % pfx).argtypes = [I, P, P, D, P]
             getattr(_lib, "oracle_%strsm_cblk" % pfx).argtypes = [P, P, I, C.c_int, P, P]
+            getattr(_lib, "oracle_%sset_blas" % pfx).argtypes = [P, P]
+            getattr(_lib, "oracle_%spotrf_par" % pfx).argtypes = [I, P, P, P, D, P, P, C.c_int]
+            getattr(_lib, "oracle_%sgetrf_par" % pfx).argtypes = [I, P, P, P, P, D, P, C.c_int]
     return _lib
+
+
+def blas_pointers(dtype=np.float64):
+    """(gemm, trsm) Fortran-convention function pointers of the BLAS that
+    SciPy exports to compiled code (scipy.linalg.cython_blas capsules)."""
+    from scipy.linalg import cython_blas
+    api = C.pythonapi
+    api.PyCapsule_GetName.restype = C.c_char_p
+    api.PyCapsule_GetName.argtypes = [C.py_object]
+    api.PyCapsule_GetPointer.restype = C.c_void_p
+    api.PyCapsule_GetPointer.argtypes = [C.py_object, C.c_char_p]
+    pfx = "d" if dtype == np.float64 else "s"
+    out = []
+    for name in (pfx + "gemm", pfx + "trsm"):
+        cap = cython_blas.__pyx_capi__[name]
+        out.append(api.PyCapsule_GetPointer(cap, api.PyCapsule_GetName(cap)))
+    return tuple(out)
 
 
 def _p(a):
@@ -92,6 +112,46 @@ def factorize(S, A, perm, factotype, crit, dtype=np.float64):
     else:
         getattr(lib, "oracle_%spotrf" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), crit, _p(npiv), _p(nneg))
     return L, U, int(npiv[0])
+
+
+def factorize_parallel(S, A, perm, crit, nthreads, use_blas=True, dtype=np.float64, factotype=0):
+    """Fill + level-parallel LL^t (factotype 0) or LU (2) on nthreads
+    threads, the dense kernels on SciPy's BLAS (one BLAS thread per task)
+    when use_blas; returns (L, U or None, npiv, seconds of the factorization
+    alone)."""
+    import time
+    lib = load()
+    pfx = "d" if dtype == np.float64 else "s"
+    colptr, rowtab, Lv, Uv = permuted_csc(A, perm)
+    L = np.zeros(S.coefnbr, dtype=dtype)
+    U = np.zeros(S.coefnbr, dtype=dtype) if factotype == 2 else None
+    getattr(lib, "oracle_%sfill" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(colptr), _p(rowtab),
+                                        _p(Lv), _p(Uv), _p(L), _p(U))
+    npiv = np.zeros(1, dtype=np.int64)
+    nneg = np.zeros(1, dtype=np.int64)
+    setb = getattr(lib, "oracle_%sset_blas" % pfx)
+    try:
+        if use_blas:
+            g, t = blas_pointers(dtype)
+            setb(C.c_void_p(g), C.c_void_p(t))
+        try:
+            from threadpoolctl import threadpool_limits
+            limit = threadpool_limits(limits=1, user_api="blas")
+        except Exception:
+            limit = None
+        t0 = time.perf_counter()
+        if factotype == 2:
+            getattr(lib, "oracle_%sgetrf_par" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), _p(U), crit,
+                                                     _p(npiv), int(nthreads))
+        else:
+            getattr(lib, "oracle_%spotrf_par" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), crit, _p(npiv),
+                                                     _p(nneg), int(nthreads))
+        sec = time.perf_counter() - t0
+        if limit is not None:
+            limit.restore_original_limits()
+    finally:
+        setb(None, None)
+    return L, U, int(npiv[0]), sec
 
 
 def solve(S, factotype, L, U, b_perm, trans=0):

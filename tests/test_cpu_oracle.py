@@ -118,6 +118,28 @@ def test_llt_factorization_is_the_unique_cholesky():
     assert npiv == 0
 
 
+@pytest.mark.parametrize("lu,use_blas", [(False, False), (False, True), (True, False), (True, True)],
+                         ids=["llt", "llt-blas", "lu", "lu-blas"])
+def test_parallel_oracle_matches_the_sequential_one(lu, use_blas):
+    """The level-parallel drivers of the CPU baseline (4 threads, optional
+    BLAS kernels) compute the factors of the sequential oracle, which the
+    tests above pin to LAPACK / Doolittle."""
+    grid = (12, 11, 10)
+    A = problems.convection_diffusion_27(*grid) if lu else problems.laplacian(*grid)
+    S, perm, invp, th = _analyse(A, grid, lu=lu)
+    crit = np.finfo(np.float64).eps / 2 * sp.linalg.norm(A)
+    L, U, npiv = orc.factorize(S, A, perm, 2 if lu else 0, crit)
+    Lp, Up, npivp, sec = orc.factorize_parallel(S, A, perm, crit, 4, use_blas=use_blas, factotype=2 if lu else 0)
+    n = A.shape[0]
+    Ld, Lpd = _expand(S, L, n), _expand(S, Lp, n)
+    if lu:
+        assert _rel(Lpd, Ld) < 1e-12
+        assert _rel(_expand(S, Up, n), _expand(S, U, n)) < 1e-12
+    else:
+        assert _rel(np.tril(Lpd), np.tril(Ld)) < 1e-12
+    assert npivp == npiv and sec > 0
+
+
 def test_lu_factorization_is_the_unique_doolittle():
     grid = (5, 4, 3)
     A = problems.convection_diffusion_27(*grid)
