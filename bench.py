@@ -57,16 +57,15 @@ def make_instance(pa, A, grid, mixed, eps_refine, factotype, order=None, refine=
     t0 = time.perf_counter()
     pa.task_analyze(h, spm, o)
     t_an = time.perf_counter() - t0
+    tuning = None
+    if isinstance(tune, (list, tuple)):
+        # a schedule chosen by an earlier tuning run, in place before any factorization
+        pa.set_tuning(h, tune)
+        tuning = {"params": list(tune) + [0] * (4 - len(tune)), "given": True}
     t0 = time.perf_counter()
     pa.task_numfact(h, spm)          # CSC permutation + upload + first factorization
     t_nf = time.perf_counter() - t0
-    tuning = None
-    if isinstance(tune, (list, tuple)):
-        # a schedule chosen by an earlier tuning run
-        pa.set_tuning(h, tune)
-        pa.task_numfact(h, spm)
-        tuning = {"params": list(tune) + [0] * (4 - len(tune)), "given": True}
-    elif tune:
+    if tune is True:
         # the schedule autotuner: candidate tile classes / kernels timed on this matrix
         t0 = time.perf_counter()
         tuning = pa.tune(h)
