@@ -156,7 +156,7 @@ def measured_traffic(schedule):
         return None
     with open(path) as f:
         d = json.load(f)
-    if schedule is None or list(d.get("schedule", []))[:3] != list(schedule)[:3]:
+    if schedule is None or list(d.get("schedule", [])) != (list(schedule) + [0] * 4)[:4]:
         return None
     return {"bytes_per_launch": d["hbm_bytes_per_launch"], "source": os.path.relpath(path, ROOT)}
 
@@ -483,7 +483,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
     sched = lambda v: [int(x) for x in v.split(",")]
     ap.add_argument("--schedule", type=sched, default=None,
-                    help="tm_min,narrow,small of the FP32 run instead of autotuning (as printed by a tuning run)")
+                    help="tm_min,narrow,small[,tall64] of the FP32 run instead of autotuning (as printed by a tuning run)")
     ap.add_argument("--schedule64", type=sched, default=None, help="the same for the FP64 run")
     ap.add_argument("--eps", type=float, default=1e-13)
     args = ap.parse_args()

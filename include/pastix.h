@@ -118,17 +118,18 @@ int pastix_amd_sampled_outer( pastix_data_t *pastix_data, const void *d_colptr, 
 
 /*
  * Autotuner of the launch schedule: the GEMM tile classes (128-row tiles
- * from how many update rows, tile width following the column-group width)
+ * from how many update rows, in FP64 too (tall64), tile width following the
+ * column-group width)
  * and the fused narrow-cblk kernels.  Each of the PASTIX_AMD_TUNE_NCAND
  * candidates that differs for this problem factors the current matrix
  * (after pastix_task_numfact) and is timed on the device; the fastest is
  * kept and its factors left in place.  times (may be NULL) receives each
  * candidate's factorization seconds (0: not applicable), params[4] (may be
- * NULL) the chosen {tm_min, narrow, small, 0}.  Returns the chosen
+ * NULL) the chosen {tm_min, narrow, small, tall64}.  Returns the chosen
  * candidate's index, or -PASTIX_ERR_*.  No reference counterpart (the
  * reference leaves these choices to its StarPU/PaRSEC runtime).
  */
-#define PASTIX_AMD_TUNE_NCAND 6
+#define PASTIX_AMD_TUNE_NCAND 8
 int pastix_amd_tune( pastix_data_t *pastix_data, double *times, pastix_int_t *params );
 /* Select a schedule chosen earlier (params from pastix_amd_tune) for the
  * next factorizations. */

@@ -74,13 +74,15 @@ struct pa_graph_entry {
 #define PA_SOLVE_GRAPHS 6
 
 /* Tunable choices of the launch schedule (pastix_amd_tune): 128-row GEMM
- * tiles for updates of at least tm_min rows (FP32), sigma width following
- * the column-group width (narrow), the fused narrow-cblk kernels (small) */
+ * tiles for updates of at least tm_min rows (FP32, and FP64 when tall64:
+ * there at most 64 columns wide, their accumulators would not fit a
+ * 128 x 128 tile), sigma width following the column-group width (narrow),
+ * the fused narrow-cblk kernels (small) */
 struct pa_tune_t {
     int32_t tm_min;
     int32_t narrow;
     int32_t small;
-    int32_t reserved;
+    int32_t tall64;
 };
 static const pa_tune_t pa_tune_default = { 384, 1, 1, 0 };
 
@@ -603,14 +605,19 @@ pa_build_schedule( pastix_data_t *pd )
                 /* tall updates use 128-row output tiles (FP32 only: in FP64
                  * their accumulators leave one wave per SIMD), the others 64;
                  * the tile width follows the remaining columns of the group */
-                const int32_t TM = ( big_tiles && R - g0 >= dev->tune.tm_min ) ? 128 : 64;
+                const bool    tall = big_tiles || dev->tune.tall64;
+                const int32_t TM   = ( tall && R - g0 >= dev->tune.tm_min ) ? 128 : 64;
                 for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
                     /* U side needs rows of strictly later blocks only */
                     if ( side == 1 && e + 1 >= lb && b == e ) continue; /* single last block */
                     for ( int32_t s = g0; s < g1; ) {
                         const int32_t rem = g1 - s;
                         int32_t TN, cls;
-                        if ( TM == 128 ) {
+                        if ( TM == 128 && !big_tiles ) {
+                            TN  = ( rem > 32 || !dev->tune.narrow ) ? 64 : 32;
+                            cls = TN == 64 ? 3 : 4;
+                        }
+                        else if ( TM == 128 ) {
                             TN  = ( rem > 64 || !dev->tune.narrow ) ? 128 : ( rem > 32 ? 64 : 32 );
                             cls = TN == 128 ? 2 : ( TN == 64 ? 3 : 4 );
                         }
@@ -669,6 +676,7 @@ pa_build_schedule( pastix_data_t *pd )
 /* Candidate schedules of the autotuner (pastix_amd_tune), the default first */
 static const pa_tune_t pa_tune_cands[] = {
     { 384, 1, 1, 0 }, { 256, 1, 1, 0 }, { 512, 1, 1, 0 }, { 384, 0, 1, 0 }, { 1 << 30, 1, 1, 0 }, { 384, 1, 0, 0 },
+    { 384, 1, 1, 1 }, { 256, 1, 1, 1 },
 };
 #define PA_NTUNE ( (int)( sizeof( pa_tune_cands ) / sizeof( pa_tune_cands[0] ) ) )
 
@@ -688,7 +696,8 @@ pa_dev_tune_applicable( const pastix_data_t *pd, int c )
     const pa_tune_t &t = pa_tune_cands[c], &d = pa_tune_default;
     const int big = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     const int lu  = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
-    if ( !big && t.tm_min != d.tm_min ) return 0;
+    if ( !big && !t.tall64 && t.tm_min != d.tm_min ) return 0; /* no 128-row tiles in FP64 without tall64 */
+    if ( big && t.tall64 ) return 0;                            /* FP32 has its 128-row tiles anyway */
     if ( lu && t.small != d.small ) return 0;
     return 1;
 }
@@ -708,7 +717,8 @@ pa_dev_set_tune( pastix_data_t *pd, int c, const int32_t *params )
     else {
         if ( !params || params[0] < 64 || params[1] < 0 || params[1] > 1 || params[2] < 0 || params[2] > 1 )
             return PASTIX_ERR_BADPARAMETER;
-        t.tm_min = params[0]; t.narrow = params[1]; t.small = params[2]; t.reserved = 0;
+        if ( params[3] < 0 || params[3] > 1 ) return PASTIX_ERR_BADPARAMETER;
+        t.tm_min = params[0]; t.narrow = params[1]; t.small = params[2]; t.tall64 = params[3];
     }
     dev->tune = t;
     if ( dev->h_c_ncol.empty() ) return PASTIX_SUCCESS; /* applied when the structure is uploaded */
@@ -722,7 +732,7 @@ pa_dev_get_tune( const pastix_data_t *pd, int32_t *params )
 {
     const pa_device_s *dev = DEV( pd );
     const pa_tune_t   &t   = dev ? dev->tune : pa_tune_default;
-    params[0] = t.tm_min; params[1] = t.narrow; params[2] = t.small; params[3] = 0;
+    params[0] = t.tm_min; params[1] = t.narrow; params[2] = t.small; params[3] = t.tall64;
 }
 
 extern "C" pastix_int_t
@@ -939,14 +949,14 @@ pa_factorize_t( pa_device_s *dev )
             switch ( c ) {
             case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
             case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
+            case 3: k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
+            case 4: k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
             default:
                 if constexpr ( sizeof( T ) == 4 ) {
-                    if ( c == 2 )      k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
-                    else if ( c == 3 ) k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
-                    else               k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
+                    k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
                 }
                 else {
-                    return PASTIX_ERR_INTERNAL; /* 128-row tiles are FP32 only */
+                    return PASTIX_ERR_INTERNAL; /* 128 x 128 tiles are FP32 only */
                 }
             }
             PA_HIP( hipGetLastError() );

@@ -780,8 +780,10 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
 /* staged in LDS and solved with a column-blocked algorithm.                 */
 /* ------------------------------------------------------------------------- */
 /* rows per workgroup (the slab lives in LDS) */
-template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = 32; };
+#ifndef PA_TRSM_TR
 #define PA_TRSM_TR 32
+#endif
+template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = PA_TRSM_TR; };
 
 /*
  * Solve `rows` (<= SR) rows whose offsets/leading dimensions inside `panel`
@@ -1011,7 +1013,9 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 /* scatter store is coalesced.                                               */
 /* ------------------------------------------------------------------------- */
 #define PA_GT  64
-#define PA_GKC 32
+#ifndef PA_GKC
+#define PA_GKC 16
+#endif
 
 template <typename T> struct pa_mfma;
 

@@ -191,15 +191,17 @@ def get_structure(handle):
     return cblk, blok, perm, invp, dims
 
 
-TUNE_NCAND = 6   # PASTIX_AMD_TUNE_NCAND
-TUNE_CANDIDATES = [  # {tm_min, narrow, small}: the order of the library's candidate list
-    (384, 1, 1), (256, 1, 1), (512, 1, 1), (384, 0, 1), (1 << 30, 1, 1), (384, 1, 0)]
+TUNE_NCAND = 8   # PASTIX_AMD_TUNE_NCAND
+TUNE_CANDIDATES = [  # {tm_min, narrow, small, tall64}: the order of the library's candidate list
+    (384, 1, 1, 0), (256, 1, 1, 0), (512, 1, 1, 0), (384, 0, 1, 0), (1 << 30, 1, 1, 0), (384, 1, 0, 0),
+    (384, 1, 1, 1), (256, 1, 1, 1)]
 
 
 def tune(handle):
     """Autotune the launch schedule on the matrix of the last numfact
     (``pastix_amd_tune``): returns {"best": index, "params": [tm_min,
-    narrow, small, 0], "times": seconds per candidate (0: not applicable)}.
+    narrow, small, tall64], "times": seconds per candidate (0: not
+    applicable)}.
     The handle keeps the chosen schedule and its factors."""
     lib = _lib.load()
     times = np.zeros(TUNE_NCAND)

@@ -53,7 +53,7 @@ def test_every_candidate_schedule_gives_the_same_factors(ft, mixed):
         t = np.array(res["times"])
         assert 0 <= res["best"] < TUNE_NCAND
         assert t[res["best"]] > 0 and t[res["best"]] == t[t > 0].min()
-        assert tuple(res["params"][:3]) == TUNE_CANDIDATES[res["best"]]
+        assert tuple(res["params"]) == TUNE_CANDIDATES[res["best"]]
         # the handle holds the chosen schedule's factorization: solve with it
         n = A.shape[0]
         xt = problems.uniform(n, seed=3)
