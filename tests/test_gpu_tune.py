@@ -19,6 +19,8 @@ def _setup(A, grid, ft, mixed):
     ip[pa.iparm.verbose] = pa.verbose.Not
     if mixed:
         ip[pa.iparm.mixed] = 1
+    # ||r||_2/||b||_2 <= 1e-13 keeps the inf-norm backward error below 1e-12
+    dp[pa.dparm.epsilon_refinement] = 1e-13
     h = pa.init(ip, dp)
     mt = pa.mtxtype.General if ft == pa.factotype.LU else pa.mtxtype.Symmetric
     spm = pa.spmatrix(A, mtxtype=mt)
