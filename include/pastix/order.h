@@ -45,6 +45,14 @@ pastix_order_t *pastixOrderGet( const pastix_data_t *pastix_data );
  * numbered i + nx*j + nx*ny*k (reference order/order_grids.c:307-384). */
 int  pastixOrderGrid( pastix_order_t **myorder, pastix_int_t nx, pastix_int_t ny, pastix_int_t nz );
 
+/* Ordering files in the reference's text format (reference
+ * include/pastix/order.h:110-114, order/order_io.c): the file is named by the
+ * PASTIX_FILE_ORDER environment variable, else "ordername" (load) /
+ * "ordergen" (save) in the current directory.  pastix_subtask_order loads
+ * with IPARM_IO_STRATEGY & PastixIOLoad and saves with & PastixIOSave. */
+int  pastixOrderLoad( const pastix_data_t *pastix_data, pastix_order_t *ordeptr );
+int  pastixOrderSave( pastix_data_t *pastix_data, const pastix_order_t *ordeptr );
+
 #ifdef __cplusplus
 }
 #endif

@@ -241,6 +241,18 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
 
     o = calloc( 1, sizeof( pastix_order_t ) );
     pd->ordemesh = o;
+    if ( pd->iparm[IPARM_IO_STRATEGY] & PastixIOLoad ) {
+        /* the ordering of a file (reference order/pastix_subtask_order.c:62) */
+        if ( ( rc = pastixOrderLoad( pd, o ) ) ) return rc;
+        pastixOrderBase( o, 0 );
+        if ( o->vertnbr != n ) {
+            pa_error( "pastix_subtask_order: the ordering file has %ld unknowns, the matrix %ld", (long)o->vertnbr,
+                      (long)n );
+            return PASTIX_ERR_BADPARAMETER;
+        }
+        if ( o->treetab == NULL ) pa_order_tree_from_rangtab( o, n, pd->graph->colptr, pd->graph->rows );
+        goto ordered;
+    }
     pastixOrderAlloc( o, n, 0 );
     if ( myorder != NULL ) {
         pastixOrderBase( myorder, 0 );
@@ -274,11 +286,16 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
     else {
         pa_order_from_etree( o, n, pd->graph->colptr, pd->graph->rows );
     }
+ordered:
     if ( pastixOrderCheck( o ) != PASTIX_SUCCESS ) {
         pa_error( "pastix_subtask_order: inconsistent ordering" );
         return PASTIX_ERR_BADPARAMETER;
     }
     pa_order_apply_level_order( o, pd->iparm[IPARM_TASKS2D_LEVEL], pd->iparm[IPARM_TASKS2D_WIDTH] );
+    if ( pd->iparm[IPARM_IO_STRATEGY] & PastixIOSave ) {
+        /* saved after the level ordering (reference pastix_subtask_order.c:485) */
+        if ( ( rc = pastixOrderSave( pd, o ) ) ) return rc;
+    }
     o->sndenbr = o->cblknbr;
     o->sndetab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
     memcpy( o->sndetab, o->rangtab, ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );

@@ -69,6 +69,8 @@ void pa_symbol_exit( symbol_matrix_t *symb );
 /* Grid / personal orderings (order.c) */
 int  pa_order_apply_level_order( pastix_order_t *order, pastix_int_t level_tasks2d,
                                  pastix_int_t width_tasks2d );
+int  pa_order_tree_from_rangtab( pastix_order_t *order, pastix_int_t n, const pastix_int_t *colptr,
+                                 const pastix_int_t *rows );
 int  pa_order_from_etree( pastix_order_t *order, pastix_int_t n,
                           const pastix_int_t *colptr, const pastix_int_t *rows );
 

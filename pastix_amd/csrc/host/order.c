@@ -488,3 +488,142 @@ pa_order_from_etree( pastix_order_t *o, pastix_int_t n, const pastix_int_t *colp
     free( ancestor );
     return PASTIX_SUCCESS;
 }
+
+/* -------------------------------------------------------------------------- */
+/* Ordering files (the reference's text format, order/order_io.c:30-255):      */
+/*   version (0 or 1), cblknbr, vertnbr, rangtab[cblknbr+1], permtab[vertnbr], */
+/*   and with version 1 treetab[cblknbr]; integers separated by white space.   */
+/* The file is named by PASTIX_FILE_ORDER, else "ordername" (load) or         */
+/* "ordergen" (save) in the current directory.                                 */
+/* -------------------------------------------------------------------------- */
+
+static int
+pa_read_int( FILE *f, pastix_int_t *v )
+{
+    long long x;
+    if ( fscanf( f, "%lld", &x ) != 1 ) return 0;
+    *v = (pastix_int_t)x;
+    return 1;
+}
+
+/* Supernode tree of a partition given without one: the father of cblk k is
+ * the cblk holding the elimination-tree parent of k's last column. */
+int
+pa_order_tree_from_rangtab( pastix_order_t *o, pastix_int_t n, const pastix_int_t *colptr, const pastix_int_t *rows )
+{
+    pastix_int_t *parent, *ancestor, *col2cblk, i, k, b = o->baseval;
+    parent   = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
+    ancestor = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
+    col2cblk = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
+    for ( k = 0; k < n; k++ ) {
+        pastix_int_t ok = o->peritab[k] - b, e;
+        parent[k]   = -1;
+        ancestor[k] = -1;
+        for ( e = colptr[ok]; e < colptr[ok + 1]; e++ ) {
+            pastix_int_t r = o->permtab[rows[e]] - b;
+            while ( r != -1 && r < k ) {
+                pastix_int_t next = ancestor[r];
+                ancestor[r] = k;
+                if ( next == -1 ) parent[r] = k;
+                r = next;
+            }
+        }
+    }
+    for ( k = 0; k < o->cblknbr; k++ )
+        for ( i = o->rangtab[k] - b; i < o->rangtab[k + 1] - b; i++ ) col2cblk[i] = k;
+    free( o->treetab );
+    o->treetab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
+    for ( k = 0; k < o->cblknbr; k++ ) {
+        pastix_int_t last = o->rangtab[k + 1] - b - 1, p = parent[last];
+        o->treetab[k] = ( p < 0 ) ? -1 : col2cblk[p] + b;
+    }
+    free( parent );
+    free( ancestor );
+    free( col2cblk );
+    return PASTIX_SUCCESS;
+}
+
+int
+pastixOrderLoad( const pastix_data_t *pastix_data, pastix_order_t *ordeptr )
+{
+    const char  *name = getenv( "PASTIX_FILE_ORDER" );
+    FILE        *f;
+    pastix_int_t vers, cblknbr, vertnbr, i;
+    int          ok;
+    (void)pastix_data;
+    if ( ordeptr == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( name == NULL ) name = "ordername";
+    f = fopen( name, "r" );
+    if ( f == NULL ) {
+        pa_error( "pastixOrderLoad: cannot open %s", name );
+        return PASTIX_ERR_FILE;
+    }
+    if ( pa_read_int( f, &vers ) + pa_read_int( f, &cblknbr ) + pa_read_int( f, &vertnbr ) != 3 ||
+         ( vers != 0 && vers != 1 ) || cblknbr > vertnbr || cblknbr < 0 || vertnbr < 0 ) {
+        pa_error( "pastixOrderLoad: bad header in %s", name );
+        fclose( f );
+        return PASTIX_ERR_FILE;
+    }
+    pastixOrderAlloc( ordeptr, vertnbr, cblknbr );
+    ok = 1;
+    for ( i = 0; ok && i <= cblknbr; i++ ) ok = pa_read_int( f, &ordeptr->rangtab[i] );
+    for ( i = 0; ok && i < vertnbr; i++ ) ok = pa_read_int( f, &ordeptr->permtab[i] );
+    if ( vers == 1 ) {
+        for ( i = 0; ok && i < cblknbr; i++ ) ok = pa_read_int( f, &ordeptr->treetab[i] );
+    }
+    else {
+        free( ordeptr->treetab );
+        ordeptr->treetab = NULL;
+    }
+    fclose( f );
+    if ( !ok ) {
+        pa_error( "pastixOrderLoad: truncated file %s", name );
+        pastixOrderExit( ordeptr );
+        return PASTIX_ERR_FILE;
+    }
+    ordeptr->baseval = ordeptr->rangtab[0];
+    for ( i = 0; i < vertnbr; i++ ) {
+        pastix_int_t p = ordeptr->permtab[i] - ordeptr->baseval;
+        if ( p < 0 || p >= vertnbr ) {
+            pa_error( "pastixOrderLoad: permutation out of range in %s", name );
+            pastixOrderExit( ordeptr );
+            return PASTIX_ERR_FILE;
+        }
+        ordeptr->peritab[p] = i + ordeptr->baseval;
+    }
+    return PASTIX_SUCCESS;
+}
+
+int
+pastixOrderSave( pastix_data_t *pastix_data, const pastix_order_t *ordeptr )
+{
+    const char  *name = getenv( "PASTIX_FILE_ORDER" );
+    FILE        *f;
+    pastix_int_t i;
+    int          ok = 1;
+    (void)pastix_data;
+    if ( ordeptr == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( ordeptr->permtab == NULL || ordeptr->rangtab == NULL || ordeptr->treetab == NULL ) {
+        pa_error( "pastixOrderSave: the ordering needs permtab, rangtab and treetab" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( name == NULL ) name = "ordergen";
+    f = fopen( name, "w" );
+    if ( f == NULL ) {
+        pa_error( "pastixOrderSave: cannot open %s", name );
+        return PASTIX_ERR_FILE;
+    }
+    ok = fprintf( f, "1\n%ld\t%ld\n", (long)ordeptr->cblknbr, (long)ordeptr->vertnbr ) > 0;
+    for ( i = 0; ok && i <= ordeptr->cblknbr; i++ )
+        ok = fprintf( f, "%ld%c", (long)ordeptr->rangtab[i], ( ( i & 7 ) == 7 || i == ordeptr->cblknbr ) ? '\n' : '\t' ) > 0;
+    for ( i = 0; ok && i < ordeptr->vertnbr; i++ )
+        ok = fprintf( f, "%ld%c", (long)ordeptr->permtab[i], ( ( i & 7 ) == 7 || i + 1 == ordeptr->vertnbr ) ? '\n' : '\t' ) > 0;
+    for ( i = 0; ok && i < ordeptr->cblknbr; i++ )
+        ok = fprintf( f, "%ld%c", (long)ordeptr->treetab[i], ( ( i & 7 ) == 7 || i + 1 == ordeptr->cblknbr ) ? '\n' : '\t' ) > 0;
+    if ( fclose( f ) != 0 ) ok = 0;
+    if ( !ok ) {
+        pa_error( "pastixOrderSave: write error on %s", name );
+        return PASTIX_ERR_FILE;
+    }
+    return PASTIX_SUCCESS;
+}

@@ -116,6 +116,17 @@ class ordering:
     ParMetis = 4
 
 
+class io:
+    """IPARM_IO_STRATEGY flags (reference include/pastix/api.h)."""
+    No = 0
+    Load = 1
+    Save = 2
+    LoadGraph = 4
+    SaveGraph = 8
+    LoadCSC = 16
+    SaveCSC = 32
+
+
 class refine:
     GMRES = 0
     CG = 1

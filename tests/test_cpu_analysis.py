@@ -4,6 +4,8 @@ factorization and the SolverMatrix layout (blend/solver_matrix_gen*.c).
 Pinned by invariants that a wrong index or a dropped block would break:
 the separator property of every dissection node, containment of the exact
 Cholesky fill pattern, and the 2D tile offsets."""
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -164,5 +166,65 @@ def test_lu_analysis_uses_the_symmetrised_pattern():
     pat.data[:] = 1.
     S = sp.csc_matrix(pat + sp.eye(C.shape[0]) * 30.)
     s2 = _analyse(S, grid, factotype=pa.factotype.LU)
+    for a, b in zip(s1[:4], s2[:4]):
+        assert np.array_equal(a, b)
+
+
+def _structure(A, grid, io, path, lu=False):
+    old = os.environ.get("PASTIX_FILE_ORDER")
+    os.environ["PASTIX_FILE_ORDER"] = path
+    try:
+        ip, dp = pa.initParam()
+        ip[pa.iparm.factorization] = pa.factotype.LU if lu else pa.factotype.LLT
+        ip[pa.iparm.verbose] = pa.verbose.Not
+        ip[pa.iparm.io_strategy] = io
+        h = pa.init(ip, dp)
+        try:
+            spm = pa.spmatrix(A, mtxtype=pa.mtxtype.General if lu else pa.mtxtype.Symmetric)
+            pa.task_analyze(h, spm, pa.order.grid(*grid) if grid else None)
+            return pa.get_structure(h), float(dp[pa.dparm.fact_thflops])
+        finally:
+            pa.finalize(h)
+    finally:
+        if old is None:
+            del os.environ["PASTIX_FILE_ORDER"]
+        else:
+            os.environ["PASTIX_FILE_ORDER"] = old
+
+
+def test_ordering_file_round_trip(tmp_path):
+    """An ordering saved with IPARM_IO_STRATEGY = Save (the reference's text
+    format, order/order_io.c) and loaded back with Load gives the same
+    supernodal structure, without the grid ordering being supplied."""
+    grid = (9, 8, 7)
+    A = problems.laplacian(*grid)
+    path = str(tmp_path / "ordergen")
+    (s1, th1) = _structure(A, grid, pa.io.Save, path)
+    txt = open(path).read().split()
+    assert txt[0] == "1" and int(txt[2]) == A.shape[0]
+    (s2, th2) = _structure(A, None, pa.io.Load, path)
+    for a, b in zip(s1[:4], s2[:4]):
+        assert np.array_equal(a, b)
+    assert th1 == th2
+
+
+def test_ordering_file_version0_without_tree(tmp_path):
+    """A version-0 file (partition and permutation, no tree) is completed
+    with the supernode tree of the elimination tree, as the reference's
+    loader leaves it to the analysis: the structure matches the one of the
+    same partition with its tree."""
+    grid = (8, 7, 6)
+    A = problems.laplacian(*grid)
+    path = str(tmp_path / "ordergen")
+    (s1, th1) = _structure(A, grid, pa.io.Save, path)
+    tok = open(path).read().split()
+    cblknbr, n = int(tok[1]), int(tok[2])
+    body = tok[3:3 + (cblknbr + 1) + n]          # rangtab + permtab, no treetab
+    p0 = str(tmp_path / "v0")
+    with open(p0, "w") as f:
+        f.write("0\n%d\t%d\n%s\n" % (cblknbr, n, "\n".join(body)))
+    (s2, th2) = _structure(A, None, pa.io.Load, p0)
+    assert s2[4][0] == n and len(s2[0]) == len(s1[0])
+    assert th1 == th2
     for a, b in zip(s1[:4], s2[:4]):
         assert np.array_equal(a, b)
