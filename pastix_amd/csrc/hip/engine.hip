@@ -204,6 +204,10 @@ pa_set_lds_limits( int bytes )
     pa_lds_attr( k_solve_diag<double, true, false>, bytes );  pa_lds_attr( k_solve_diag<double, true, true>, bytes );
     pa_lds_attr( k_solve_diag<float, false, false>, bytes );  pa_lds_attr( k_solve_diag<float, false, true>, bytes );
     pa_lds_attr( k_solve_diag<float, true, false>, bytes );   pa_lds_attr( k_solve_diag<float, true, true>, bytes );
+    pa_lds_attr( k_diag<double, true, 512>, bytes ); pa_lds_attr( k_diag<double, false, 512>, bytes );
+    pa_lds_attr( k_diag<double, true, 64>, bytes );  pa_lds_attr( k_diag<double, false, 64>, bytes );
+    pa_lds_attr( k_diag<float, true, 512>, bytes );  pa_lds_attr( k_diag<float, false, 512>, bytes );
+    pa_lds_attr( k_diag<float, true, 64>, bytes );   pa_lds_attr( k_diag<float, false, 64>, bytes );
 }
 
 extern "C" int
@@ -664,7 +668,7 @@ pa_build_schedule( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
 
     /* LDS needs of the diagonal, TRSM and solve kernels */
-    size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 ) + 4096;
+    size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 * 2 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {
         fprintf( stderr, "pastix_amd: cblks of %d columns exceed the LDS budget (%d bytes); lower IPARM_MAX_BLOCKSIZE\n",
                  colmax, dev->lds_limit );
@@ -900,7 +904,8 @@ pa_factorize_t( pa_device_s *dev )
         int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg ), nsm = (int32_t)( lev.cblk_end - lev.fac_end );
         if ( ncb == 0 && nsm == 0 ) continue;
         PA_PROF_MARK( 0 );
-        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax_fac * sizeof( T );
+        /* the panel P, and for LU also the row panel U12 */
+        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax_fac * sizeof( T ) * ( lu ? 2 : 1 );
         const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
         if ( nsm > 0 ) {

@@ -13,6 +13,43 @@
 
 extern "C" {
 #include "pastix_hip.h"
+
+
+}
+
+/* The kernels of these entry points stage up to the device's LDS per
+ * workgroup: raise their dynamic limit once per process (the current device) */
+template <typename K>
+static void
+pa_api_lds_attr( K kernel, int bytes )
+{
+    hipFuncAttributes fa;
+    int               stat = 0;
+    if ( hipFuncGetAttributes( &fa, reinterpret_cast<const void *>( kernel ) ) == hipSuccess ) stat = (int)fa.sharedSizeBytes;
+    (void)hipGetLastError();
+    if ( hipFuncSetAttribute( reinterpret_cast<const void *>( kernel ), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              bytes - stat ) != hipSuccess )
+        (void)hipGetLastError();
+}
+
+static void
+pa_api_lds_limits( void )
+{
+    static bool done = false;
+    if ( done ) return;
+    done = true;
+    int dev = 0, bytes = 65536;
+    if ( hipGetDevice( &dev ) == hipSuccess ) {
+        int v = 0;
+        if ( hipDeviceGetAttribute( &v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev ) == hipSuccess && v > 0 ) bytes = v;
+    }
+    (void)hipGetLastError();
+    pa_api_lds_attr( k_diag_one<double, true>, bytes ); pa_api_lds_attr( k_diag_one<double, false>, bytes );
+    pa_api_lds_attr( k_diag_one<float, true>, bytes );  pa_api_lds_attr( k_diag_one<float, false>, bytes );
+    pa_api_lds_attr( k_trsm_call<double, 0>, bytes ); pa_api_lds_attr( k_trsm_call<double, 1>, bytes );
+    pa_api_lds_attr( k_trsm_call<double, 2>, bytes );
+    pa_api_lds_attr( k_trsm_call<float, 0>, bytes );  pa_api_lds_attr( k_trsm_call<float, 1>, bytes );
+    pa_api_lds_attr( k_trsm_call<float, 2>, bytes );
 }
 
 static inline pastix_int_t
@@ -235,6 +272,7 @@ pa_gpublok_trsmsp( pastix_side_t side, pastix_uplo_t uplo, pastix_trans_t trans,
     memset( &R, 0, sizeof( R ) );
     int32_t rows = 0;
     size_t lds = pa_trsm_lds<T>( N );
+    pa_api_lds_limits();
     for ( ; blok < lblok && blok->fcblknm == cblk_m; blok++ ) {
         int32_t b = R.nblk;
         R.pref[b] = rows;
@@ -299,7 +337,9 @@ pa_gpucblk_diag( const SolverCblk *cblk, void *L, void *U, double criterion, int
         return -1.;
     }
     int32_t nc = (int32_t)cblk_colnbr( cblk );
-    size_t lds = (size_t)pa_diag_nb<T>::value * nc * sizeof( T );
+    /* the panel, and for LU also the row panel U12 */
+    size_t lds = (size_t)pa_diag_nb<T>::value * nc * sizeof( T ) * ( LU ? 2 : 1 );
+    pa_api_lds_limits();
     /* negative diagonal terms are counted in the second word of the counter */
     if ( !LU && sizeof( T ) == 4 && nc > 64 && nc <= 16 * PA_DREG_NT )
         k_diag_potrf_reg_one<<<1, 1024, 0, stream>>>( (float *)L, nc, criterion, d_npiv, d_npiv + 1 );

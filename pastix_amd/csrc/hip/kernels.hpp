@@ -296,7 +296,10 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                 if ( j < w ) D[( p + i ) + (int64_t)( p + j ) * nc] = P[j * m + i];
         }
         if ( LU && q > 0 ) {
-            /* (C) U12 = L11^{-1} A12, unit lower, one column per thread */
+            /* (C) U12 = L11^{-1} A12, unit lower, one column per thread; the
+             * row panel is also kept in LDS (Ur[j * q + c]) for the trailing
+             * update's MFMA operands */
+            T *Ur = P + (int64_t)NB * nc;
             for ( int32_t c = p + w + t; c < nc; c += TH ) {
                 T u[NB];
 #pragma unroll
@@ -307,8 +310,10 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     for ( int l = 0; l < j; l++ ) u[j] -= P[l * m + j] * u[l];
                 }
 #pragma unroll
-                for ( int j = 0; j < NB; j++ )
+                for ( int j = 0; j < NB; j++ ) {
                     if ( j < w ) D[( p + j ) + (int64_t)c * nc] = u[j];
+                    Ur[j * q + ( c - p - w )] = ( j < w ) ? u[j] : T( 0 );
+                }
             }
         }
         __syncthreads();
@@ -355,7 +360,7 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     for ( int u = 0; u < 4; u++ ) {
                         T ra = ( row[u] < q && kk < w ) ? P[kk * m + w + row[u]] : T( 0 );
                         T ca;
-                        if ( LU ) ca = ( col[u] < q && kk < w ) ? D[( p + kk ) + (int64_t)( p + w + col[u] ) * nc] : T( 0 );
+                        if ( LU ) ca = ( col[u] < q && kk < w ) ? P[(int64_t)NB * nc + kk * q + col[u]] : T( 0 );
                         else      ca = ( col[u] < q && kk < w ) ? P[kk * m + w + col[u]] : T( 0 );
                         acc[u] = pa_mfma<T>::run( ca, ra, acc[u] );
                     }
