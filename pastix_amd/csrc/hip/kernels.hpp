@@ -801,6 +801,20 @@ template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = 
  * the block column are staged in LDS first.  LDS: SR*nc + 16*nc + 4*SR*16 T.
  */
 #define PA_TRSM_U 8
+/* LDS strides of pa_trsm_rows, padded so that the MFMA operand reads and the
+ * partial-sum writes hit distinct banks (the unpadded layout spent more
+ * cycles in bank conflicts than in LDS instructions): FP32 reads 32 lanes at
+ * a time (stride = 2 mod 32 words), FP64 16 lanes (stride = 1 mod 16 doubles) */
+template <typename T> struct pa_trsm_pad;
+template <> struct pa_trsm_pad<float> {
+    static __host__ __device__ constexpr int32_t ld( int32_t n ) { return n + ( ( 2 - n ) & 31 ); }
+    static constexpr int RS = PA_TRSM_TR + 4; /* 4 RS = 16 mod 32 */
+};
+template <> struct pa_trsm_pad<double> {
+    static __host__ __device__ constexpr int32_t ld( int32_t n ) { return n + ( ( 1 - n ) & 15 ); }
+    static constexpr int RS = PA_TRSM_TR;
+};
+
 template <typename T, int MODE>
 __device__ void
 pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xoff, const int32_t *xld, T *smem )
@@ -809,9 +823,11 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
     constexpr int SR  = pa_trsm_rows_per_wg<T>::value;
     constexpr int NRT = SR / 16;   /* row tiles                  */
     constexpr int KS  = 4 / NRT;   /* K split among the waves    */
-    T *Xs  = smem;                 /* [nc][SR]                   */
-    T *Ls  = Xs + (int64_t)nc * SR; /* [16][nc]: Ls[jj*nc + l] = C(l, jb+jj) */
-    T *red = Ls + 16 * nc;         /* [KS][16][SR] partial sums  */
+    constexpr int RS  = pa_trsm_pad<T>::RS;
+    const int32_t XT  = pa_trsm_pad<T>::ld( nc );   /* row stride of Xs and Ls */
+    T *Xs  = smem;                                  /* [SR][XT]: Xs[r*XT + j] = X(r, j)        */
+    T *Ls  = Xs + (int64_t)SR * XT;                 /* [16][XT]: Ls[jj*XT + l] = C(l, jb+jj)   */
+    T *red = Ls + 16 * XT;                          /* [KS][16][RS] partial sums               */
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 
     /* global <-> LDS staging loops issue PA_TRSM_U independent loads before
@@ -825,8 +841,8 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
         }
 #pragma unroll
         for ( int u = 0; u < PA_TRSM_U; u++ ) {
-            const int32_t idx = i0 + 256 * u;
-            if ( idx < nc * SR ) Xs[idx] = v[u]; /* idx = j * SR + r */
+            const int32_t idx = i0 + 256 * u, r = idx % SR, j = idx / SR;
+            if ( idx < nc * SR ) Xs[r * XT + j] = v[u];
         }
     }
 
@@ -844,7 +860,7 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
                 if ( MODE == 0 ) { jj = idx % 16; l = idx / 16; }  /* L[jb+jj][l]: contiguous in jj */
                 else             { l = idx % kend; jj = idx / kend; } /* D[l][jb+jj]: contiguous in l  */
                 const bool in = idx < 16 * kend;
-                dst[u] = in ? jj * nc + l : -1;
+                dst[u] = in ? jj * XT + l : -1;
                 v[u]   = ( in && jj < cw ) ? ( MODE == 0 ? D[( jb + jj ) + (int64_t)l * nc] : D[l + (int64_t)( jb + jj ) * nc] )
                                            : T( 0 );
             }
@@ -862,14 +878,14 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
             for ( int r = 0; r < 4; r++ ) acc[r] = T( 0 );
             for ( int32_t k0 = kb; k0 < ke; k0 += 4 ) {
                 const int32_t kk = k0 + ( lane >> 4 );
-                T ca = ( kk < ke ) ? Ls[( lane & 15 ) * nc + kk] : T( 0 );         /* column side (j) */
-                T ra = ( kk < ke ) ? Xs[kk * SR + rt * 16 + ( lane & 15 )] : T( 0 ); /* row side (r)    */
+                T ca = ( kk < ke ) ? Ls[( lane & 15 ) * XT + kk] : T( 0 );           /* column side (j) */
+                T ra = ( kk < ke ) ? Xs[( rt * 16 + ( lane & 15 ) ) * XT + kk] : T( 0 ); /* row side (r)    */
                 acc = pa_mfma<T>::run( ca, ra, acc );
             }
             /* lane holds (row rt*16 + lane%16, column j = row(lane, r)) */
             for ( int r = 0; r < 4; r++ ) {
                 int j = pa_mfma<T>::row( lane, r );
-                red[( ks * 16 + j ) * SR + rt * 16 + ( lane & 15 )] = acc[r];
+                red[( ks * 16 + j ) * RS + rt * 16 + ( lane & 15 )] = acc[r];
             }
         }
         __syncthreads();
@@ -878,10 +894,10 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
             T x[16];
 #pragma unroll
             for ( int jj = 0; jj < 16; jj++ ) {
-                T v = ( jj < cw ) ? Xs[( jb + jj ) * SR + r] : T( 0 );
+                T v = ( jj < cw ) ? Xs[r * XT + jb + jj] : T( 0 );
                 if ( jb > 0 ) {
 #pragma unroll
-                    for ( int ks = 0; ks < KS; ks++ ) v -= red[( ks * 16 + jj ) * SR + r];
+                    for ( int ks = 0; ks < KS; ks++ ) v -= red[( ks * 16 + jj ) * RS + r];
                 }
                 x[jj] = v;
             }
@@ -890,10 +906,10 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
                 if ( jj < cw ) {
                     T v = x[jj];
 #pragma unroll
-                    for ( int l = 0; l < jj; l++ ) v -= x[l] * Ls[jj * nc + jb + l];
-                    if ( MODE != 2 ) v /= Ls[jj * nc + jb + jj];
+                    for ( int l = 0; l < jj; l++ ) v -= x[l] * Ls[jj * XT + jb + l];
+                    if ( MODE != 2 ) v /= Ls[jj * XT + jb + jj];
                     x[jj] = v;
-                    Xs[( jb + jj ) * SR + r] = v;
+                    Xs[r * XT + jb + jj] = v;
                 }
             }
         }
@@ -901,7 +917,7 @@ pa_trsm_rows( T *panel, const T *D, int32_t nc, int32_t rows, const int64_t *xof
     }
     for ( int32_t idx = t; idx < nc * SR; idx += 256 ) {
         int32_t r = idx % SR, j = idx / SR;
-        if ( r < rows ) panel[xoff[r] + (int64_t)j * xld[r]] = Xs[j * SR + r];
+        if ( r < rows ) panel[xoff[r] + (int64_t)j * xld[r]] = Xs[r * XT + j];
     }
 }
 #undef PA_TRSM_U
@@ -911,8 +927,8 @@ template <typename T>
 __host__ __device__ constexpr size_t
 pa_trsm_lds( int32_t nc )
 {
-    return ( (size_t)nc * pa_trsm_rows_per_wg<T>::value + 16 * (size_t)nc +
-             4 * 16 * (size_t)pa_trsm_rows_per_wg<T>::value ) * sizeof( T );
+    return ( (size_t)pa_trsm_rows_per_wg<T>::value * pa_trsm_pad<T>::ld( nc ) + 16 * (size_t)pa_trsm_pad<T>::ld( nc ) +
+             4 * 16 * (size_t)pa_trsm_pad<T>::RS ) * sizeof( T );
 }
 
 template <typename T, int MODE>
