@@ -55,6 +55,8 @@ struct pa_level_t {
     int64_t sm_end[3];            /* by row slots: [fac_end, sm_end[0]) of  */
                                   /* <= 64 rows, then <= 128, <= 192        */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
+    int64_t tinv_beg, tinv_end;   /* into d_tinv_tasks (diagonal tiles)     */
+    bool    trsm_inv;             /* TRSM by inverted tiles (k_trsm_inv)    */
     int64_t gc_beg[PA_NGC], gc_end[PA_NGC]; /* into d_gemm_tasks, per tile class */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
     int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
@@ -114,6 +116,13 @@ struct pa_device_s {
     int2         *d_trsm_tasks;
     pa_gemm_task *d_gemm_tasks;
     int4         *d_solve_tasks, *d_bsolve_tasks;
+    /* TRSM by inverted diagonal tiles: first tile of each cblk (-1: none),
+     * the (cblk, tile) list of every level and the inverses (one region per
+     * TRSM mode: two for LU) */
+    int64_t      *d_c_tinv = nullptr;
+    int2         *d_tinv_tasks = nullptr;
+    void         *d_tinv = nullptr;
+    int64_t       tinv_tiles = 0;
     /* matrix */
     int64_t *d_colptr;
     int32_t *d_rowtab;
@@ -313,6 +322,10 @@ pa_free_structure( pa_device_s *dev )
                      dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info,
                      dev->d_r_blk };
     for ( void *p : ptrs ) hipFree( p );
+    hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv );
+    dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL;
+    dev->tinv_tiles = 0;
+    dev->S.c_tinv = NULL;
     dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
     dev->d_c_coef = NULL;
     dev->d_b_frow = dev->d_b_lrow = dev->d_b_fcblk = dev->d_b_rowoff = NULL;
@@ -494,6 +507,9 @@ pa_build_schedule( pastix_data_t *pd )
         for ( void *p : ptrs ) hipFree( p );
         dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
         dev->d_bsolve_tasks = NULL;
+        hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv );
+        dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL;
+        dev->tinv_tiles = 0;
         dev->levels.clear();
     }
 
@@ -508,7 +524,12 @@ pa_build_schedule( pastix_data_t *pd )
      * (64 doubles per lane) leave one wave per SIMD, so 64 x 64 is kept */
     const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
     std::vector<int32_t>      lev_cblks;
-    std::vector<int2>         trsm_tasks;
+    std::vector<int2>         trsm_tasks, tinv_tasks;
+    std::vector<int64_t>      c_tinv( std::max<pastix_int_t>( cblknbr, 1 ), -1 );
+    int64_t                   tinv_tiles = 0;
+    /* PASTIX_AMD_TRSM=0 keeps the substitution TRSM (k_trsm) on every level */
+    const char *tr_env  = getenv( "PASTIX_AMD_TRSM" );
+    const bool  tr_inv  = !( tr_env && tr_env[0] == '0' );
     std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gtasks[PA_NGC];
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
@@ -552,6 +573,7 @@ pa_build_schedule( pastix_data_t *pd )
         pa_level_t &L = dev->levels[l];
         L.cblk_beg  = lev_cblks.size();
         L.trsm_beg  = trsm_tasks.size();
+        L.tinv_beg  = tinv_tasks.size();
         for ( int c = 0; c < PA_NGC; c++ ) L.gc_beg[c] = gtasks[c].size();
         L.solve_beg = solve_tasks.size();
         L.bsolve_beg = bsolve_tasks.size();
@@ -598,6 +620,11 @@ pa_build_schedule( pastix_data_t *pd )
             for ( int32_t r = 0; r < R; r += 1024 / ncg )
                 for ( int32_t c = 0; c < nc; c += 16 * ncg ) bsolve_tasks.push_back( make_int4( k, r, c, 0 ) );
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
+            if ( R > 0 ) {
+                c_tinv[k] = tinv_tiles;
+                for ( int32_t tl = 0; tl < ( nc + 15 ) / 16; tl++ ) tinv_tasks.push_back( make_int2( k, tl ) );
+                tinv_tiles += ( nc + 15 ) / 16;
+            }
             /* column groups: consecutive blocks facing the same cblk */
             int32_t b = c_fblok[k] + 1, lb = c_fblok[k + 1];
             while ( b < lb ) {
@@ -646,6 +673,8 @@ pa_build_schedule( pastix_data_t *pd )
         for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
         L.cblk_end  = lev_cblks.size();
         L.trsm_end  = trsm_tasks.size();
+        L.tinv_end  = tinv_tasks.size();
+        L.trsm_inv  = tr_inv && L.ncmax_fac <= PA_TI_MAXNC;
         for ( int c = 0; c < PA_NGC; c++ ) L.gc_end[c] = gtasks[c].size();
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
@@ -653,6 +682,12 @@ pa_build_schedule( pastix_data_t *pd )
     dev->lu = lu;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_tinv_tasks, tinv_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_c_tinv, c_tinv ) ) ) return rc;
+    dev->tinv_tiles = tinv_tiles;
+    /* inverses in the widest precision, one region per TRSM mode */
+    PA_HIP( hipMalloc( &dev->d_tinv, (size_t)std::max<int64_t>( tinv_tiles, 1 ) * 256 * sizeof( double ) * ( lu ? 2 : 1 ) ) );
+    dev->S.c_tinv = dev->d_c_tinv;
     /* the task lists of all classes in one array, class after class */
     std::vector<pa_gemm_task> gemm_tasks;
     for ( int c = 0; c < PA_NGC; c++ ) {
@@ -931,7 +966,26 @@ pa_factorize_t( pa_device_s *dev )
         PA_PROF_MARK( 1 );
         int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
-        if ( ntr > 0 ) {
+        if ( ntr > 0 && lev.trsm_inv ) {
+            const int32_t ntl = (int32_t)( lev.tinv_end - lev.tinv_beg );
+            const unsigned gti = (unsigned)( ( ntl + 3 ) / 4 );
+            const int2   *tt   = dev->d_tinv_tasks + lev.tinv_beg;
+            const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
+            T            *Ti   = (T *)dev->d_tinv;
+            T            *Ti2  = Ti + dev->tinv_tiles * 256;
+            if ( lu ) {
+                k_tile_inv<T, 1><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti );
+                k_tile_inv<T, 2><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti2 );
+                k_trsm_inv<T, 1><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti );
+                k_trsm_inv<T, 2><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti2 );
+            }
+            else {
+                k_tile_inv<T, 0><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti );
+                k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti );
+            }
+            PA_HIP( hipGetLastError() );
+        }
+        else if ( ntr > 0 ) {
             size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
             if ( lu ) {
                 k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );

@@ -37,6 +37,8 @@ struct pa_dstruct {
     const int64_t *r_base;   /* [cblknbr]   first row descriptor of the cblk     */
     const int4    *r_info;   /* [sum of off-diagonal rows]                       */
     const int32_t *r_blk;    /* [sum of off-diagonal rows] block of the row      */
+    const int64_t *c_tinv;   /* [cblknbr]   first inverted diagonal tile of the
+                                            cblk (k_tile_inv, k_trsm_inv)        */
 };
 
 /* LDS row padding of the GEMM operand tiles: 64 bytes, so that the four
@@ -283,6 +285,8 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     for ( int l = 0; l < j; l++ ) s -= x[l] * ( LU ? P[j * m + l] : P[l * m + j] );
                     x[j] = s / P[j * m + j];
                 }
+                /* one column's coefficient loads at a time (register budget) */
+                __builtin_amdgcn_sched_barrier( 0 );
             }
 #pragma unroll
             for ( int j = 0; j < NB; j++ )
@@ -308,6 +312,7 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                 for ( int j = 0; j < NB; j++ ) {
 #pragma unroll
                     for ( int l = 0; l < j; l++ ) u[j] -= P[l * m + j] * u[l];
+                    __builtin_amdgcn_sched_barrier( 0 );
                 }
 #pragma unroll
                 for ( int j = 0; j < NB; j++ ) {
@@ -450,12 +455,12 @@ k_diag_one( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 #define PA_DREG_S  16 /* slots per wave: 16 * 16 >= 22 * 23 / 2 tiles */
 #define PA_DREG_LD 17
 
-__device__ void
+__device__ __forceinline__ void
 pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 {
     typedef pa_mfma<float>::acc_t acc_t;
     __shared__ float P[2][PA_DREG_NT * 16][PA_DREG_LD];
-    __shared__ float Rinv[16];
+    __shared__ __align__( 16 ) float Lt[16][16]; /* Lt[c][j] = L(j, c), 1 / L(c, c) on the diagonal */
     const int     t = threadIdx.x, lane = t & 63;
     const int     wave = __builtin_amdgcn_readfirstlane( t >> 6 );
     const int32_t nt = ( nc + 15 ) >> 4, ntiles = ( nt * ( nt + 1 ) ) >> 1;
@@ -472,12 +477,15 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
             j++;
         }
         const bool used = idx < ntiles;
-        ti[s] = used ? j + rem : -1;
-        tj[s] = used ? j : -1;
+        ti[s] = __builtin_amdgcn_readfirstlane( used ? j + rem : -1 );
+        tj[s] = __builtin_amdgcn_readfirstlane( used ? j : -1 );
 #pragma unroll
         for ( int r = 0; r < 4; r++ ) {
+            /* unconditional load from a clamped address (no branch per load) */
             const int32_t row = 16 * ti[s] + ( lane & 15 ), col = 16 * tj[s] + 4 * ( lane >> 4 ) + r;
-            acc[s][r] = ( used && row < nc && col < nc ) ? D[row + (int64_t)col * nc] : 0.f;
+            const int32_t rc = max( 0, min( row, nc - 1 ) ), cc = max( 0, min( col, nc - 1 ) );
+            const float   v  = D[rc + (int64_t)cc * nc];
+            acc[s][r] = ( used && row < nc && col < nc ) ? v : 0.f;
         }
     }
 
@@ -494,7 +502,11 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
             }
         }
         __syncthreads();
-        /* (2) 16 x 16 diagonal tile by wave 0, lane i (mod 16) holding row i */
+        /* (2) 16 x 16 diagonal tile by wave 0, lane i (mod 16) holding row i.
+         * Branch-free: every lane applies every column update (the entries
+         * right of the diagonal of a row are scratch, never stored or read
+         * back), and columns past the last one of the matrix (last panel
+         * only) factor as an identity without being counted as pivots. */
         if ( wave == 0 ) {
             const int i = lane & 15;
             float     a[16];
@@ -503,54 +515,59 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
             float rinv = 1.f;
 #pragma unroll
             for ( int k = 0; k < 16; k++ ) {
-                if ( k < w ) {
-                    /* l_kk = sqrt(a_kk), column scaled by 1 / l_kk as the
-                     * reference kernel (hardware sqrt / reciprocal: the FP32
-                     * factor is refined in FP64 anyway) */
-                    float      d   = pa_readlane( a[k], k );
-                    const bool rep = fabsf( d ) < tcrit;
-                    if ( rep ) d = tcrit;
-                    if ( lane == 0 ) {
-                        mypiv += rep ? 1 : 0;
-                        myneg += ( d < 0.f ) ? 1 : 0;
-                    }
-                    const float l     = __builtin_amdgcn_sqrtf( d );
-                    const float alpha = __builtin_amdgcn_rcpf( l );
-                    if ( i == k ) {
-                        a[k] = l;
-                        rinv = alpha;
-                    }
-                    else if ( i > k ) a[k] *= alpha;
+                /* l_kk = sqrt(a_kk), column scaled by 1 / l_kk as the
+                 * reference kernel (hardware sqrt / reciprocal: the FP32
+                 * factor is refined in FP64 anyway) */
+                float      d   = pa_readlane( a[k], k );
+                const bool pad = k >= w;
+                const bool rep = !pad && fabsf( d ) < tcrit;
+                d = pad ? 1.f : ( rep ? tcrit : d );
+                mypiv += rep ? 1 : 0;
+                myneg += ( d < 0.f ) ? 1 : 0;
+                const float l     = __builtin_amdgcn_sqrtf( d );
+                const float alpha = __builtin_amdgcn_rcpf( l );
+                a[k] = ( i == k ) ? l : a[k] * alpha;
+                rinv = ( i == k ) ? alpha : rinv;
 #pragma unroll
-                    for ( int j = k + 1; j < 16; j++ ) {
-                        if ( j < w ) {
-                            const float ljk = pa_readlane( a[k], j );
-                            if ( i >= j ) a[j] -= a[k] * ljk;
-                        }
+                for ( int j = 0; j < 16; j++ ) {
+                    if ( j > k ) {
+                        const float ljk = pa_readlane( a[k], j );
+                        a[j] -= a[k] * ljk;
                     }
                 }
             }
             if ( lane < 16 ) {
 #pragma unroll
                 for ( int j = 0; j < 16; j++ ) Pp[i][j] = a[j];
-                Rinv[i] = rinv;
+                /* column c of the tile, contiguous, 1 / l_cc on the diagonal */
+#pragma unroll
+                for ( int j = 0; j < 16; j++ )
+                    if ( j <= i ) Lt[j][i] = ( j == i ) ? rinv : a[j];
             }
         }
         __syncthreads();
         /* (3) panel rows below the diagonal tile: X L^T = A by substitution,
          * column by column (x_j -= x_c l_jc in increasing c, the order of
-         * the reference kernel) so the dependent chain is 2 ops per column */
+         * the reference kernel) so the dependent chain is 2 ops per column;
+         * column c of the tile is read with 16-byte LDS loads.  Only full
+         * panels have rows below the tile. */
         if ( t < m - 16 ) {
             float x[16];
 #pragma unroll
             for ( int c = 0; c < 16; c++ ) x[c] = Pp[16 + t][c];
 #pragma unroll
             for ( int c = 0; c < 16; c++ ) {
-                if ( c < w ) {
-                    x[c] *= Rinv[c];
+                float cur[16];
 #pragma unroll
-                    for ( int j = c + 1; j < 16; j++ ) x[j] -= x[c] * Pp[j][c];
+                for ( int q = c / 4; q < 4; q++ ) {
+                    const float4 v = *reinterpret_cast<const float4 *>( &Lt[c][4 * q] );
+                    cur[4 * q] = v.x; cur[4 * q + 1] = v.y; cur[4 * q + 2] = v.z; cur[4 * q + 3] = v.w;
                 }
+                x[c] *= cur[c];
+#pragma unroll
+                for ( int j = 0; j < 16; j++ )
+                    if ( j > c ) x[j] -= x[c] * cur[j];
+                __builtin_amdgcn_sched_barrier( 0 );
             }
 #pragma unroll
             for ( int c = 0; c < 16; c++ ) Pp[16 + t][c] = x[c];
@@ -558,8 +575,12 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
         __syncthreads();
         /* (4) store the finished panel (lower part of the diagonal tile) */
         if ( t < m ) {
-            for ( int c = 0; c < w; c++ )
-                if ( t >= c ) D[( 16 * p + t ) + (int64_t)( 16 * p + c ) * nc] = Pp[t][c];
+            float v[16];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) v[c] = Pp[t][c];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ )
+                if ( c < w && t >= c ) D[( 16 * p + t ) + (int64_t)( 16 * p + c ) * nc] = v[c];
         }
         /* (5) trailing update of the owned tiles right of the panel:
          * C(i, j)^T -= L(j, p) L(i, p)^T */
@@ -574,6 +595,9 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     acc[s] = pa_mfma<float>::run( a, b, acc[s] );
                 }
             }
+            /* keep each slot's operand loads next to its MFMAs: hoisting
+             * them over all slots exceeds the register budget */
+            __builtin_amdgcn_sched_barrier( 0 );
         }
     }
     if ( t == 0 ) {
@@ -963,6 +987,207 @@ k_trsm( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
     }
     __syncthreads();
     pa_trsm_rows<T, MODE>( panel, panel, nc, rows, xoff, xld, reinterpret_cast<T *>( pa_smem_raw ) );
+}
+
+/* ------------------------------------------------------------------------- */
+/* TRSM by inverted diagonal tiles (the scheme of the rocBLAS and MAGMA       */
+/* TRSMs).  The rows X below the diagonal block of a cblk solve X T = B with  */
+/* T upper triangular, T(l, c) = Q[c + l nc] for the diagonal block Q that    */
+/* holds it contiguously in c: L^T of LL^t (Q = L), U of LU (Q = the U^T      */
+/* copy) and the unit L^T of LU (Q = L).  k_tile_inv inverts the 16 x 16      */
+/* diagonal tiles of T once per level (one wave per tile); k_trsm_inv keeps   */
+/* 32 rows of X in MFMA accumulators and sweeps the column tiles left to      */
+/* right: X_j = X_j T_jj^{-1} by the wave that owns column tile j, then       */
+/* X_c -= X_j T_jc for the later column tiles c by their owners.  The        */
+/* accumulators hold X^T tiles, so the 16 lanes of an output row are 16      */
+/* consecutive rows of X (coalesced loads and stores), and an accumulator    */
+/* tile is already the B operand of the next 16x16x4 MFMA when the K index   */
+/* of lane group q, step s is the accumulator row row(l, s).                 */
+/* ------------------------------------------------------------------------- */
+#define PA_TI_NJ 6 /* column tiles per wave: nc <= 4 * 16 * PA_TI_NJ */
+#define PA_TI_MAXNC ( 64 * PA_TI_NJ )
+
+template <typename T, int MODE>
+__global__ __launch_bounds__( 256 ) void
+k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T *Uc, T *tinv )
+{
+    __shared__ __align__( 16 ) T sT[4][16][16]; /* per wave: sT[c][l] = T(l, c) of the tile */
+    const int     lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t task  = blockIdx.x * 4 + wave;
+    const bool    valid = task < ntask;
+    int32_t k = 0, tl = 0, nc = 16;
+    if ( valid ) {
+        const int2 tk = tasks[task];
+        k  = tk.x;
+        tl = tk.y;
+        nc = S.c_ncol[k];
+    }
+    const int32_t c0 = 16 * tl, w = min( 16, nc - c0 );
+    const T      *Q  = ( MODE == 1 ? Uc : Lc ) + S.c_coef[k];
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        const int idx = lane + 64 * u, c = idx >> 4, l = idx & 15;
+        /* columns past the cblk: identity */
+        T v = ( c == l ) ? T( 1 ) : T( 0 );
+        if ( valid && c < w && l <= c && !( MODE == 2 && l == c ) ) v = Q[( c0 + c ) + (int64_t)( c0 + l ) * nc];
+        sT[wave][c][l] = v;
+    }
+    __syncthreads();
+    if ( valid && lane < 16 ) {
+        /* row r of M = T^{-1}: sum_{l <= c} M(r, l) T(l, c) = delta_rc */
+        const int r = lane;
+        T m[16];
+#pragma unroll
+        for ( int c = 0; c < 16; c++ ) {
+            T v = ( r == c ) ? T( 1 ) : T( 0 );
+#pragma unroll
+            for ( int l = 0; l < 16; l++ )
+                if ( l < c ) v -= m[l] * sT[wave][c][l];
+            m[c] = ( MODE == 2 ) ? v : v / sT[wave][c][c];
+        }
+        T *out = tinv + ( S.c_tinv[k] + tl ) * 256 + r * 16;
+#pragma unroll
+        for ( int c = 0; c < 16; c++ ) out[c] = m[c];
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__( 256 ) void
+k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv )
+{
+    typedef typename pa_mfma<T>::acc_t acc_t;
+    constexpr int SR  = PA_TRSM_TR;
+    constexpr int NRT = SR / 16;
+    constexpr int NJ  = PA_TI_NJ;
+    __shared__ int64_t xoff[SR];
+    __shared__ int32_t xld[SR];
+    __shared__ T       xb[2][NRT][4][64]; /* solved column tile, B-operand layout */
+    const int t = threadIdx.x, lane = t & 63, li = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane( t >> 6 );
+
+    const int2    task = tasks[blockIdx.x];
+    const int32_t k = task.x, r0 = task.y;
+    const int32_t nc = S.c_ncol[k], R = S.c_stride[k] - nc;
+    const int32_t rows = R - r0 < SR ? R - r0 : SR;
+    const int32_t nt = ( nc + 15 ) >> 4;
+    T            *panel = ( MODE == 2 ? Uc : Lc ) + S.c_coef[k];
+    const T      *Q     = ( MODE == 1 ? Uc : Lc ) + S.c_coef[k];
+    const T      *Mi    = tinv + S.c_tinv[k] * 256;
+
+    if ( t < SR ) {
+        if ( t < rows ) {
+            int32_t x = nc + r0 + t;
+            int32_t b = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, x );
+            int32_t loc = x - S.b_rowoff[b];
+            xoff[t] = (int64_t)S.b_rowoff[b] * nc + loc;
+            xld[t]  = S.b_lrow[b] - S.b_frow[b] + 1;
+        }
+        else {
+            xoff[t] = 0;
+            xld[t]  = 0;
+        }
+    }
+    __syncthreads();
+
+    /* this lane's rows rt * 16 + li; empty rows read X(0, 0) and are zeroed */
+    int64_t ro[NRT];
+    int32_t rl[NRT];
+    bool    rv[NRT];
+#pragma unroll
+    for ( int rt = 0; rt < NRT; rt++ ) {
+        const int row = rt * 16 + li;
+        rv[rt] = row < rows;
+        ro[rt] = xoff[row];
+        rl[rt] = xld[row];
+    }
+    /* accumulator tile (rt, jj): X^T of rows rt*16.., column tile wave + 4 jj */
+    acc_t acc[NRT][NJ];
+#pragma unroll
+    for ( int jj = 0; jj < NJ; jj++ ) {
+        const int32_t j = wave + 4 * jj;
+#pragma unroll
+        for ( int s = 0; s < 4; s++ ) {
+            const int32_t col = 16 * j + pa_mfma<T>::row( lane, s );
+            const int32_t cc  = min( col, nc - 1 );
+#pragma unroll
+            for ( int rt = 0; rt < NRT; rt++ ) {
+                const T v = panel[ro[rt] + (int64_t)cc * rl[rt]];
+                acc[rt][jj][s] = ( rv[rt] && col < nc ) ? v : T( 0 );
+            }
+        }
+    }
+    /* operands of the first step: T_0c tiles of the owned columns, and the
+     * inverse of the first owned diagonal tile */
+    T a[NJ][4], ai[4];
+#pragma unroll
+    for ( int jj = 0; jj < NJ; jj++ ) {
+        const int32_t j = wave + 4 * jj, col = min( 16 * j + li, nc - 1 );
+#pragma unroll
+        for ( int s = 0; s < 4; s++ ) a[jj][s] = -Q[col + (int64_t)min( pa_mfma<T>::row( lane, s ), nc - 1 ) * nc];
+    }
+#pragma unroll
+    for ( int s = 0; s < 4; s++ ) ai[s] = Mi[min( wave, nt - 1 ) * 256 + pa_mfma<T>::row( lane, s ) * 16 + li];
+
+    for ( int32_t jb = 0; jb < nt; jb++ ) {
+        T( *xs )[4][64] = xb[jb & 1];
+        if ( wave == ( jb & 3 ) ) {
+            const int jo = jb >> 2;
+#pragma unroll
+            for ( int jj = 0; jj < NJ; jj++ ) {
+                if ( jj == jo ) {
+#pragma unroll
+                    for ( int rt = 0; rt < NRT; rt++ ) {
+                        acc_t x;
+                        for ( int r = 0; r < 4; r++ ) x[r] = T( 0 );
+#pragma unroll
+                        for ( int s = 0; s < 4; s++ ) x = pa_mfma<T>::run( ai[s], acc[rt][jj][s], x );
+                        acc[rt][jj] = x;
+#pragma unroll
+                        for ( int s = 0; s < 4; s++ ) xs[rt][s][lane] = x[s];
+                    }
+                }
+            }
+            /* inverse of the next owned diagonal tile */
+            const int32_t jn = min( jb + 4, nt - 1 );
+#pragma unroll
+            for ( int s = 0; s < 4; s++ ) ai[s] = Mi[jn * 256 + pa_mfma<T>::row( lane, s ) * 16 + li];
+        }
+        __syncthreads();
+        T bx[NRT][4];
+#pragma unroll
+        for ( int rt = 0; rt < NRT; rt++ )
+#pragma unroll
+            for ( int s = 0; s < 4; s++ ) bx[rt][s] = xs[rt][s][lane];
+        const int32_t kn = min( 16 * ( jb + 1 ), nc - 1 ); /* first row of the next step's T tiles */
+#pragma unroll
+        for ( int jj = 0; jj < NJ; jj++ ) {
+            const int32_t j = wave + 4 * jj;
+            if ( j > jb && j < nt ) {
+#pragma unroll
+                for ( int rt = 0; rt < NRT; rt++ )
+#pragma unroll
+                    for ( int s = 0; s < 4; s++ ) acc[rt][jj] = pa_mfma<T>::run( a[jj][s], bx[rt][s], acc[rt][jj] );
+                /* T_(jb+1) c for the next step */
+                const int32_t col = min( 16 * j + li, nc - 1 );
+#pragma unroll
+                for ( int s = 0; s < 4; s++ )
+                    a[jj][s] = -Q[col + (int64_t)min( kn + pa_mfma<T>::row( lane, s ), nc - 1 ) * nc];
+            }
+        }
+    }
+#pragma unroll
+    for ( int jj = 0; jj < NJ; jj++ ) {
+        const int32_t j = wave + 4 * jj;
+        if ( j < nt ) {
+#pragma unroll
+            for ( int s = 0; s < 4; s++ ) {
+                const int32_t col = 16 * j + pa_mfma<T>::row( lane, s );
+#pragma unroll
+                for ( int rt = 0; rt < NRT; rt++ )
+                    if ( rv[rt] && col < nc ) panel[ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s];
+            }
+        }
+    }
 }
 
 /* Per-call description of a run of blocks (reference-signature entry points):
