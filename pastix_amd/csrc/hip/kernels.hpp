@@ -202,11 +202,12 @@ template <> struct pa_mfma<float> {
 /* TH threads per workgroup: 1024 for wide diagonal blocks, 64 for the
  * small blocks of the lower levels */
 template <typename T, bool LU, int TH>
-__device__ void
+__device__ __forceinline__ void
 pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg, T *P )
 {
     constexpr int NB = pa_diag_nb<T>::value;
     typedef typename pa_mfma<T>::acc_t acc_t;
+    __shared__ T pa_rinv[NB]; /* reciprocals of the panel's pivots */
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     int32_t   mypiv = 0, myneg = 0;
     const T   tcrit = (T)crit;
@@ -236,61 +237,73 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
         }
         __syncthreads();
 
-        /* (A) w x w block by wave 0, lane i holding row i */
+        /* (A) w x w block by wave 0, lane i holding row i.  Branch-free:
+         * broadcasts by v_readlane, updates masked by selects, and columns
+         * past the last one of the matrix (last panel only) factored as an
+         * identity without being counted as pivots.  As the reference kernel
+         * the column is scaled by the reciprocal of the pivot, which lane k
+         * keeps (rinv) for the rows below. */
         if ( wave == 0 ) {
             T a[NB];
             const int i = lane;
 #pragma unroll
             for ( int j = 0; j < NB; j++ ) a[j] = ( i < w && j < w ) ? P[j * m + i] : T( 0 );
+            T rinv = T( 1 );
 #pragma unroll
             for ( int k = 0; k < NB; k++ ) {
-                if ( k < w ) {
-                    T    d   = __shfl( a[k], k );
-                    bool rep = pa_abs( d ) < tcrit;
-                    if ( rep ) d = ( LU && d < T( 0 ) ) ? -tcrit : tcrit;
-                    if ( lane == 0 && rep ) mypiv++;
-                    if ( !LU ) {
-                        if ( lane == 0 && d < T( 0 ) ) myneg++;
-                        d = sqrt( d );
-                    }
-                    if ( i == k ) a[k] = d;
-                    else if ( i > k ) a[k] = a[k] / d;
+                const bool pad = k >= w;
+                T          d   = pa_readlane( a[k], k );
+                const bool rep = !pad && pa_abs( d ) < tcrit;
+                if ( LU ) d = pad ? T( 1 ) : ( rep ? ( d < T( 0 ) ? -tcrit : tcrit ) : d );
+                else      d = pad ? T( 1 ) : ( rep ? tcrit : d );
+                mypiv += rep ? 1 : 0;
+                if ( !LU ) {
+                    myneg += ( d < T( 0 ) ) ? 1 : 0;
+                    d = sqrt( d );
+                }
+                const T alpha = T( 1 ) / d;
+                rinv = ( i == k ) ? alpha : rinv;
+                a[k] = ( i == k ) ? d : ( ( i > k ) ? a[k] * alpha : a[k] );
 #pragma unroll
-                    for ( int j = k + 1; j < NB; j++ ) {
-                        if ( j < w ) {
-                            T f = LU ? __shfl( a[j], k ) : __shfl( a[k], j );
-                            if ( i > k && ( LU || i >= j ) ) a[j] -= a[k] * f;
+                for ( int j = 0; j < NB; j++ ) {
+                    if ( j > k ) {
+                        if ( LU ) {
+                            const T ukj = pa_readlane( a[j], k );
+                            a[j] = ( i > k ) ? a[j] - a[k] * ukj : a[j];
+                        }
+                        else {
+                            const T ljk = pa_readlane( a[k], j );
+                            a[j] = ( i >= j ) ? a[j] - a[k] * ljk : a[j];
                         }
                     }
                 }
             }
             if ( i < w ) {
 #pragma unroll
-                for ( int j = 0; j < NB; j++ )
-                    if ( j < w ) P[j * m + i] = a[j];
+                for ( int j = 0; j < NB; j++ ) P[j * m + i] = a[j];
             }
+            if ( i < NB ) pa_rinv[i] = rinv;
         }
         __syncthreads();
 
-        /* (B) rows below: X L11^T = A21 (POTRF) or X U11 = A21 (LU) */
+        /* (B) rows below: X L11^T = A21 (POTRF) or X U11 = A21 (LU), column
+         * by column (x_c *= 1 / d_c, then x_j -= x_c t_cj for j > c, the
+         * order of the reference kernel).  Only full panels have rows below. */
         for ( int32_t r = w + t; r < m; r += TH ) {
             T x[NB];
 #pragma unroll
-            for ( int j = 0; j < NB; j++ ) x[j] = ( j < w ) ? P[j * m + r] : T( 0 );
+            for ( int j = 0; j < NB; j++ ) x[j] = P[j * m + r];
 #pragma unroll
-            for ( int j = 0; j < NB; j++ ) {
-                if ( j < w ) {
-                    T s = x[j];
+            for ( int c = 0; c < NB; c++ ) {
+                x[c] *= pa_rinv[c];
 #pragma unroll
-                    for ( int l = 0; l < j; l++ ) s -= x[l] * ( LU ? P[j * m + l] : P[l * m + j] );
-                    x[j] = s / P[j * m + j];
-                }
+                for ( int j = 0; j < NB; j++ )
+                    if ( j > c ) x[j] -= x[c] * ( LU ? P[j * m + c] : P[c * m + j] );
                 /* one column's coefficient loads at a time (register budget) */
                 __builtin_amdgcn_sched_barrier( 0 );
             }
 #pragma unroll
-            for ( int j = 0; j < NB; j++ )
-                if ( j < w ) P[j * m + r] = x[j];
+            for ( int j = 0; j < NB; j++ ) P[j * m + r] = x[j];
         }
         __syncthreads();
 
@@ -300,24 +313,26 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                 if ( j < w ) D[( p + i ) + (int64_t)( p + j ) * nc] = P[j * m + i];
         }
         if ( LU && q > 0 ) {
-            /* (C) U12 = L11^{-1} A12, unit lower, one column per thread; the
-             * row panel is also kept in LDS (Ur[j * q + c]) for the trailing
-             * update's MFMA operands */
+            /* (C) U12 = L11^{-1} A12, unit lower, one column per thread
+             * (u_j -= l_jc u_c for j > c, column by column); the row panel
+             * is also kept in LDS (Ur[j * q + c]) for the trailing update's
+             * MFMA operands.  q > 0: a full panel. */
             T *Ur = P + (int64_t)NB * nc;
             for ( int32_t c = p + w + t; c < nc; c += TH ) {
                 T u[NB];
 #pragma unroll
-                for ( int j = 0; j < NB; j++ ) u[j] = ( j < w ) ? D[( p + j ) + (int64_t)c * nc] : T( 0 );
+                for ( int j = 0; j < NB; j++ ) u[j] = D[( p + j ) + (int64_t)c * nc];
 #pragma unroll
-                for ( int j = 0; j < NB; j++ ) {
+                for ( int l = 0; l < NB; l++ ) {
 #pragma unroll
-                    for ( int l = 0; l < j; l++ ) u[j] -= P[l * m + j] * u[l];
+                    for ( int j = 0; j < NB; j++ )
+                        if ( j > l ) u[j] -= P[l * m + j] * u[l];
                     __builtin_amdgcn_sched_barrier( 0 );
                 }
 #pragma unroll
                 for ( int j = 0; j < NB; j++ ) {
-                    if ( j < w ) D[( p + j ) + (int64_t)c * nc] = u[j];
-                    Ur[j * q + ( c - p - w )] = ( j < w ) ? u[j] : T( 0 );
+                    D[( p + j ) + (int64_t)c * nc] = u[j];
+                    Ur[j * q + ( c - p - w )] = u[j];
                 }
             }
         }
