@@ -199,6 +199,94 @@ template <> struct pa_mfma<float> {
     static __device__ __forceinline__ int row( int lane, int r ) { return 4 * ( lane >> 4 ) + r; }
 };
 
+/* Transposes between the two diagonal blocks of an LU cblk:
+ * ADD: D(i, j) += Du(j, i) for i < j (the strict upper part of A, stored
+ * transposed in the U^T block, merged into the block being factored);
+ * otherwise Du = D^T (the factored block's U^T copy).  Wide blocks go through
+ * LDS tiles of 32 x 32 (buf: at least 32 nc elements, the panel area of the
+ * diagonal kernel) so that reads and writes are both coalesced; each thread
+ * issues all of its loads of a batch of tiles before storing any. */
+template <typename T, int TH, bool ADD>
+__device__ __forceinline__ void
+pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
+{
+    const int t = threadIdx.x;
+    constexpr int MB = 8, EPT = 1024 / TH; /* tiles per batch, elements per thread and tile */
+    const int32_t cap = min( MB, ( 32 * nc ) / ( 32 * 33 ) );
+    if constexpr ( TH >= 256 ) {
+        if ( cap >= 2 ) {
+            const int32_t ntb    = ( nc + 31 ) >> 5;
+            const int32_t npairs = ADD ? ( ntb * ( ntb + 1 ) ) / 2 : ntb * ntb;
+            /* tile pair pi -> (a, b): ADD enumerates a <= b, column of tiles b by b */
+            auto pair = [&]( int32_t pi, int32_t &a, int32_t &b ) {
+                if ( ADD ) {
+                    b = (int32_t)( ( sqrtf( 8.f * (float)pi + 1.f ) - 1.f ) * 0.5f );
+                    while ( ( b * ( b + 1 ) ) / 2 > pi ) b--;
+                    while ( ( ( b + 1 ) * ( b + 2 ) ) / 2 <= pi ) b++;
+                    a = pi - ( b * ( b + 1 ) ) / 2;
+                }
+                else {
+                    a = pi % ntb;
+                    b = pi / ntb;
+                }
+            };
+            for ( int32_t base = 0; base < npairs; base += cap ) {
+                const int32_t nb = min( cap, npairs - base );
+                T v[MB][EPT];
+                /* source tile S, rows r fastest: ADD S = Du(rows b, cols a), else D(rows a, cols b) */
+#pragma unroll
+                for ( int tl = 0; tl < MB; tl++ ) {
+                    int32_t a = 0, b = 0;
+                    if ( tl < nb ) pair( base + tl, a, b );
+                    const T *src = ADD ? Du : D;
+#pragma unroll
+                    for ( int u = 0; u < EPT; u++ ) {
+                        const int32_t e = t + u * TH, r = e & 31, c = e >> 5;
+                        const int32_t sr = ( ADD ? b : a ) * 32 + r, sc = ( ADD ? a : b ) * 32 + c;
+                        v[tl][u] = ( tl < nb && sr < nc && sc < nc ) ? src[sr + (int64_t)sc * nc] : T( 0 );
+                    }
+                }
+#pragma unroll
+                for ( int tl = 0; tl < MB; tl++ )
+#pragma unroll
+                    for ( int u = 0; u < EPT; u++ ) {
+                        const int32_t e = t + u * TH, r = e & 31, c = e >> 5;
+                        if ( tl < nb ) buf[tl * 1056 + r * 33 + c] = v[tl][u];
+                    }
+                __syncthreads();
+                /* destination, rows r fastest: ADD D(a*32 + r, b*32 + c) += S(c, r), else Du(b*32 + r, a*32 + c) = S(c, r) */
+#pragma unroll
+                for ( int tl = 0; tl < MB; tl++ ) {
+                    int32_t a = 0, b = 0;
+                    if ( tl < nb ) pair( base + tl, a, b );
+#pragma unroll
+                    for ( int u = 0; u < EPT; u++ ) {
+                        const int32_t e = t + u * TH, r = e & 31, c = e >> 5;
+                        const int32_t dr = ( ADD ? a : b ) * 32 + r, dc = ( ADD ? b : a ) * 32 + c;
+                        if ( tl < nb ) {
+                            const T x = buf[tl * 1056 + c * 33 + r];
+                            if ( ADD ) {
+                                if ( dr < dc && dc < nc ) D[dr + (int64_t)dc * nc] += x;
+                            }
+                            else if ( dr < nc && dc < nc ) Du[dr + (int64_t)dc * nc] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            return;
+        }
+    }
+    for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
+        const int32_t i = idx % nc, j = idx / nc;
+        if ( ADD ) {
+            if ( i < j ) D[i + (int64_t)j * nc] += Du[j + (int64_t)i * nc];
+        }
+        else Du[j + (int64_t)i * nc] = D[i + (int64_t)j * nc];
+    }
+    __syncthreads();
+}
+
 /* TH threads per workgroup: 1024 for wide diagonal blocks, 64 for the
  * small blocks of the lower levels */
 template <typename T, bool LU, int TH>
@@ -213,13 +301,10 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
     const T   tcrit = (T)crit;
 
     if ( LU ) {
-        /* L += U^T on the full square (strict lower of U^T holds the original
-         * strict upper of A, never updated; L holds the rest) */
-        for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
-            int32_t i = idx % nc, j = idx / nc;
-            if ( i < j ) D[i + (int64_t)j * nc] += Du[j + (int64_t)i * nc];
-        }
-        __syncthreads();
+        /* L += U^T on the full square: the strict lower part of U^T holds the
+         * strict upper part of A with the updates of the descendant cblks
+         * (U-side scatter-GEMMs), L holds the rest */
+        pa_lu_transpose<T, TH, true>( D, Du, nc, P );
     }
 
     for ( int32_t p = 0; p < nc; p += NB ) {
@@ -338,16 +423,17 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
         }
         __syncthreads();
 
-        /* (D) trailing update on 16x16 MFMA tiles, 4 tiles in flight per
-         * wave.  The product is formed as (col side) x (row side)^T so that the
-         * 16 lanes of an output row hold 16 consecutive rows of one column. */
+        /* (D) trailing update on 16x16 MFMA tiles, 4 tiles per wave and
+         * round; the trailing-matrix tiles of the next round are loaded while
+         * the current round runs its MFMAs.  The product is formed as (col
+         * side) x (row side)^T so that the 16 lanes of an output row hold 16
+         * consecutive rows of one column. */
         if ( q > 0 ) {
             const int32_t nt = ( q + 15 ) >> 4;
             const int32_t ntiles = LU ? nt * nt : ( nt * ( nt + 1 ) ) / 2;
-            for ( int32_t tile0 = wave * 4; tile0 < ntiles; tile0 += ( TH / 64 ) * 4 ) {
-                acc_t acc[4];
-                T     cold[4][4];
-                int32_t row[4], col[4], tcol[4];
+            T *Dt = D + ( p + w ) + (int64_t)( p + w ) * nc; /* trailing matrix */
+            /* tile -> row-side index of this lane and first column (row = q: none) */
+            auto coords = [&]( int32_t tile0, int32_t ( &row )[4], int32_t ( &tcol )[4] ) {
 #pragma unroll
                 for ( int u = 0; u < 4; u++ ) {
                     const int32_t tile = tile0 + u;
@@ -364,24 +450,44 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                             tc = tile - ( ti * ( ti + 1 ) ) / 2;
                         }
                     }
-                    row[u]  = ( tile < ntiles ) ? ti * 16 + ( lane & 15 ) : q; /* row side index */
-                    col[u]  = tc * 16 + ( lane & 15 );                          /* col side index */
+                    row[u]  = ( tile < ntiles ) ? ti * 16 + ( lane & 15 ) : q;
                     tcol[u] = tc * 16;
+                }
+            };
+            auto load = [&]( const int32_t ( &row )[4], const int32_t ( &tcol )[4], T ( &cold )[4][4] ) {
+#pragma unroll
+                for ( int u = 0; u < 4; u++ )
 #pragma unroll
                     for ( int r = 0; r < 4; r++ ) {
-                        int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
-                        cold[u][r] = ( row[u] < q && c < q ) ? D[( p + w + row[u] ) + (int64_t)( p + w + c ) * nc] : T( 0 );
-                        acc[u][r]  = T( 0 );
+                        const int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
+                        cold[u][r] = ( row[u] < q && c < q ) ? Dt[row[u] + (int64_t)c * nc] : T( 0 );
                     }
-                }
-                for ( int32_t k0 = 0; k0 < w; k0 += 4 ) {
+            };
+            int32_t row[4], tcol[4];
+            T       cold[4][4];
+            int32_t tile0 = wave * 4;
+            coords( tile0, row, tcol );
+            load( row, tcol, cold );
+            for ( ; tile0 < ntiles; tile0 += ( TH / 64 ) * 4 ) {
+                int32_t rown[4], tcoln[4];
+                T       coldn[4][4];
+                coords( tile0 + ( TH / 64 ) * 4, rown, tcoln );
+                load( rown, tcoln, coldn );
+                acc_t acc[4];
+#pragma unroll
+                for ( int u = 0; u < 4; u++ )
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ ) acc[u][r] = T( 0 );
+#pragma unroll
+                for ( int32_t k0 = 0; k0 < NB; k0 += 4 ) {
                     const int32_t kk = k0 + ( lane >> 4 );
 #pragma unroll
                     for ( int u = 0; u < 4; u++ ) {
-                        T ra = ( row[u] < q && kk < w ) ? P[kk * m + w + row[u]] : T( 0 );
+                        const int32_t col = tcol[u] + ( lane & 15 );
+                        T ra = ( row[u] < q ) ? P[kk * m + w + row[u]] : T( 0 );
                         T ca;
-                        if ( LU ) ca = ( col[u] < q && kk < w ) ? P[(int64_t)NB * nc + kk * q + col[u]] : T( 0 );
-                        else      ca = ( col[u] < q && kk < w ) ? P[kk * m + w + col[u]] : T( 0 );
+                        if ( LU ) ca = ( col < q ) ? P[(int64_t)NB * nc + kk * q + col] : T( 0 );
+                        else      ca = ( col < q ) ? P[kk * m + w + col] : T( 0 );
                         acc[u] = pa_mfma<T>::run( ca, ra, acc[u] );
                     }
                 }
@@ -389,20 +495,22 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                 for ( int u = 0; u < 4; u++ )
 #pragma unroll
                     for ( int r = 0; r < 4; r++ ) {
-                        int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
-                        if ( row[u] < q && c < q ) D[( p + w + row[u] ) + (int64_t)( p + w + c ) * nc] = cold[u][r] - acc[u][r];
+                        const int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
+                        if ( row[u] < q && c < q ) Dt[row[u] + (int64_t)c * nc] = cold[u][r] - acc[u][r];
                     }
+#pragma unroll
+                for ( int u = 0; u < 4; u++ ) {
+                    row[u]  = rown[u];
+                    tcol[u] = tcoln[u];
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ ) cold[u][r] = coldn[u][r];
+                }
             }
         }
         __syncthreads();
     }
 
-    if ( LU ) {
-        for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
-            int32_t i = idx % nc, j = idx / nc;
-            Du[j + (int64_t)i * nc] = D[i + (int64_t)j * nc];
-        }
-    }
+    if ( LU ) pa_lu_transpose<T, TH, false>( D, Du, nc, P ); /* U^T copy of the factored block */
     if ( t == 0 ) {
         if ( mypiv ) atomicAdd( npiv, mypiv );
         if ( myneg ) atomicAdd( nneg, myneg );
