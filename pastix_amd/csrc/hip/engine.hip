@@ -213,10 +213,9 @@ pa_set_lds_limits( int bytes )
     pa_lds_attr( k_solve_diag<double, true, false>, bytes );  pa_lds_attr( k_solve_diag<double, true, true>, bytes );
     pa_lds_attr( k_solve_diag<float, false, false>, bytes );  pa_lds_attr( k_solve_diag<float, false, true>, bytes );
     pa_lds_attr( k_solve_diag<float, true, false>, bytes );   pa_lds_attr( k_solve_diag<float, true, true>, bytes );
-    pa_lds_attr( k_diag<double, true, 512>, bytes ); pa_lds_attr( k_diag<double, false, 512>, bytes );
     pa_lds_attr( k_diag<double, true, 64>, bytes );  pa_lds_attr( k_diag<double, false, 64>, bytes );
-    pa_lds_attr( k_diag<float, true, 512>, bytes );  pa_lds_attr( k_diag<float, false, 512>, bytes );
-    pa_lds_attr( k_diag<float, true, 1024>, bytes );
+    pa_lds_attr( k_diag<float, true, 1024>, bytes );  pa_lds_attr( k_diag<float, false, 1024>, bytes );
+    pa_lds_attr( k_diag<double, true, 1024>, bytes ); pa_lds_attr( k_diag<double, false, 1024>, bytes );
     pa_lds_attr( k_diag<float, true, 64>, bytes );   pa_lds_attr( k_diag<float, false, 64>, bytes );
 }
 
@@ -956,12 +955,10 @@ pa_factorize_t( pa_device_s *dev )
             k_diag_potrf_reg<<<ncb, 1024, 0, dev->stream>>>( dev->S, cb, (float *)L, crit, c0, c1 );
         }
         else if ( lev.ncmax_fac > 64 ) {
-            /* FP32 LU fits 1024 threads without spilling: twice the waves for
-             * the trailing updates */
-            if ( lu && sizeof( T ) == 4 )
-                k_diag<T, true, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else if ( lu ) k_diag<T, true, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else      k_diag<T, false, 512><<<ncb, 512, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            /* 1024 threads fit without spilling (FP64 with 2 trailing tiles
+             * per wave and round): the waves of the trailing updates */
+            if ( lu ) k_diag<T, true, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else      k_diag<T, false, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
         }
         else {
             if ( lu ) k_diag<T, true, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );

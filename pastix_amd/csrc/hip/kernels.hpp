@@ -423,19 +423,21 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
         }
         __syncthreads();
 
-        /* (D) trailing update on 16x16 MFMA tiles, 4 tiles per wave and
-         * round; the trailing-matrix tiles of the next round are loaded while
-         * the current round runs its MFMAs.  The product is formed as (col
+        /* (D) trailing update on 16x16 MFMA tiles, NU tiles per wave and
+         * round (2 for FP64 with 1024 threads: the register budget); the
+         * trailing-matrix tiles of the next round are loaded while the
+         * current round runs its MFMAs.  The product is formed as (col
          * side) x (row side)^T so that the 16 lanes of an output row hold 16
          * consecutive rows of one column. */
         if ( q > 0 ) {
+            constexpr int NU = ( sizeof( T ) == 8 && TH > 512 ) ? 2 : 4;
             const int32_t nt = ( q + 15 ) >> 4;
             const int32_t ntiles = LU ? nt * nt : ( nt * ( nt + 1 ) ) / 2;
             T *Dt = D + ( p + w ) + (int64_t)( p + w ) * nc; /* trailing matrix */
             /* tile -> row-side index of this lane and first column (row = q: none) */
-            auto coords = [&]( int32_t tile0, int32_t ( &row )[4], int32_t ( &tcol )[4] ) {
+            auto coords = [&]( int32_t tile0, int32_t ( &row )[NU], int32_t ( &tcol )[NU] ) {
 #pragma unroll
-                for ( int u = 0; u < 4; u++ ) {
+                for ( int u = 0; u < NU; u++ ) {
                     const int32_t tile = tile0 + u;
                     int32_t ti = 0, tc = 0;
                     if ( tile < ntiles ) {
@@ -454,35 +456,35 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     tcol[u] = tc * 16;
                 }
             };
-            auto load = [&]( const int32_t ( &row )[4], const int32_t ( &tcol )[4], T ( &cold )[4][4] ) {
+            auto load = [&]( const int32_t ( &row )[NU], const int32_t ( &tcol )[NU], T ( &cold )[NU][4] ) {
 #pragma unroll
-                for ( int u = 0; u < 4; u++ )
+                for ( int u = 0; u < NU; u++ )
 #pragma unroll
                     for ( int r = 0; r < 4; r++ ) {
                         const int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
                         cold[u][r] = ( row[u] < q && c < q ) ? Dt[row[u] + (int64_t)c * nc] : T( 0 );
                     }
             };
-            int32_t row[4], tcol[4];
-            T       cold[4][4];
-            int32_t tile0 = wave * 4;
+            int32_t row[NU], tcol[NU];
+            T       cold[NU][4];
+            int32_t tile0 = wave * NU;
             coords( tile0, row, tcol );
             load( row, tcol, cold );
-            for ( ; tile0 < ntiles; tile0 += ( TH / 64 ) * 4 ) {
-                int32_t rown[4], tcoln[4];
-                T       coldn[4][4];
-                coords( tile0 + ( TH / 64 ) * 4, rown, tcoln );
+            for ( ; tile0 < ntiles; tile0 += ( TH / 64 ) * NU ) {
+                int32_t rown[NU], tcoln[NU];
+                T       coldn[NU][4];
+                coords( tile0 + ( TH / 64 ) * NU, rown, tcoln );
                 load( rown, tcoln, coldn );
-                acc_t acc[4];
+                acc_t acc[NU];
 #pragma unroll
-                for ( int u = 0; u < 4; u++ )
+                for ( int u = 0; u < NU; u++ )
 #pragma unroll
                     for ( int r = 0; r < 4; r++ ) acc[u][r] = T( 0 );
 #pragma unroll
                 for ( int32_t k0 = 0; k0 < NB; k0 += 4 ) {
                     const int32_t kk = k0 + ( lane >> 4 );
 #pragma unroll
-                    for ( int u = 0; u < 4; u++ ) {
+                    for ( int u = 0; u < NU; u++ ) {
                         const int32_t col = tcol[u] + ( lane & 15 );
                         T ra = ( row[u] < q ) ? P[kk * m + w + row[u]] : T( 0 );
                         T ca;
@@ -492,14 +494,14 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     }
                 }
 #pragma unroll
-                for ( int u = 0; u < 4; u++ )
+                for ( int u = 0; u < NU; u++ )
 #pragma unroll
                     for ( int r = 0; r < 4; r++ ) {
                         const int32_t c = tcol[u] + pa_mfma<T>::row( lane, r );
                         if ( row[u] < q && c < q ) Dt[row[u] + (int64_t)c * nc] = cold[u][r] - acc[u][r];
                     }
 #pragma unroll
-                for ( int u = 0; u < 4; u++ ) {
+                for ( int u = 0; u < NU; u++ ) {
                     row[u]  = rown[u];
                     tcol[u] = tcoln[u];
 #pragma unroll
