@@ -1441,24 +1441,52 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     const int64_t aof = ald > 0 ? a_off[arow] : 0, bof = bld > 0 ? b_off[brow] : 0;
     const int32_t alds = ald > 0 ? ald : 1, blds = bld > 0 ? bld : 1;
     T pa[NQA], pb[NQB];
-
+    /* Full K chunks read through per-thread column pointers advanced once per
+     * chunk (no clamp, index product or K mask per load); the one partial
+     * chunk at the end of K, if any, uses clamped in-range addresses and is
+     * masked when stored. */
+    const int32_t nfull = K / PA_GKC;
+    const T *pA[NQA], *pB[NQB];
 #pragma unroll
-    for ( int q = 0; q < NQA; q++ ) pa[q] = Asrc[aof + (int64_t)min( ak + KLA * q, K - 1 ) * alds];
+    for ( int q = 0; q < NQA; q++ ) pA[q] = Asrc + aof + (int64_t)( ak + KLA * q ) * alds;
 #pragma unroll
-    for ( int q = 0; q < NQB; q++ ) pb[q] = Bsrc[bof + (int64_t)min( bk + KLB * q, K - 1 ) * blds];
-    for ( int32_t k0 = 0; k0 < K; k0 += PA_GKC ) {
+    for ( int q = 0; q < NQB; q++ ) pB[q] = Bsrc + bof + (int64_t)( bk + KLB * q ) * blds;
+    const int64_t ca = (int64_t)PA_GKC * alds, cb = (int64_t)PA_GKC * blds;
+    auto load_full = [&]() {
 #pragma unroll
-        for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = ( ald > 0 && k0 + ak + KLA * q < K ) ? pa[q] : T( 0 );
+        for ( int q = 0; q < NQA; q++ ) pa[q] = *pA[q];
 #pragma unroll
-        for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = ( bld > 0 && k0 + bk + KLB * q < K ) ? pb[q] : T( 0 );
+        for ( int q = 0; q < NQB; q++ ) pb[q] = *pB[q];
+    };
+    auto load_tail = [&]( int32_t k0 ) {
+#pragma unroll
+        for ( int q = 0; q < NQA; q++ ) pa[q] = Asrc[aof + (int64_t)min( k0 + ak + KLA * q, K - 1 ) * alds];
+#pragma unroll
+        for ( int q = 0; q < NQB; q++ ) pb[q] = Bsrc[bof + (int64_t)min( k0 + bk + KLB * q, K - 1 ) * blds];
+    };
+    if ( nfull > 0 ) load_full();
+    else             load_tail( 0 );
+    for ( int32_t c = 0, k0 = 0; k0 < K; c++, k0 += PA_GKC ) {
+        if ( c < nfull ) {
+#pragma unroll
+            for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = ( ald > 0 ) ? pa[q] : T( 0 );
+#pragma unroll
+            for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = ( bld > 0 ) ? pb[q] : T( 0 );
+        }
+        else {
+#pragma unroll
+            for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = ( ald > 0 && k0 + ak + KLA * q < K ) ? pa[q] : T( 0 );
+#pragma unroll
+            for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = ( bld > 0 && k0 + bk + KLB * q < K ) ? pb[q] : T( 0 );
+        }
         __syncthreads();
         if ( k0 + PA_GKC < K ) {
 #pragma unroll
-            for ( int q = 0; q < NQA; q++ )
-                pa[q] = Asrc[aof + (int64_t)min( k0 + PA_GKC + ak + KLA * q, K - 1 ) * alds];
+            for ( int q = 0; q < NQA; q++ ) pA[q] += ca;
 #pragma unroll
-            for ( int q = 0; q < NQB; q++ )
-                pb[q] = Bsrc[bof + (int64_t)min( k0 + PA_GKC + bk + KLB * q, K - 1 ) * blds];
+            for ( int q = 0; q < NQB; q++ ) pB[q] += cb;
+            if ( c + 1 < nfull ) load_full();
+            else                 load_tail( k0 + PA_GKC );
         }
 #pragma unroll
         for ( int ks = 0; ks < PA_GKC / 4; ks++ ) {
