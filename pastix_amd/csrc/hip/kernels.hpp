@@ -1437,7 +1437,7 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
 
     const int arow = t % TM, ak = t / TM, brow = t % TN, bk = t / TN;
     const int32_t ald = a_ld[arow], bld = b_ld[brow];
-    /* an empty row reads row 0 of its operand (in range) and stores zeros */
+    /* an empty row reads row 0 of its operand (in range) */
     const int64_t aof = ald > 0 ? a_off[arow] : 0, bof = bld > 0 ? b_off[brow] : 0;
     const int32_t alds = ald > 0 ? ald : 1, blds = bld > 0 ? bld : 1;
     T pa[NQA], pb[NQB];
@@ -1468,10 +1468,13 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     else             load_tail( 0 );
     for ( int32_t c = 0, k0 = 0; k0 < K; c++, k0 += PA_GKC ) {
         if ( c < nfull ) {
+            /* an empty row (ld 0) holds finite values of row 0 of its
+             * operand here: its products land in output rows / columns the
+             * epilogue skips, so no mask is needed */
 #pragma unroll
-            for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = ( ald > 0 ) ? pa[q] : T( 0 );
+            for ( int q = 0; q < NQA; q++ ) As[ak + KLA * q][arow] = pa[q];
 #pragma unroll
-            for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = ( bld > 0 ) ? pb[q] : T( 0 );
+            for ( int q = 0; q < NQB; q++ ) Bs[bk + KLB * q][brow] = pb[q];
         }
         else {
 #pragma unroll
