@@ -614,6 +614,22 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
         }
     }
 
+    /* (4) store of a finished panel (lower part of its diagonal tile) by
+     * threads t0..: panel p is stored while wave 0 factors the diagonal tile
+     * of panel p + 1 (its LDS buffer is rewritten only by panel p + 2) */
+    auto store_panel = [&]( int32_t pp, int32_t tt ) {
+        const float( *Ps )[PA_DREG_LD] = P[pp & 1];
+        const int32_t ms = nc - 16 * pp, ws = min( 16, ms );
+        if ( tt >= 0 && tt < ms ) {
+            float v[16];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ ) v[c] = Ps[tt][c];
+#pragma unroll
+            for ( int c = 0; c < 16; c++ )
+                if ( c < ws && tt >= c ) D[( 16 * pp + tt ) + (int64_t)( 16 * pp + c ) * nc] = v[c];
+        }
+    };
+
     for ( int32_t p = 0; p < nt; p++ ) {
         float( *Pp )[PA_DREG_LD] = P[p & 1];
         const int32_t w = min( 16, nc - 16 * p ); /* columns of the panel */
@@ -670,6 +686,7 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
                     if ( j <= i ) Lt[j][i] = ( j == i ) ? rinv : a[j];
             }
         }
+        else if ( p > 0 ) store_panel( p - 1, t - 64 );
         __syncthreads();
         /* (3) panel rows below the diagonal tile: X L^T = A by substitution,
          * column by column (x_j -= x_c l_jc in increasing c, the order of
@@ -698,15 +715,6 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
             for ( int c = 0; c < 16; c++ ) Pp[16 + t][c] = x[c];
         }
         __syncthreads();
-        /* (4) store the finished panel (lower part of the diagonal tile) */
-        if ( t < m ) {
-            float v[16];
-#pragma unroll
-            for ( int c = 0; c < 16; c++ ) v[c] = Pp[t][c];
-#pragma unroll
-            for ( int c = 0; c < 16; c++ )
-                if ( c < w && t >= c ) D[( 16 * p + t ) + (int64_t)( 16 * p + c ) * nc] = v[c];
-        }
         /* (5) trailing update of the owned tiles right of the panel:
          * C(i, j)^T -= L(j, p) L(i, p)^T */
 #pragma unroll
@@ -725,6 +733,8 @@ pa_diag_potrf_reg( float *D, int32_t nc, double crit, int32_t *npiv, int32_t *nn
             __builtin_amdgcn_sched_barrier( 0 );
         }
     }
+    __syncthreads();
+    store_panel( nt - 1, t );
     if ( t == 0 ) {
         if ( mypiv ) atomicAdd( npiv, mypiv );
         if ( myneg ) atomicAdd( nneg, myneg );
