@@ -2125,17 +2125,26 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     }
     const int col = lane >> 2;
     if ( !MR ) { /* one right-hand side */
+        /* every load of the wave is unconditional, from an in-range address
+         * (an empty row reads x[0] and the panel's first entries; a column
+         * past the cblk repeats its last one), and the contributions are
+         * masked: the loads of the four row groups are all in flight at once */
+        T xv[4];
+#pragma unroll
+        for ( int q = 0; q < 4; q++ ) {
+            const T v = x[ok[q] ? ri[q].x : 0];
+            xv[q] = ok[q] ? v : T( 0 );
+        }
         T v[16];
 #pragma unroll
         for ( int cc = 0; cc < 16; cc++ ) v[cc] = T( 0 );
 #pragma unroll
         for ( int q = 0; q < 4; q++ ) {
-            if ( ok[q] ) {
-                const T  xv = x[ri[q].x];
-                const T *A  = Coef + pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y;
+            const T *A = Coef + ( ok[q] ? pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y : 0 );
 #pragma unroll
-                for ( int cc = 0; cc < 16; cc++ )
-                    if ( cc < ncols ) v[cc] += A[(int64_t)cc * ri[q].y] * xv;
+            for ( int cc = 0; cc < 16; cc++ ) {
+                const T a = A[(int64_t)min( cc, max( ncols - 1, 0 ) ) * ri[q].y];
+                v[cc] += ( cc < ncols ? a : T( 0 ) ) * xv[q];
             }
         }
         T s = pa_wave_sum16<T>( v, lane );
