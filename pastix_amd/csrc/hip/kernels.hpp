@@ -157,7 +157,8 @@ k_fill( pa_dstruct S, int32_t n, const int64_t *colptr, const int32_t *rowtab,
 }
 
 /* ------------------------------------------------------------------------- */
-/* Diagonal block factorization, one workgroup (4 waves) per cblk.            */
+/* Diagonal block factorization, one workgroup per cblk (1024 threads above  */
+/* 64 columns, 64 below).                                                    */
 /*                                                                           */
 /* POTRF: Cholesky with static pivoting: a pivot below `crit` in magnitude   */
 /* is replaced by `crit` and counted (reference kernels/core_zpotrfsp.c:     */
@@ -166,11 +167,11 @@ k_fill( pa_dstruct S, int32_t n, const int64_t *colptr, const int32_t *rowtab,
 /* diagonal block of U^T is first merged into L and the factored block is    */
 /* transposed back into U^T (core_zgeadd / core_zgetmo, :253-264).           */
 /*                                                                           */
-/* Right-looking over panels of NB columns (16 in FP64, 32 in FP32) held in  */
-/* LDS: the NB x NB top block is factored by one wave in registers with      */
-/* cross-lane shuffles (no workgroup barrier per column), the rows below are */
-/* solved by all threads, and the trailing update of the (L2-resident)       */
-/* diagonal tile runs on MFMA 16x16x4 tiles.                                 */
+/* Right-looking over panels of NB = 16 columns held in LDS: the NB x NB    */
+/* top block is factored by one wave in registers with v_readlane broadcasts */
+/* (no workgroup barrier per column, no branches), the rows below are solved */
+/* by all threads against the pivots' reciprocals, and the trailing update   */
+/* of the (L2-resident) diagonal block runs on MFMA 16x16x4 tiles.           */
 /* ------------------------------------------------------------------------- */
 template <typename T> struct pa_diag_nb;
 template <> struct pa_diag_nb<double> { static constexpr int value = 16; };
@@ -572,9 +573,10 @@ k_diag_one( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg )
 /* Per 16-column panel p: the owners stage tile column p in LDS (double      */
 /* buffered), wave 0 factors the 16 x 16 diagonal tile with a v_readlane     */
 /* chain (static pivoting as the reference kernel), every thread solves one  */
-/* panel row against it by substitution, the finished panel is stored and    */
-/* the owners apply it to their trailing tiles with MFMA 16x16x4.  Three     */
-/* barriers per panel and no global round trip inside the sweep.            */
+/* panel row against it by substitution, the owners apply the panel to their */
+/* trailing tiles with MFMA 16x16x4, and the idle waves store the panel      */
+/* while wave 0 factors the next diagonal tile.  Three barriers per panel    */
+/* and no global round trip inside the sweep.                               */
 /* ------------------------------------------------------------------------- */
 #define PA_DREG_NT 22 /* 352 columns */
 #define PA_DREG_S  16 /* slots per wave: 16 * 16 >= 22 * 23 / 2 tiles */
@@ -941,7 +943,9 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
 /*   MODE 1 (LU, L):  X <- X U^{-1}, U upper non-unit part of the L tile     */
 /*   MODE 2 (LU, U):  X <- X V^{-1}, V upper unit part of the U^T tile       */
 /* A task is (cblk, first below-diagonal row); TR rows per workgroup are     */
-/* staged in LDS and solved with a column-blocked algorithm.                 */
+/* staged in LDS and solved with a column-blocked algorithm.  The level      */
+/* factorization uses it above 384 columns only (and the reference-signature */
+/* gpublok_*trsmsp calls); below, k_trsm_inv replaces it.                    */
 /* ------------------------------------------------------------------------- */
 /* rows per workgroup (the slab lives in LDS) */
 #ifndef PA_TRSM_TR
@@ -957,7 +961,7 @@ template <typename T> struct pa_trsm_rows_per_wg { static constexpr int value = 
  *                                                     waves when SR < 64)
  *   X[:, jb:jb+16] <- X[:, jb:jb+16] C[jb:jb+16, jb:jb+16]^{-1} (one row per thread)
  * where C(l, j) = L[j][l] (MODE 0) or D[l][j] (MODE 1, 2); the coefficients of
- * the block column are staged in LDS first.  LDS: SR*nc + 16*nc + 4*SR*16 T.
+ * the block column are staged in LDS first.  LDS: pa_trsm_lds (padded strides).
  */
 #define PA_TRSM_U 8
 /* LDS strides of pa_trsm_rows, padded so that the MFMA operand reads and the
