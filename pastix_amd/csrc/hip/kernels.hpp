@@ -824,11 +824,13 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
             mypiv += rep ? 1 : 0;
             myneg += ( d < T( 0 ) ) ? 1 : 0;
             d = sqrt( d );
+            /* column scaled by the pivot's reciprocal (the reference kernel's
+             * order); selects instead of per-lane branches */
+            const T alpha = T( 1 ) / d;
 #pragma unroll
             for ( int q = 0; q < NS; q++ ) {
                 const int32_t r = lane + 64 * q;
-                if ( r == j ) P[q][j] = d;
-                else if ( r > j ) P[q][j] = P[q][j] / d;
+                P[q][j] = ( r == j ) ? d : ( ( r > j ) ? P[q][j] * alpha : P[q][j] );
             }
 #pragma unroll
             for ( int c = 1; c < PA_SM_NC; c++ ) {
@@ -836,7 +838,7 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
                     const T f = pa_readlane( P[0][j], c ); /* L[c][j] */
 #pragma unroll
                     for ( int q = 0; q < NS; q++ )
-                        if ( lane + 64 * q > j ) P[q][c] -= P[q][j] * f;
+                        P[q][c] = ( lane + 64 * q > j ) ? P[q][c] - P[q][j] * f : P[q][c];
                 }
             }
         }
