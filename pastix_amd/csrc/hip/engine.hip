@@ -974,16 +974,15 @@ pa_factorize_t( pa_device_s *dev )
             const int2   *tt   = dev->d_tinv_tasks + lev.tinv_beg;
             const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
             T            *Ti   = (T *)dev->d_tinv;
-            T            *Ti2  = Ti + dev->tinv_tiles * 256;
+            const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
             if ( lu ) {
-                k_tile_inv<T, 1><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti );
-                k_tile_inv<T, 2><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti2 );
-                k_trsm_inv<T, 1><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti );
-                k_trsm_inv<T, 2><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti2 );
+                /* both LU modes (L and U^T panels) in one launch each */
+                k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
+                k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, ti2 );
             }
             else {
-                k_tile_inv<T, 0><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti );
-                k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti );
+                k_tile_inv<T, 0><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+                k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, 0 );
             }
             PA_HIP( hipGetLastError() );
         }
@@ -1060,7 +1059,7 @@ pa_collect_profile( pa_device_s *dev )
                      lev.flops[2] / ( t[2] * 1e9 ), lev.flops[1] / ( t[1] * 1e9 ) );
         }
         dev->prof_launch[0] += 1;
-        dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lu_launches( dev ) ) : 0;
+        dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lev.trsm_inv ? 1 : lu_launches( dev ) ) : 0;
         for ( int c = 0; c < PA_NGC; c++ ) dev->prof_launch[2] += ( lev.gc_end[c] > lev.gc_beg[c] ) ? 1 : 0;
     }
     return PASTIX_SUCCESS;

@@ -1150,8 +1150,12 @@ k_trsm( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc )
 
 template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
-k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T *Uc, T *tinv )
+k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T *Uc, T *tinv, int64_t tinv2 )
 {
+    /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y; mode 2's
+     * inverses start tinv2 elements further */
+    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : MODE;
+    if ( md == 2 ) tinv += tinv2;
     __shared__ __align__( 16 ) T sT[4][16][16]; /* per wave: sT[c][l] = T(l, c) of the tile */
     const int     lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t task  = blockIdx.x * 4 + wave;
@@ -1164,13 +1168,13 @@ k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T
         nc = S.c_ncol[k];
     }
     const int32_t c0 = 16 * tl, w = min( 16, nc - c0 );
-    const T      *Q  = ( MODE == 1 ? Uc : Lc ) + S.c_coef[k];
+    const T      *Q  = ( md == 1 ? Uc : Lc ) + S.c_coef[k];
 #pragma unroll
     for ( int u = 0; u < 4; u++ ) {
         const int idx = lane + 64 * u, c = idx >> 4, l = idx & 15;
         /* columns past the cblk: identity */
         T v = ( c == l ) ? T( 1 ) : T( 0 );
-        if ( valid && c < w && l <= c && !( MODE == 2 && l == c ) ) v = Q[( c0 + c ) + (int64_t)( c0 + l ) * nc];
+        if ( valid && c < w && l <= c && !( md == 2 && l == c ) ) v = Q[( c0 + c ) + (int64_t)( c0 + l ) * nc];
         sT[wave][c][l] = v;
     }
     __syncthreads();
@@ -1184,7 +1188,7 @@ k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T
 #pragma unroll
             for ( int l = 0; l < 16; l++ )
                 if ( l < c ) v -= m[l] * sT[wave][c][l];
-            m[c] = ( MODE == 2 ) ? v : v / sT[wave][c][c];
+            m[c] = ( md == 2 ) ? v : v / sT[wave][c][c];
         }
         T *out = tinv + ( S.c_tinv[k] + tl ) * 256 + r * 16;
 #pragma unroll
@@ -1194,8 +1198,11 @@ k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T
 
 template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
-k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv )
+k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_t tinv2 )
 {
+    /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y */
+    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : MODE;
+    if ( md == 2 ) tinv += tinv2;
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int SR  = PA_TRSM_TR;
     constexpr int NRT = SR / 16;
@@ -1211,8 +1218,8 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv )
     const int32_t nc = S.c_ncol[k], R = S.c_stride[k] - nc;
     const int32_t rows = R - r0 < SR ? R - r0 : SR;
     const int32_t nt = ( nc + 15 ) >> 4;
-    T            *panel = ( MODE == 2 ? Uc : Lc ) + S.c_coef[k];
-    const T      *Q     = ( MODE == 1 ? Uc : Lc ) + S.c_coef[k];
+    T            *panel = ( md == 2 ? Uc : Lc ) + S.c_coef[k];
+    const T      *Q     = ( md == 1 ? Uc : Lc ) + S.c_coef[k];
     const T      *Mi    = tinv + S.c_tinv[k] * 256;
 
     if ( t < SR ) {
