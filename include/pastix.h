@@ -56,11 +56,24 @@ int pastix_subtask_spm2bcsc( pastix_data_t *pastix_data, spmatrix_t *spm );
 int pastix_subtask_bcsc2ctab( pastix_data_t *pastix_data );
 int pastix_subtask_sopalin( pastix_data_t *pastix_data );
 
+/* Solve and refinement subtasks (reference include/pastix.h:133-152).  The
+ * right-hand sides live in host memory, in the matrix's arithmetic; each
+ * subtask copies them to the device and back. */
+int pastix_subtask_applyorder( pastix_data_t *pastix_data, pastix_dir_t dir, pastix_int_t m,
+                               pastix_int_t n, void *B, pastix_int_t ldb, pastix_rhs_t Bp );
+int pastix_subtask_trsm( pastix_data_t *pastix_data, pastix_side_t side, pastix_uplo_t uplo,
+                         pastix_trans_t trans, pastix_diag_t diag, pastix_rhs_t b );
+int pastix_subtask_diag( pastix_data_t *pastix_data, pastix_rhs_t b );
+int pastix_subtask_solve( pastix_data_t *pastix_data, pastix_rhs_t b );
+int pastix_subtask_refine( pastix_data_t *pastix_data, pastix_rhs_t b, pastix_rhs_t x );
 int pastix_subtask_solve_adv( pastix_data_t *pastix_data, pastix_trans_t transA,
                               pastix_rhs_t b );
 
+/* Right-hand side handles (reference include/pastix.h:172-182) */
 int pastixRhsInit( pastix_rhs_t *rhs );
 int pastixRhsFinalize( pastix_rhs_t rhs );
+int pastixRhsDoubletoSingle( const pastix_rhs_t dB, pastix_rhs_t sB );
+int pastixRhsSingleToDouble( const pastix_rhs_t sB, pastix_rhs_t dB );
 
 int  pastixCheckParam( const pastix_int_t *iparm, const double *dparm );
 void pastixDumpParam( const pastix_data_t *pastix_data );
@@ -85,6 +98,11 @@ int pastix_amd_solve_dev( pastix_data_t *pastix_data, pastix_trans_t transA,
  */
 int pastix_amd_refine_dev( pastix_data_t *pastix_data, pastix_int_t nrhs,
                            const void *dB, pastix_int_t ldb, void *dX, pastix_int_t ldx );
+/* Same with op(A) given explicitly (PastixNoTrans or PastixTrans) instead of
+ * IPARM_TRANSPOSE_SOLVE: the refined adjoint solve A^T x = b of the
+ * differentiable wrapper. */
+int pastix_amd_refine_dev_adv( pastix_data_t *pastix_data, pastix_trans_t transA, pastix_int_t nrhs,
+                               const void *dB, pastix_int_t ldb, void *dX, pastix_int_t ldx );
 
 /* y = alpha*op(A)*x + beta*y with the user's matrix, x/y device FP64 vectors
  * in the user numbering. */

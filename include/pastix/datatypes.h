@@ -21,8 +21,12 @@ typedef uint64_t pastix_uint_t;
 
 typedef double pastix_fixdbl_t;
 
-/* No MPI in this build: a communicator is a plain integer handle. */
-typedef int PASTIX_Comm;
+/* No MPI in this build: a communicator is an opaque handle, as in the
+ * reference's build without MPI (reference include/pastix.h:36-44). */
+typedef uintptr_t PASTIX_Comm;
+#ifndef MPI_COMM_WORLD
+#define MPI_COMM_WORLD 0
+#endif
 
 struct pastix_data_s;
 typedef struct pastix_data_s pastix_data_t;

@@ -13,6 +13,11 @@
  *                                            reference kernels/core_zpotrfsp.c:68-190
  *   - diagonal GETRF, 64-blocked, static pivoting, geadd/getmo
  *                                            reference kernels/core_zgetrfsp.c:65-272
+ *   - sequential LDL^t driver, diagonal SYTRF (64-blocked, static
+ *     pivoting), panel TRSM against D L^t and the scaled copy L D
+ *                                            reference sopalin/sequential_zsytrf.c,
+ *                                            kernels/core_zsytrfsp.c:28-380,
+ *                                            kernels/core_zscalo.c
  *   - panel TRSM on 2D tiles                 reference kernels/core_ztrsmsp.c:149-180
  *   - 2D scatter GEMM                        reference kernels/core_zgemmsp.c:404-481
  *   - supernodal forward/backward solve      reference kernels/solve_ztrsmsp.c:302-661
@@ -291,6 +296,78 @@ getrfsp( oint n, REAL *A, oint lda, oint *npiv, double crit )
     }
 }
 
+/* X(m x n) <- X * L^{-T}, L lower unit (Right, Lower, Trans, Unit) */
+static void
+trsm_rltu( oint m, oint n, const REAL *L, oint lda, REAL *X, oint ldx )
+{
+    oint i, j, l;
+    if ( blas_trsm && m > 0 && n > 0 ) {
+        const int  im = (int)m, in = (int)n, ia = (int)lda, ix = (int)ldx;
+        const REAL one = (REAL)1;
+        blas_trsm( "R", "L", "T", "U", &im, &in, &one, L, &ia, X, &ix );
+        return;
+    }
+    for ( j = 0; j < n; j++ ) {
+        REAL *xj = X + j * ldx;
+        for ( l = 0; l < j; l++ ) {
+            REAL c = L[j + l * lda];
+            const REAL *xl = X + l * ldx;
+            for ( i = 0; i < m; i++ ) xj[i] -= xl[i] * c;
+        }
+    }
+}
+
+/* unblocked LDL^t of an n x n block with static pivoting (reference
+ * kernels/core_zsytrfsp.c:68-120): the pivot keeps its sign when replaced;
+ * the unscaled column is copied into the row (D L^t) before the column is
+ * scaled by 1/d_k, then the trailing lower part takes -d_k l l^T */
+static void
+sytf2sp( oint n, REAL *A, oint lda, oint *npiv, double crit )
+{
+    oint k, i, j;
+    for ( k = 0; k < n; k++ ) {
+        REAL *akk = A + k * ( lda + 1 ), alpha, d;
+        oint  m = n - k - 1;
+        if ( fabs( (double)*akk ) < crit ) {
+            *akk = ( *akk < (REAL)0 ) ? (REAL)( -crit ) : (REAL)crit;
+            ( *npiv )++;
+        }
+        d     = *akk;
+        alpha = (REAL)1 / d;
+        for ( i = 1; i <= m; i++ ) akk[i * lda] = akk[i];
+        for ( i = 1; i <= m; i++ ) akk[i] *= alpha;
+        for ( j = 1; j <= m; j++ ) {
+            REAL t = -d * akk[j];
+            for ( i = j; i <= m; i++ ) akk[i + j * lda] += akk[i] * t;
+        }
+    }
+}
+
+/* reference kernels/core_zsytrfsp.c:150-215: 64-column blocks; below each
+ * factored block, X L_kk^{-T} (unit), then row copy and 1/d scaling column
+ * by column, and the trailing square -= (L D) L^T */
+static void
+sytrfsp( oint n, REAL *A, oint lda, oint *npiv, double crit )
+{
+    oint k, blocknbr = ( n + MAXSIZEOFBLOCKS - 1 ) / MAXSIZEOFBLOCKS;
+    for ( k = 0; k < blocknbr; k++ ) {
+        oint bs = n - k * MAXSIZEOFBLOCKS < MAXSIZEOFBLOCKS ? n - k * MAXSIZEOFBLOCKS : MAXSIZEOFBLOCKS;
+        REAL *Akk = A + ( k * MAXSIZEOFBLOCKS ) * ( lda + 1 );
+        REAL *Amk = Akk + bs, *Akm = Akk + bs * lda, *Amm = Amk + bs * lda;
+        sytf2sp( bs, Akk, lda, npiv, crit );
+        if ( k * MAXSIZEOFBLOCKS + bs < n ) {
+            oint ms = n - ( k * MAXSIZEOFBLOCKS + bs ), col, i;
+            trsm_rltu( ms, bs, Akk, lda, Amk, lda );
+            for ( col = 0; col < bs; col++ ) {
+                REAL alpha = (REAL)1 / Akk[col * ( lda + 1 )];
+                for ( i = 0; i < ms; i++ ) Akm[col + i * lda] = Amk[i + col * lda];
+                for ( i = 0; i < ms; i++ ) Amk[i + col * lda] *= alpha;
+            }
+            gemm_nn_minus( ms, ms, bs, Amk, lda, Akm, lda, Amm, lda );
+        }
+    }
+}
+
 /* ---- supernodal kernels ------------------------------------------------- */
 
 /* 2D scatter GEMM of block `bj` of cblk k into its facing cblk
@@ -460,6 +537,32 @@ PFX( getrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *U
     return 0;
 }
 
+/* sequential LDL^t with static pivoting (reference
+ * sopalin/sequential_zsytrf.c with cpucblk_zsytrfsp1d,
+ * kernels/core_zsytrfsp.c:296-380): diagonal SYTRF, the panel solved against
+ * the D L^t stored in the upper part of the diagonal block, its copy L D
+ * into DLt (cpucblk_zscalo), and the updates C -= L_i (L D)_j^T */
+int
+PFX( sytrf )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *DLt, double crit, oint *npiv )
+{
+    oint k, b, i, j;
+    *npiv = 0;
+    for ( k = 0; k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k ), lb = C_FBLK( k + 1 );
+        REAL *D = L + C_COEF( k );
+        sytrfsp( nc, D, nc, npiv, crit );
+        for ( b = C_FBLK( k ) + 1; b < lb; b++ ) {
+            oint  M = B_NROW( b );
+            REAL *X = L + C_COEF( k ) + B_COEF( b ), *Y = DLt + C_COEF( k ) + B_COEF( b );
+            trsm_run( M, nc, D, nc, 0, X, M );
+            for ( j = 0; j < nc; j++ )
+                for ( i = 0; i < M; i++ ) Y[i + j * M] = X[i + j * M] * D[j + j * nc];
+        }
+        for ( b = C_FBLK( k ) + 1; b < lb; b++ ) gemmsp_2d2d( cblk, blok, k, b, 0, L, DLt, L );
+    }
+    return 0;
+}
+
 /* Level-parallel LU (CPU-baseline leg), the scheme of potrf_par with the
  * LU kernels of the sequential driver. */
 int
@@ -536,7 +639,7 @@ PFX( getrf_par )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REA
  * kernels/solve_ztrsmsp.c:302-661). */
 static void
 solve_sweeps( oint cblknbr, const oint *cblk, const oint *blok, const REAL *fwdC, int unit_f, const REAL *bwdC,
-              int unit_b, oint nrhs, REAL *x, oint ld )
+              int unit_b, const REAL *dscale, oint nrhs, REAL *x, oint ld )
 {
     oint k, b, r, i, j;
     for ( k = 0; k < cblknbr; k++ ) {
@@ -556,6 +659,13 @@ solve_sweeps( oint cblknbr, const oint *cblk, const oint *blok, const REAL *fwdC
                     for ( i = 0; i < M; i++ ) y[i] -= A[i + j * M] * xk[j];
             }
         }
+    }
+    /* D^{-1} between the sweeps (LDL^t, reference solve_cblk_zdiag,
+     * kernels/solve_ztrsmsp.c:664-716): the diagonal of each diagonal block */
+    for ( k = 0; dscale && k < cblknbr; k++ ) {
+        oint nc = C_NCOL( k ), f0 = C_FCOL( k );
+        for ( r = 0; r < nrhs; r++ )
+            for ( j = 0; j < nc; j++ ) x[r * ld + f0 + j] /= dscale[C_COEF( k ) + j * ( nc + 1 )];
     }
     for ( k = cblknbr - 1; k >= 0; k-- ) {
         oint nc = C_NCOL( k ), f0 = C_FCOL( k );
@@ -581,17 +691,21 @@ solve_sweeps( oint cblknbr, const oint *cblk, const oint *blok, const REAL *fwdC
     }
 }
 
-/* factotype: 0 = LL^t, 2 = LU; trans: 0 = A x = b, 1 = A^T x = b */
+/* factotype: 0 (or 3) = LL^t, 1 (or 4) = LDL^t, 2 = LU; trans: 0 = A x = b,
+ * 1 = A^T x = b */
 int
 PFX( solve )( oint cblknbr, const oint *cblk, const oint *blok, int factotype, int trans, const REAL *L,
               const REAL *U, oint nrhs, REAL *x, oint ld )
 {
     if ( factotype == 2 ) {
-        if ( !trans ) solve_sweeps( cblknbr, cblk, blok, L, 1, U, 0, nrhs, x, ld );
-        else          solve_sweeps( cblknbr, cblk, blok, U, 0, L, 1, nrhs, x, ld );
+        if ( !trans ) solve_sweeps( cblknbr, cblk, blok, L, 1, U, 0, NULL, nrhs, x, ld );
+        else          solve_sweeps( cblknbr, cblk, blok, U, 0, L, 1, NULL, nrhs, x, ld );
+    }
+    else if ( factotype == 1 || factotype == 4 ) {
+        solve_sweeps( cblknbr, cblk, blok, L, 1, L, 1, L, nrhs, x, ld );
     }
     else {
-        solve_sweeps( cblknbr, cblk, blok, L, 0, L, 0, nrhs, x, ld );
+        solve_sweeps( cblknbr, cblk, blok, L, 0, L, 0, NULL, nrhs, x, ld );
     }
     return 0;
 }
@@ -625,6 +739,13 @@ PFX( getrf_diag )( oint nc, REAL *D, REAL *Du, double crit, oint *npiv )
     getrfsp( nc, D, nc, npiv, crit );
     for ( j = 0; j < nc; j++ )
         for ( i = 0; i < nc; i++ ) Du[j + i * nc] = D[i + j * nc];
+    return 0;
+}
+int
+PFX( sytrf_diag )( oint nc, REAL *D, double crit, oint *npiv )
+{
+    *npiv = 0;
+    sytrfsp( nc, D, nc, npiv, crit );
     return 0;
 }
 /* Panel TRSM of one cblk (per-call parity of gpublok_*trsmsp):

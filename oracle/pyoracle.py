@@ -29,10 +29,12 @@ def load():
             getattr(_lib, "oracle_%sfill" % pfx).argtypes = [I, P, P, P, P, P, P, P, P]
             getattr(_lib, "oracle_%spotrf" % pfx).argtypes = [I, P, P, P, D, P, P]
             getattr(_lib, "oracle_%sgetrf" % pfx).argtypes = [I, P, P, P, P, D, P]
+            getattr(_lib, "oracle_%ssytrf" % pfx).argtypes = [I, P, P, P, P, D, P]
             getattr(_lib, "oracle_%ssolve" % pfx).argtypes = [I, P, P, C.c_int, C.c_int, P, P, I, P, I]
             getattr(_lib, "oracle_%sgemmsp_one" % pfx).argtypes = [P, P, I, I, C.c_int, P, P, P]
             getattr(_lib, "oracle_%spotrf_diag" % pfx).argtypes = [I, P, D, P, P]
             getattr(_lib, "oracle_%sgetrf_diag" % pfx).argtypes = [I, P, P, D, P]
+            getattr(_lib, "oracle_%ssytrf_diag" % pfx).argtypes = [I, P, D, P]
             getattr(_lib, "oracle_%strsm_cblk" % pfx).argtypes = [P, P, I, C.c_int, P, P]
             getattr(_lib, "oracle_%sset_blas" % pfx).argtypes = [P, P]
             getattr(_lib, "oracle_%spotrf_par" % pfx).argtypes = [I, P, P, P, D, P, P, C.c_int]
@@ -97,18 +99,21 @@ def permuted_csc(A, perm):
 
 
 def factorize(S, A, perm, factotype, crit, dtype=np.float64):
-    """Fill + sequential factorization; returns (L, U or None, npiv)."""
+    """Fill + sequential factorization; returns (L, U or None, npiv).
+    factotype 0/3: LL^t; 1/4: LDL^t (U is the L D copy of the panels); 2: LU."""
     lib = load()
     pfx = "d" if dtype == np.float64 else "s"
     colptr, rowtab, Lv, Uv = permuted_csc(A, perm)
     L = np.zeros(S.coefnbr, dtype=dtype)
-    U = np.zeros(S.coefnbr, dtype=dtype) if factotype == 2 else None
+    U = np.zeros(S.coefnbr, dtype=dtype) if factotype in (1, 2, 4) else None
     getattr(lib, "oracle_%sfill" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(colptr), _p(rowtab),
-                                        _p(Lv), _p(Uv), _p(L), _p(U))
+                                        _p(Lv), _p(Uv), _p(L), _p(U if factotype == 2 else None))
     npiv = np.zeros(1, dtype=np.int64)
     nneg = np.zeros(1, dtype=np.int64)
     if factotype == 2:
         getattr(lib, "oracle_%sgetrf" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), _p(U), crit, _p(npiv))
+    elif factotype in (1, 4):
+        getattr(lib, "oracle_%ssytrf" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), _p(U), crit, _p(npiv))
     else:
         getattr(lib, "oracle_%spotrf" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), crit, _p(npiv), _p(nneg))
     return L, U, int(npiv[0])

@@ -66,16 +66,25 @@ class SolverCblk(C.Structure):
 
 assert C.sizeof(SolverBlok) == 104 and C.sizeof(SolverCblk) == 160
 
+
+class pastix_rhs_s(C.Structure):
+    """Mirror of ``struct pastix_rhs_s`` (pastix_amd/csrc/host/internal.h):
+    read-only view of a right-hand-side handle made by ``pastixRhsInit``."""
+    _fields_ = [
+        ("allocated", C.c_int), ("flttype", C.c_int), ("m", pastix_int_t), ("n", pastix_int_t),
+        ("ld", pastix_int_t), ("b", C.c_void_p),
+    ]
+
 # exported entry points: name -> (restype, argtypes)
 _P = C.c_void_p
 _I = pastix_int_t
 _D = C.c_double
 PROTOTYPES = {
     "pastixInitParam": (None, [_P, _P]),
-    "pastixInit": (None, [_P, C.c_int, _P, _P]),
-    "pastixInitWithAffinity": (None, [_P, C.c_int, _P, _P, _P]),
+    "pastixInit": (None, [_P, C.c_size_t, _P, _P]),
+    "pastixInitWithAffinity": (None, [_P, C.c_size_t, _P, _P, _P]),
     "pastixFinalize": (None, [_P]),
-    "pastix": (C.c_int, [_P, C.c_int, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "pastix": (C.c_int, [_P, C.c_size_t, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "pastix_task_analyze": (C.c_int, [_P, _P]),
     "pastix_task_numfact": (C.c_int, [_P, _P]),
     "pastix_task_solve": (C.c_int, [_P, _I, _I, _P, _I]),
@@ -88,9 +97,16 @@ PROTOTYPES = {
     "pastix_subtask_spm2bcsc": (C.c_int, [_P, _P]),
     "pastix_subtask_bcsc2ctab": (C.c_int, [_P]),
     "pastix_subtask_sopalin": (C.c_int, [_P]),
+    "pastix_subtask_applyorder": (C.c_int, [_P, C.c_int, _I, _I, _P, _I, _P]),
+    "pastix_subtask_trsm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "pastix_subtask_diag": (C.c_int, [_P, _P]),
+    "pastix_subtask_solve": (C.c_int, [_P, _P]),
+    "pastix_subtask_refine": (C.c_int, [_P, _P, _P]),
     "pastix_subtask_solve_adv": (C.c_int, [_P, C.c_int, _P]),
     "pastixRhsInit": (C.c_int, [_P]),
     "pastixRhsFinalize": (C.c_int, [_P]),
+    "pastixRhsDoubletoSingle": (C.c_int, [_P, _P]),
+    "pastixRhsSingleToDouble": (C.c_int, [_P, _P]),
     "pastixCheckParam": (C.c_int, [_P, _P]),
     "pastixDumpParam": (None, [_P]),
     "pastixOrderAlloc": (C.c_int, [_P, _I, _I]),
@@ -103,6 +119,7 @@ PROTOTYPES = {
     "pastixOrderGrid": (C.c_int, [_P, _I, _I, _I]),
     "pastix_amd_solve_dev": (C.c_int, [_P, C.c_int, _I, _P, _I]),
     "pastix_amd_refine_dev": (C.c_int, [_P, _I, _P, _I, _P, _I]),
+    "pastix_amd_refine_dev_adv": (C.c_int, [_P, C.c_int, _I, _P, _I, _P, _I]),
     "pastix_amd_sampled_outer": (C.c_int, [_P, _P, _P, _I, _P, _I, _P, _I, C.c_double, _P]),
     "pastix_amd_values_dev": (C.c_int, [_P, _P]),
     "pastix_amd_tune": (C.c_int, [_P, _P, _P]),

@@ -8,7 +8,8 @@ from the device tensor straight into the device CSC
 (``pastix_amd_values_dev``), so a forward pass moves no matrix data over
 PCIe.  The backward pass reuses the factors:
 
-    lambda   = A^{-T} g          (transposed device solve, refined in FP64)
+    lambda   = A^{-T} g          (transposed device solve, refined in FP64
+                                 with the transposed operator)
     dL/db    = lambda
     dL/dA_ij = -lambda_i x_j     only on the stored pattern (HIP kernel
                                  ``pastix_amd_sampled_outer``)
@@ -55,9 +56,11 @@ class SparseSolver(object):
         self.d_colptr = torch.from_numpy(self.indptr).to(dev)
         self.d_rows = torch.from_numpy(self.indices).to(dev)
         self._spm = None
+        self.version = 0   # bumped by every factorization
 
     def factor(self, values):
         """Numerical factorization with `values` (nnz, CSC order)."""
+        self.version += 1
         if values.shape[0] != self.nnz:
             raise ValueError("expected %d values, got %d" % (self.nnz, values.shape[0]))
         if self._spm is None or not values.is_cuda:
@@ -83,11 +86,11 @@ class SparseSolver(object):
         torch.cuda.current_stream(x.device).synchronize()
         tr = _tr.Trans if transpose else _tr.NoTrans
         _check(lib.pastix_amd_solve_dev(self.h, tr, nrhs, _dptr(x), self.n), "pastix_amd_solve_dev")
-        # FP64 refinement of A x = b; A^T = A for the symmetric factorization
-        if self.refine and (not transpose or self.symmetric):
-            it = lib.pastix_amd_refine_dev(self.h, nrhs, _dptr(bcm), self.n, _dptr(x), self.n)
+        # FP64 refinement of op(A) x = b with the same operator
+        if self.refine:
+            it = lib.pastix_amd_refine_dev_adv(self.h, tr, nrhs, _dptr(bcm), self.n, _dptr(x), self.n)
             if it < 0:
-                raise PastixError("pastix_amd_refine_dev failed with error code %d" % it)
+                raise PastixError("pastix_amd_refine_dev_adv failed with error code %d" % it)
         return x if b.dim() == 1 else x.t().contiguous()
 
     def sampled_outer(self, u, v, alpha=-1.0):
@@ -121,13 +124,19 @@ class _Solve(torch.autograd.Function):
         solver.factor(values)
         x = solver.solve(b)
         ctx.solver = solver
-        ctx.save_for_backward(x)
+        ctx.version = solver.version
+        ctx.save_for_backward(x, values.detach())
         return x
 
     @staticmethod
     def backward(ctx, g):
-        x, = ctx.saved_tensors
+        x, values = ctx.saved_tensors
         s = ctx.solver
+        if s.version != ctx.version:
+            # the solver was refactored since this forward: the adjoint
+            # needs the factors of the forward's values
+            s.factor(values)
+            ctx.version = s.version
         lam = s.solve(g.contiguous(), transpose=True)
         gv = s.sampled_outer(lam, x, -1.0) if ctx.needs_input_grad[0] else None
         gb = lam if ctx.needs_input_grad[1] else None

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libpastix_amd.so"
 
-HOST_SOURCES = ["host/order.c", "host/symbol.c", "host/split.c", "host/solver.c",
+HOST_SOURCES = ["host/order.c", "host/amalgamate.c", "host/symbol.c", "host/split.c", "host/solver.c",
                 "host/bcsc.c", "host/api.c", "host/refine.c"]
 HIP_SOURCES = ["hip/engine.hip", "hip/gpu_api.hip"]
 

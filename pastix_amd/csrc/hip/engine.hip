@@ -28,6 +28,7 @@ double pa_flops_getrf( double n );
 double pa_flops_trsm( double m, double n );
 double pa_flops_gemm( double m, double n, double k );
 double pa_flops_syrk( double k, double n );
+double pa_flops_sytrf( double n );
 }
 
 #define PA_HIP( call )                                                                       \
@@ -130,7 +131,8 @@ struct pa_device_s {
     /* value map of the permuted CSC (device-resident value updates) */
     int64_t *d_vm_Lptr, *d_vm_Lsrc, *d_vm_Uptr, *d_vm_Usrc;
     /* factors */
-    int   prec, lu;
+    int   prec, lu;   /* lu: two factor arrays (LU: U^T; LDL^t: the D L^t copy) */
+    int   ldlt;
     void *d_L, *d_U;
     void *d_DT;          /* LL^t: transposed diagonal blocks (solve)     */
     /* hipGraphs of the factorization and of recent solves (the launch
@@ -152,6 +154,10 @@ struct pa_device_s {
     /* host copy of the structure the schedule is built from */
     std::vector<int32_t> h_c_ncol, h_c_stride, h_c_fblok, h_b_frow, h_b_lrow, h_b_fcblk, h_b_rowoff;
     pa_tune_t tune = pa_tune_default;
+    /* factorization kind and arithmetic the task lists were built for: the
+     * schedule is rebuilt when the parameters change after the analysis */
+    int sched_fact = -1, sched_big = -1;
+    int fact_kind = -1;   /* factorization kind of the factors in HBM        */
     hipEvent_t ev0, ev1;
     double  last_fact_event_s;
     /* per kernel class profiling (diag, trsm, gemm) with HIP events */
@@ -163,6 +169,15 @@ struct pa_device_s {
 };
 
 static inline pa_device_s *DEV( const pastix_data_t *d ) { return (pa_device_s *)d->dev; }
+
+/* LDL^t in real arithmetic: PastixFactLDLT and PastixFactLDLH */
+static inline bool pa_is_ldlt( int fact ) { return fact == PastixFactLDLT || fact == PastixFactLDLH; }
+
+/* FP32 factors (mixed precision or a single precision matrix) */
+static inline int pa_sched_big( const pastix_data_t *pd )
+{
+    return ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
+}
 
 template <typename T>
 static int
@@ -519,10 +534,17 @@ pa_build_schedule( pastix_data_t *pd )
     std::vector<std::vector<int32_t>> bylev( nlev );
     for ( pastix_int_t k = 0; k < cblknbr; k++ ) bylev[level[k]].push_back( (int32_t)k );
 
-    const int lu = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    const int fact = (int)pd->iparm[IPARM_FACTORIZATION];
+    const int ldlt = pa_is_ldlt( fact );
+    /* lu: the U-side updates of an LU factorization; two: a second factor
+     * array (U^T for LU, the D L^t copy of the panels for LDL^t) */
+    const int lu   = ( fact == PastixFactLU );
+    const int two  = lu || ldlt;
     /* 128 x 128 GEMM tiles pay off in FP32; in FP64 their accumulators
      * (64 doubles per lane) leave one wave per SIMD, so 64 x 64 is kept */
-    const int big_tiles = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
+    const int big_tiles = pa_sched_big( pd );
+    dev->sched_fact = (int)pd->iparm[IPARM_FACTORIZATION];
+    dev->sched_big  = big_tiles;
     std::vector<int32_t>      lev_cblks;
     std::vector<int2>         trsm_tasks, tinv_tasks;
     std::vector<int64_t>      c_tinv( std::max<pastix_int_t>( cblknbr, 1 ), -1 );
@@ -539,7 +561,7 @@ pa_build_schedule( pastix_data_t *pd )
      * (k_small_llt); PASTIX_AMD_SMALL=0 sends them through the general
      * kernels (same results up to rounding) */
     const char *sm_env = getenv( "PASTIX_AMD_SMALL" );
-    const bool  use_small = !lu && dev->tune.small && !( sm_env && sm_env[0] == '0' );
+    const bool  use_small = !two && dev->tune.small && !( sm_env && sm_env[0] == '0' );
     auto is_small = [&]( pastix_int_t k ) {
         return use_small && c_ncol[k] <= PA_SM_NC && c_stride[k] <= PA_SM_ROWS;
     };
@@ -552,6 +574,11 @@ pa_build_schedule( pastix_data_t *pd )
             f[0] = pa_flops_getrf( N );
             f[1] = 2. * pa_flops_trsm( M, N );
             f[2] = pa_flops_gemm( M, M, N );
+        }
+        else if ( ldlt ) {
+            f[0] = pa_flops_sytrf( N );
+            f[1] = pa_flops_trsm( M, N ) + N * M;
+            f[2] = pa_flops_syrk( N, M ) + M * M;
         }
         else {
             f[0] = pa_flops_potrf( N );
@@ -679,7 +706,8 @@ pa_build_schedule( pastix_data_t *pd )
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
     }
-    dev->lu = lu;
+    dev->lu   = two;
+    dev->ldlt = ldlt;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_tinv_tasks, tinv_tasks ) ) ) return rc;
@@ -734,7 +762,8 @@ pa_dev_tune_applicable( const pastix_data_t *pd, int c )
     if ( c < 0 || c >= PA_NTUNE ) return 0;
     const pa_tune_t &t = pa_tune_cands[c], &d = pa_tune_default;
     const int big = ( pd->iparm[IPARM_MIXED] != 0 ) || ( pd->iparm[IPARM_FLOAT] == SpmFloat );
-    const int lu  = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    const int fa  = (int)pd->iparm[IPARM_FACTORIZATION];
+    const int lu  = ( fa == PastixFactLU ) || pa_is_ldlt( fa ); /* no narrow-cblk kernels */
     if ( !big && !t.tall64 && t.tm_min != d.tm_min ) return 0; /* no 128-row tiles in FP64 without tall64 */
     if ( big && t.tall64 ) return 0;                            /* FP32 has its 128-row tiles anyway */
     if ( lu && t.small != d.small ) return 0;
@@ -873,6 +902,22 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
+    /* the task lists, the tile-inverse regions and the narrow-cblk kernels
+     * depend on the factorization kind and the arithmetic: a change of
+     * IPARM_FACTORIZATION or IPARM_MIXED after the analysis (read again at
+     * each numerical factorization, as the reference does in
+     * sopalin/pastix_task_sopalin.c:303,509) rebuilds them */
+    if ( dev->sched_fact != (int)pd->iparm[IPARM_FACTORIZATION] || dev->sched_big != pa_sched_big( pd ) ) {
+        int rc;
+        PA_HIP( hipStreamSynchronize( dev->stream ) );
+        if ( ( rc = pa_build_schedule( pd ) ) ) return rc;
+    }
+    if ( ( prec == PA_PREC_FLOAT ) != ( dev->sched_big != 0 ) ||
+         lu != ( dev->sched_fact == PastixFactLU || pa_is_ldlt( dev->sched_fact ) ) ) {
+        fprintf( stderr, "pastix_amd: coeftab arithmetic or factorization kind does not match the schedule\n" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    dev->fact_kind = dev->sched_fact;
     /* the HBM coeftab is kept across numerical factorizations of the same
      * structure and arithmetic; it is only reallocated when they change */
     if ( dev->prec != prec || dev->lu != lu || ( lu && !dev->d_U ) ) pa_free_factors( dev );
@@ -901,9 +946,9 @@ pa_small_llt_launch( pa_device_s *dev, const pa_level_t &lev, T *L, const double
         const int32_t *cb = dev->d_lev_cblks + beg;
         const unsigned g  = (unsigned)( ( n + 3 ) / 4 );
         if ( n > 0 ) {
-            if ( ns == 1 )      k_small_llt<T, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
-            else if ( ns == 2 ) k_small_llt<T, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
-            else                k_small_llt<T, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1 );
+            if ( ns == 1 )      k_small_llt<T, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1, c0 + 3 );
+            else if ( ns == 2 ) k_small_llt<T, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1, c0 + 3 );
+            else                k_small_llt<T, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, L, crit, c0, c1, c0 + 3 );
         }
         beg = lev.sm_end[ns - 1];
     }
@@ -975,7 +1020,12 @@ pa_factorize_t( pa_device_s *dev )
             const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
             T            *Ti   = (T *)dev->d_tinv;
             const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
-            if ( lu ) {
+            if ( dev->ldlt ) {
+                /* L panel against D L^t, with its copy L D into the second array */
+                k_tile_inv<T, 4><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+                k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, 0 );
+            }
+            else if ( lu ) {
                 /* both LU modes (L and U^T panels) in one launch each */
                 k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
                 k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, ti2 );
@@ -988,7 +1038,11 @@ pa_factorize_t( pa_device_s *dev )
         }
         else if ( ntr > 0 ) {
             size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
-            if ( lu ) {
+            if ( dev->ldlt ) {
+                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+                k_ldlt_scalo<T><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            }
+            else if ( lu ) {
                 k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
                 k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
             }
@@ -1005,15 +1059,16 @@ pa_factorize_t( pa_device_s *dev )
             if ( ng <= 0 ) continue;
             const pa_gemm_task *tk = dev->d_gemm_tasks + lev.gc_beg[c];
             T *Up = lu ? U : nullptr;
+            int32_t *nerr = dev->d_counters + 3;
             const unsigned g = (unsigned)ng;
             switch ( c ) {
-            case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
-            case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
-            case 3: k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
-            case 4: k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up ); break;
+            case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+            case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+            case 3: k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+            case 4: k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
             default:
                 if constexpr ( sizeof( T ) == 4 ) {
-                    k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up );
+                    k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr );
                 }
                 else {
                     return PASTIX_ERR_INTERNAL; /* 128 x 128 tiles are FP32 only */
@@ -1071,7 +1126,8 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     pa_device_s *dev = DEV( pd );
     int rc;
     PA_HIP( hipSetDevice( dev->device ) );
-    PA_HIP( hipMemsetAsync( dev->d_counters, 0, 2 * sizeof( int32_t ), dev->stream ) );
+    /* pivots, negative terms, (fill overflow), dropped updates */
+    PA_HIP( hipMemsetAsync( dev->d_counters, 0, 4 * sizeof( int32_t ), dev->stream ) );
     k_set_f64<<<1, 64, 0, dev->stream>>>( dev->d_crit, crit );
     PA_HIP( hipGetLastError() );
     PA_HIP( hipStreamSynchronize( dev->stream ) );
@@ -1113,10 +1169,15 @@ pa_dev_factorize( pastix_data_t *pd, double crit, int32_t *npiv, int32_t *nneg, 
     hipEventElapsedTime( &ms, dev->ev0, dev->ev1 );
     dev->last_fact_event_s = ms * 1e-3;
     if ( seconds ) *seconds = std::chrono::duration<double>( t1 - t0 ).count();
-    int32_t h[2];
+    int32_t h[4];
     PA_HIP( hipMemcpy( h, dev->d_counters, sizeof( h ), hipMemcpyDeviceToHost ) );
     if ( npiv ) *npiv = h[0];
     if ( nneg ) *nneg = h[1];
+    if ( h[3] != 0 ) {
+        fprintf( stderr, "pastix_amd: %d update rows/columns fall outside the block structure of their target "
+                         "(inconsistent symbolic factorization)\n", h[3] );
+        return PASTIX_ERR_INTERNAL;
+    }
     return PASTIX_SUCCESS;
 }
 
@@ -1173,12 +1234,12 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
     return PASTIX_SUCCESS;
 }
 
-/* Cb is the factor of the backward sweep (lower storage, solved with its
- * transpose); CbT/cbt_off locate the transposes of its diagonal blocks */
+/* One forward sweep (op(C) y = b, C in lower 2D-tile storage, C's own
+ * diagonal blocks) or backward sweep (op(C)^T x = y; CT/ct_off locate the
+ * transposes of C's diagonal blocks) over every level. */
 template <typename T>
 static int
-pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off,
-            T *x, int32_t nrhs, int64_t ld )
+pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, int64_t ld )
 {
     const size_t nl = dev->levels.size();
     int rc;
@@ -1199,6 +1260,16 @@ pa_solve_t( pa_device_s *dev, const T *Cf, bool unit_f, const T *Cb, bool unit_b
         if ( nsm > 0 ) pa_small_solve_launch<T, false>( dev, lev, Cf, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
     }
+    return PASTIX_SUCCESS;
+}
+
+template <typename T>
+static int
+pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off, T *x, int32_t nrhs,
+                int64_t ld )
+{
+    const size_t nl = dev->levels.size();
+    int rc;
     for ( size_t li = nl; li-- > 0; ) {
         const pa_level_t &lev = dev->levels[li];
         if ( lev.cblk_end == lev.cblk_beg ) continue;
@@ -1226,44 +1297,75 @@ pa_grid( int64_t n )
     return (unsigned)std::max<int64_t>( 1, std::min<int64_t>( g, 65536 ) );
 }
 
-/*
- * Factors used by each sweep (reference sopalin/pastix_task_solve.c:543-649):
- *   LL^t:        L y = b (non-unit),  L^t x = y (non-unit)
- *   LU:          L y = b (unit),      U x = y (U^T tiles, non-unit)
- *   LU, trans:   U^t y = b (U^T tiles, non-unit), L^t x = y (unit)
- */
-extern "C" int
-pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_x, int64_t ld )
+/* A step of a solve (reference sopalin/pastix_task_solve.c:596-649 and
+ * pastix_subtask_trsm / pastix_subtask_diag): a forward or backward sweep over
+ * one factor array (side 0: L, side 1: the second array, U^T for LU), or the
+ * diagonal scaling x <- D^{-1} x with D the diagonal of the L diagonal
+ * blocks (reference kernels/solve_ztrsmsp.c:664-716). */
+enum { PA_OP_FWD = 1, PA_OP_BWD = 2, PA_OP_DIAG = 3 };
+struct pa_sweep_op {
+    int kind, side, unit;
+};
+
+template <typename T>
+static int
+pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t nrhs, int64_t ld )
+{
+    int rc;
+    for ( int o = 0; o < nops; o++ ) {
+        const pa_sweep_op &op = ops[o];
+        const T *C = (const T *)( op.side ? dev->d_U : dev->d_L );
+        if ( op.kind == PA_OP_FWD ) {
+            if ( ( rc = pa_sweep_fwd_t<T>( dev, C, op.unit != 0, x, nrhs, ld ) ) ) return rc;
+        }
+        else if ( op.kind == PA_OP_BWD ) {
+            /* transposed diagonal blocks of the swept factor: for two-array
+             * factorizations the other array's diagonal square, for LL^t the
+             * separate transposed array */
+            const bool     two = dev->d_U != nullptr;
+            const int64_t *ct_off = two ? dev->d_c_coef : dev->d_c_dtoff;
+            const T       *ct     = (const T *)( two ? ( op.side ? dev->d_L : dev->d_U ) : dev->d_DT );
+            if ( ( rc = pa_sweep_bwd_t<T>( dev, C, op.unit != 0, ct, ct_off, x, nrhs, ld ) ) ) return rc;
+        }
+        else {
+            if ( dev->n > 0 ) {
+                dim3 grid( pa_grid( dev->n ), (unsigned)std::min<int32_t>( nrhs, 64 ) );
+                k_solve_dscale<T><<<grid, 256, 0, dev->stream>>>( dev->S, (int32_t)dev->n, dev->d_col2cblk,
+                                                                 (const T *)dev->d_L, x, nrhs, ld );
+                PA_HIP( hipGetLastError() );
+            }
+        }
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* Run a sequence of solve steps in place on a device FP64 buffer in the
+ * factorization numbering; FP32 factors work on an FP32 copy (down- and
+ * up-conversion, reference sopalin/pastix_task_solve.c:583-592,
+ * sopalin/pastix_rhs.c:162-242).  The launch sequence is replayed from a
+ * cached hipGraph per (buffer, nrhs, ld, steps). */
+static int
+pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
     if ( !dev->d_L ) return PASTIX_ERR_BADPARAMETER;
-    bool lu = dev->lu, tr = ( trans != PastixNoTrans );
-    int fwd_side = ( lu && tr ) ? 1 : 0, bwd_side = ( lu && !tr ) ? 1 : 0;
-    bool unit_f = lu && !tr, unit_b = lu && tr;
-    /* transposed diagonal blocks of the backward factor: for LU the other
-     * factor's diagonal square, for LL^t the separate transposed array */
-    const int64_t *ct_off = lu ? dev->d_c_coef : dev->d_c_dtoff;
-    void          *ct     = lu ? ( bwd_side ? dev->d_L : dev->d_U ) : dev->d_DT;
+    for ( int o = 0; o < nops; o++ )
+        if ( ops[o].side && !dev->d_U ) return PASTIX_ERR_BADPARAMETER;
     int64_t len = dev->n * (int64_t)nrhs;
     if ( dev->prec != PA_PREC_DOUBLE && dev->xs_len < len ) {
         hipFree( dev->d_xs );
+        dev->d_xs = nullptr;
         PA_HIP( hipMalloc( (void **)&dev->d_xs, std::max<int64_t>( len, 1 ) * sizeof( float ) ) );
         dev->xs_len = len;
     }
+    int64_t code = 0;
+    for ( int o = 0; o < nops; o++ ) code = code * 16 + ops[o].kind * 4 + ops[o].side * 2 + ops[o].unit;
     auto launch = [&]() -> int {
-        if ( dev->prec == PA_PREC_DOUBLE ) {
-            const double *L = (const double *)dev->d_L, *U = (const double *)dev->d_U;
-            return pa_solve_t<double>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const double *)ct,
-                                       ct_off, d_x, nrhs, ld );
-        }
-        /* mixed precision: down-convert, FP32 sweeps, up-convert
-         * (reference sopalin/pastix_task_solve.c:583-592, pastix_rhs.c:162-242) */
+        if ( dev->prec == PA_PREC_DOUBLE ) return pa_run_ops_t<double>( dev, ops, nops, d_x, nrhs, ld );
         for ( int32_t r = 0; r < nrhs; r++ )
             k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
-        const float *L = (const float *)dev->d_L, *U = (const float *)dev->d_U;
-        int rc = pa_solve_t<float>( dev, fwd_side ? U : L, unit_f, bwd_side ? U : L, unit_b, (const float *)ct, ct_off,
-                                    dev->d_xs, nrhs, dev->n );
+        int rc = pa_run_ops_t<float>( dev, ops, nops, dev->d_xs, nrhs, dev->n );
         if ( rc ) return rc;
         for ( int32_t r = 0; r < nrhs; r++ )
             k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );
@@ -1271,11 +1373,11 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
         return PASTIX_SUCCESS;
     };
     if ( !dev->use_graphs ) return launch();
-    /* cached graph per (buffer, nrhs, ld, trans) among the most recent ones */
+    /* cached graph per (buffer, nrhs, ld, steps) among the most recent ones */
     pa_graph_entry *slot = nullptr, *oldest = &dev->solve_graphs[0];
     for ( auto &g : dev->solve_graphs ) {
-        if ( g.exec && g.key[0] == d_x && g.key[1] == dev->d_xs && g.key[2] == dev->d_L && g.ikey[0] == nrhs &&
-             g.ikey[1] == ld && g.ikey[2] == (int64_t)trans && g.ikey[3] == dev->prec ) {
+        if ( g.exec && g.key[0] == d_x && g.key[1] == dev->d_xs && g.key[2] == dev->d_L && g.key[3] == dev->d_U &&
+             g.ikey[0] == nrhs && g.ikey[1] == ld && g.ikey[2] == code && g.ikey[3] == dev->prec ) {
             slot = &g;
             break;
         }
@@ -1285,10 +1387,64 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
     if ( !hit ) slot = oldest;
     int rc = pa_graph_run( dev, *slot, hit, launch );
     if ( rc ) return rc;
-    slot->key[0] = d_x; slot->key[1] = dev->d_xs; slot->key[2] = dev->d_L;
-    slot->ikey[0] = nrhs; slot->ikey[1] = ld; slot->ikey[2] = (int64_t)trans; slot->ikey[3] = dev->prec;
+    slot->key[0] = d_x; slot->key[1] = dev->d_xs; slot->key[2] = dev->d_L; slot->key[3] = dev->d_U;
+    slot->ikey[0] = nrhs; slot->ikey[1] = ld; slot->ikey[2] = code; slot->ikey[3] = dev->prec;
     slot->stamp = ++dev->graph_clock;
     return PASTIX_SUCCESS;
+}
+
+/*
+ * Steps of the full solve (reference sopalin/pastix_task_solve.c:543-649):
+ *   LL^t:        L y = b (non-unit),  L^t x = y (non-unit)
+ *   LDL^t:       L y = b (unit),      D z = y,  L^t x = z (unit)
+ *   LU:          L y = b (unit),      U x = y (U^T tiles, non-unit)
+ *   LU, trans:   U^t y = b (U^T tiles, non-unit), L^t x = y (unit)
+ */
+extern "C" int
+pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_x, int64_t ld )
+{
+    pa_device_s *dev = DEV( pd );
+    const int    fact = dev->fact_kind;
+    const bool   tr   = ( trans != PastixNoTrans );
+    pa_sweep_op  ops[3];
+    int          nops = 0;
+    if ( fact == PastixFactLU ) {
+        ops[nops++] = { PA_OP_FWD, tr ? 1 : 0, tr ? 0 : 1 };
+        ops[nops++] = { PA_OP_BWD, tr ? 0 : 1, tr ? 1 : 0 };
+    }
+    else if ( pa_is_ldlt( fact ) ) {
+        ops[nops++] = { PA_OP_FWD, 0, 1 };
+        ops[nops++] = { PA_OP_DIAG, 0, 0 };
+        ops[nops++] = { PA_OP_BWD, 0, 1 };
+    }
+    else {
+        ops[nops++] = { PA_OP_FWD, 0, 0 };
+        ops[nops++] = { PA_OP_BWD, 0, 0 };
+    }
+    return pa_dev_run_ops( pd, ops, nops, nrhs, d_x, ld );
+}
+
+/* One triangular sweep in the factorization numbering (reference
+ * pastix_subtask_trsm, sopalin/pastix_task_solve.c:296 and the cases of
+ * kernels/solve_ztrsmsp.c:326-350, 491-510 with side = Left):
+ *   (Lower, NoTrans) forward over L, (Lower, Trans) backward over L,
+ *   (Upper, Trans) forward over the stored U^T, (Upper, NoTrans) backward
+ *   over the stored U^T. */
+extern "C" int
+pa_dev_trsm_perm( pastix_data_t *pd, int upper, int transposed, int unit, int32_t nrhs, double *d_x, int64_t ld )
+{
+    pa_device_s *dev = DEV( pd );
+    if ( upper && !dev->d_U ) return PASTIX_ERR_BADPARAMETER;
+    pa_sweep_op op = { ( upper ? !transposed : transposed ) ? PA_OP_BWD : PA_OP_FWD, upper ? 1 : 0, unit ? 1 : 0 };
+    return pa_dev_run_ops( pd, &op, 1, nrhs, d_x, ld );
+}
+
+/* x <- D^{-1} x (reference pastix_subtask_diag, sopalin/pastix_task_solve.c:422) */
+extern "C" int
+pa_dev_diag_perm( pastix_data_t *pd, int32_t nrhs, double *d_x, int64_t ld )
+{
+    pa_sweep_op op = { PA_OP_DIAG, 0, 0 };
+    return pa_dev_run_ops( pd, &op, 1, nrhs, d_x, ld );
 }
 
 extern "C" int

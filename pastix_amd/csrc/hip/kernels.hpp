@@ -781,7 +781,7 @@ k_diag_potrf_reg_one( float *D, int32_t nc, double crit, int32_t *npiv, int32_t 
 template <typename T, int NS>
 __global__ __launch_bounds__( 256 ) void
 k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const double *critp, int32_t *npiv,
-             int32_t *nneg )
+             int32_t *nneg, int32_t *nerr )
 {
     const int     lane = threadIdx.x & 63;
     const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
@@ -900,6 +900,11 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
                 }
             }
         }
+        /* a row of these blocks outside F's structure (missing fill) is
+         * not written: counted for the host to report */
+#pragma unroll
+        for ( int q = 0; q < NS; q++ )
+            if ( bidx[q] >= gb && !tok[q] ) atomicAdd( nerr, 1 );
         /* the columns of F updated by this group: the rows of blocks gb..ge */
         const int32_t b0 = S.b_rowoff[gb], b1 = S.b_rowoff[ge] + ( S.b_lrow[ge] - S.b_frow[ge] + 1 );
         for ( int32_t rbrow = b0; rbrow < b1; rbrow++ ) {
@@ -1153,8 +1158,9 @@ __global__ __launch_bounds__( 256 ) void
 k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T *Uc, T *tinv, int64_t tinv2 )
 {
     /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y; mode 2's
-     * inverses start tinv2 elements further */
-    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : MODE;
+     * inverses start tinv2 elements further.  MODE 4 (LDL^t): the tiles of
+     * DL^t, stored as in mode 1 */
+    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : ( MODE == 4 ? 1 : MODE );
     if ( md == 2 ) tinv += tinv2;
     __shared__ __align__( 16 ) T sT[4][16][16]; /* per wave: sT[c][l] = T(l, c) of the tile */
     const int     lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1200,8 +1206,12 @@ template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
 k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_t tinv2 )
 {
-    /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y */
-    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : MODE;
+    /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y.  MODE 4
+     * (LDL^t, reference cpucblk_zsytrfsp1d_panel, kernels/core_zsytrfsp.c:
+     * 296-318): X <- X (DL^t)^{-1} as mode 1, the upper part of the factored
+     * diagonal block being DL^t, and the copy X D (cpucblk_zscalo) stored in
+     * the second array for the updates */
+    const int md = ( MODE == 3 ) ? 1 + (int)blockIdx.y : ( MODE == 4 ? 1 : MODE );
     if ( md == 2 ) tinv += tinv2;
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int SR  = PA_TRSM_TR;
@@ -1330,11 +1340,37 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_
 #pragma unroll
             for ( int s = 0; s < 4; s++ ) {
                 const int32_t col = 16 * j + pa_mfma<T>::row( lane, s );
+                /* D(col) for the LDL^t copy: the diagonal of the factored block */
+                const T d = ( MODE == 4 ) ? Q[(int64_t)min( col, nc - 1 ) * ( nc + 1 )] : T( 0 );
 #pragma unroll
                 for ( int rt = 0; rt < NRT; rt++ )
-                    if ( rv[rt] && col < nc ) panel[ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s];
+                    if ( rv[rt] && col < nc ) {
+                        panel[ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s];
+                        if ( MODE == 4 ) Uc[S.c_coef[k] + ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s] * d;
+                    }
             }
         }
+    }
+}
+
+/* LDL^t panel copy X D of the rows below the diagonal block (reference
+ * cpucblk_zscalo, kernels/core_zscalo.c), after the substitution TRSM of
+ * the widest cblks: one TRSM slab (cblk, 32 rows) per workgroup, D the
+ * diagonal of the factored diagonal block of Lc. */
+template <typename T>
+__global__ __launch_bounds__( 256 ) void
+k_ldlt_scalo( pa_dstruct S, const int2 *tasks, const T *Lc, T *Uc )
+{
+    const int2    task = tasks[blockIdx.x];
+    const int32_t k = task.x, r0 = task.y;
+    const int32_t nc = S.c_ncol[k], R = S.c_stride[k] - nc;
+    const int32_t rows = min( R - r0, (int32_t)PA_TRSM_TR );
+    const int64_t cb = S.c_coef[k], rb = S.r_base[k];
+    for ( int32_t e = threadIdx.x; e < rows * nc; e += 256 ) {
+        const int32_t r = e % rows, c = e / rows;
+        const int4    ri = S.r_info[rb + r0 + r];
+        const int64_t o  = pa_rinfo_off( ri ) + (int64_t)c * ri.y;
+        Uc[o] = Lc[o] * Lc[cb + (int64_t)c * ( nc + 1 )];
     }
 }
 
@@ -1552,10 +1588,13 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     }
 }
 
-/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task (TM x TN tile). */
+/* Level-batched scatter-GEMM: one workgroup per pa_gemm_task (TM x TN tile).
+ * An update row or column that the target cblk's block structure does not
+ * hold (a symbolic factorization that misses fill) is not written and is
+ * counted in *nerr, which the host turns into an error. */
 template <typename T, int TM, int TN>
 __global__ __launch_bounds__( 256 ) void
-k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
+k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *nerr )
 {
     constexpr int PAD = pa_gpad<T>::value;
     __shared__ T As[PA_GKC][TM + PAD];
@@ -1592,9 +1631,11 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
         }
         else {
             const int32_t col = ri.x - S.c_fcol[F];
+            const bool    in  = col >= 0 && col < S.c_ncol[F];
             b_off[tt] = off; b_blk[tt] = blk; c_col[tt] = col;
             /* a column outside the target cblk is never written */
-            b_ld[tt] = ( ok && col >= 0 && col < S.c_ncol[F] ) ? ri.y : 0;
+            b_ld[tt] = ( ok && in ) ? ri.y : 0;
+            if ( ok && !in ) atomicAdd( nerr, 1 );
         }
     }
     if ( staged ) {
@@ -1631,7 +1672,10 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc )
         c_off[tt] = coefF + (int64_t)ro * ncF + ( grow - fr );
         c_ld[tt]  = lr - fr + 1;
         /* a row outside the target structure is never written */
-        if ( grow < fr || grow > lr ) a_ld[tt] = 0;
+        if ( grow < fr || grow > lr ) {
+            a_ld[tt] = 0;
+            atomicAdd( nerr, 1 );
+        }
     }
     __syncthreads();
 
@@ -2414,6 +2458,22 @@ k_sampled_outer( int64_t n, const int64_t *colptr, const int64_t *rows, int32_t 
         double        s = 0.;
         for ( int32_t q = 0; q < nrhs; q++ ) s += u[r + q * ldu] * v[c + q * ldv];
         out[e] = alpha * s;
+    }
+}
+
+/* x[j] /= D_j for every column j of the factorization (nrhs columns,
+ * blockIdx.y walks them), D_j the diagonal entry of the L diagonal block of
+ * j's cblk (reference solve_cblk_zdiag, kernels/solve_ztrsmsp.c:664-716) */
+template <typename T>
+__global__ __launch_bounds__( 256 ) void
+k_solve_dscale( pa_dstruct S, int32_t n, const int32_t *col2cblk, const T *Lc, T *x, int32_t nrhs, int64_t ldx )
+{
+    for ( int32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256 ) {
+        const int32_t k  = col2cblk[j];
+        const int32_t nc = S.c_ncol[k];
+        const int32_t jl = j - S.c_fcol[k];
+        const T       d  = Lc[S.c_coef[k] + (int64_t)jl * ( nc + 1 )];
+        for ( int32_t r = blockIdx.y; r < nrhs; r += gridDim.y ) x[j + r * ldx] /= d;
     }
 }
 

@@ -47,6 +47,13 @@ int    pa_dev_factorize( pastix_data_t *pastix_data, double criterion, int32_t *
  * numbering (nrhs columns, leading dimension ld) */
 int    pa_dev_solve_perm( pastix_data_t *pastix_data, pastix_trans_t trans, int32_t nrhs,
                           double *d_x, int64_t ld );
+/* one triangular sweep on a device FP64 buffer in the factorization
+ * numbering: upper = 0 sweeps L, upper = 1 the stored U^T; transposed
+ * selects op(factor) = factor^T; unit ignores the stored diagonal */
+int    pa_dev_trsm_perm( pastix_data_t *pastix_data, int upper, int transposed, int unit, int32_t nrhs,
+                         double *d_x, int64_t ld );
+/* x <- D^{-1} x with D the diagonal of the L diagonal blocks */
+int    pa_dev_diag_perm( pastix_data_t *pastix_data, int32_t nrhs, double *d_x, int64_t ld );
 /* y = alpha op(A) x + beta y, factorization numbering */
 int    pa_dev_spmv_perm( pastix_data_t *pastix_data, pastix_trans_t trans, double alpha,
                          const double *d_x, double beta, double *d_y );

@@ -125,19 +125,22 @@ int
 pastixCheckParam( const pastix_int_t *iparm, const double *dparm )
 {
     (void)dparm;
-    if ( iparm[IPARM_FACTORIZATION] != PastixFactLLT && iparm[IPARM_FACTORIZATION] != PastixFactPOTRF &&
-         iparm[IPARM_FACTORIZATION] != PastixFactLU ) {
-        pa_error( "factorization %ld is not available in this build (LL^t and LU only)",
+    if ( iparm[IPARM_FACTORIZATION] < PastixFactPOTRF || iparm[IPARM_FACTORIZATION] > PastixFactHETRF ) {
+        pa_error( "factorization %ld is not a factorization kind (POTRF, SYTRF, GETRF, PXTRF, HETRF)",
                   (long)iparm[IPARM_FACTORIZATION] );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( iparm[IPARM_INCOMPLETE] != 0 ) {
+        pa_error( "incomplete (ILU(k)) factorization is not available in this build" );
         return PASTIX_ERR_NOTIMPLEMENTED;
     }
     if ( iparm[IPARM_MIN_BLOCKSIZE] < 1 || iparm[IPARM_MAX_BLOCKSIZE] < iparm[IPARM_MIN_BLOCKSIZE] ) {
         return PASTIX_ERR_BADPARAMETER;
     }
-    if ( iparm[IPARM_REFINEMENT] < PastixRefineGMRES || iparm[IPARM_REFINEMENT] > PastixRefineSR ) {
-        pa_error( "refinement %ld is not available in this build (GMRES, CG, SR)",
+    if ( iparm[IPARM_REFINEMENT] < PastixRefineGMRES || iparm[IPARM_REFINEMENT] > PastixRefineBiCGSTAB ) {
+        pa_error( "refinement %ld is not a refinement method (GMRES, CG, SR, BiCGSTAB)",
                   (long)iparm[IPARM_REFINEMENT] );
-        return PASTIX_ERR_NOTIMPLEMENTED;
+        return PASTIX_ERR_BADPARAMETER;
     }
     return PASTIX_SUCCESS;
 }
@@ -188,6 +191,9 @@ pastixFinalize( pastix_data_t **pastix_data )
     pa_dev_close( pd );
     pa_clear_analysis( pd );
     if ( pd->bcsc ) { pa_bcsc_exit( pd->bcsc ); free( pd->bcsc ); }
+    free( pd->legacy_spm );
+    free( pd->b );
+    free( pd->x0 );
     free( pd );
     *pastix_data = NULL;
 }
@@ -250,7 +256,6 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
                       (long)n );
             return PASTIX_ERR_BADPARAMETER;
         }
-        if ( o->treetab == NULL ) pa_order_tree_from_rangtab( o, n, pd->graph->colptr, pd->graph->rows );
         goto ordered;
     }
     pastixOrderAlloc( o, n, 0 );
@@ -283,10 +288,17 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
         memcpy( o->rangtab, myorder->rangtab, ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
         memcpy( o->treetab, myorder->treetab, o->cblknbr * sizeof( pastix_int_t ) );
     }
-    else {
-        pa_order_from_etree( o, n, pd->graph->colptr, pd->graph->rows );
-    }
 ordered:
+    if ( o->rangtab == NULL || o->treetab == NULL ) {
+        /* a permutation without its supernode partition and tree: fundamental
+         * supernodes of the postordered elimination tree, then amalgamation
+         * (reference order/pastix_subtask_order.c:362-381) */
+        if ( ( rc = pa_order_find_supernodes( o, n, pd->graph->colptr, pd->graph->rows ) ) ) return rc;
+        if ( ( rc = pa_order_amalgamate( o, n, pd->graph->colptr, pd->graph->rows, 0,
+                                         pd->iparm[IPARM_AMALGAMATION_LVLCBLK],
+                                         pd->iparm[IPARM_AMALGAMATION_LVLBLAS] ) ) )
+            return rc;
+    }
     if ( pastixOrderCheck( o ) != PASTIX_SUCCESS ) {
         pa_error( "pastix_subtask_order: inconsistent ordering" );
         return PASTIX_ERR_BADPARAMETER;
@@ -459,10 +471,23 @@ pastix_subtask_bcsc2ctab( pastix_data_t *pd )
     int32_t overflow = 0;
     int prec, rc;
     if ( pd == NULL || !( pd->steps & STEP_CSC2BCSC ) ) return PASTIX_ERR_BADPARAMETER;
+    if ( ( rc = pastixCheckParam( pd->iparm, pd->dparm ) ) ) return rc;
     prec = pd->iparm[IPARM_MIXED] ? PA_PREC_FLOAT : PA_PREC_DOUBLE;
     if ( pd->iparm[IPARM_FLOAT] == SpmFloat ) prec = PA_PREC_FLOAT;
     pd->solvmatr->flttype = ( prec == PA_PREC_FLOAT ) ? PastixFloat : PastixDouble;
-    rc = pa_dev_coeftab_init( pd, prec, pd->iparm[IPARM_FACTORIZATION] == PastixFactLU, &overflow );
+    if ( pd->solvmatr->factotype != pd->iparm[IPARM_FACTORIZATION] ) {
+        /* the factorization kind changed after the analysis: the flop count
+         * of the new kind (the device schedule follows at the fill) */
+        pd->solvmatr->factotype       = (int)pd->iparm[IPARM_FACTORIZATION];
+        pd->dparm[DPARM_FACT_THFLOPS] = pa_symbol_flops( pd->symbmtx, (int)pd->iparm[IPARM_FACTORIZATION] );
+    }
+    /* a second factor array: U^T of LU, the D L^t copy of LDL^t / LDL^h
+     * (real arithmetic: LL^h is LL^t and LDL^h is LDL^t) */
+    rc = pa_dev_coeftab_init( pd, prec,
+                              pd->iparm[IPARM_FACTORIZATION] == PastixFactLU ||
+                                  pd->iparm[IPARM_FACTORIZATION] == PastixFactLDLT ||
+                                  pd->iparm[IPARM_FACTORIZATION] == PastixFactLDLH,
+                              &overflow );
     if ( rc ) return rc;
     if ( overflow ) {
         pa_warning( "%d coefficients overflow the single precision range", overflow );
@@ -582,19 +607,170 @@ pastix_amd_set_tuning( pastix_data_t *pd, const pastix_int_t *params )
     return PASTIX_SUCCESS;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Right-hand sides (reference sopalin/pastix_rhs.c)                         */
+/* ------------------------------------------------------------------------- */
 int
 pastixRhsInit( pastix_rhs_t *rhs )
 {
-    *rhs = calloc( 1, sizeof( struct pastix_rhs_s ) );
-    return *rhs ? PASTIX_SUCCESS : PASTIX_ERR_ALLOC;
+    pastix_rhs_t B;
+    if ( rhs == NULL ) return PASTIX_ERR_BADPARAMETER;
+    B = calloc( 1, sizeof( struct pastix_rhs_s ) );
+    if ( B == NULL ) return PASTIX_ERR_ALLOC;
+    B->allocated = -1;
+    B->flttype   = PastixPattern;
+    B->m = B->n = B->ld = -1;
+    *rhs = B;
+    return PASTIX_SUCCESS;
 }
 
 int
 pastixRhsFinalize( pastix_rhs_t rhs )
 {
-    if ( rhs && rhs->allocated ) free( rhs->b );
+    if ( rhs == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( rhs->b != NULL ) {
+        if ( rhs->allocated > 0 ) {
+            free( rhs->b );
+        }
+        else {
+            pa_warning( "pastixRhsFinalize called before the ordering of b was restored "
+                        "(pastix_subtask_applyorder with PastixDirBackward)" );
+        }
+    }
     free( rhs );
     return PASTIX_SUCCESS;
+}
+
+static size_t
+pa_flt_size( int flttype )
+{
+    return ( flttype == PastixFloat ) ? sizeof( float ) : sizeof( double );
+}
+
+/* FP64 -> FP32 copy of an RHS (reference pastixRhsDoubletoSingle,
+ * sopalin/pastix_rhs.c:162-236): an entry outside the single precision range
+ * is an error, as LAPACK dlag2s reports it. */
+int
+pastixRhsDoubletoSingle( const pastix_rhs_t dB, pastix_rhs_t sB )
+{
+    pastix_int_t i, j;
+    int          ovf = 0, tofree = 0;
+    if ( dB == NULL || sB == NULL || dB->flttype != PastixDouble || dB->b == NULL ) {
+        pa_error( "pastixRhsDoubletoSingle: invalid float type for mixed precision" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( sB->allocated == -1 ) {
+        *sB           = *dB;
+        sB->allocated = 1;
+        sB->flttype   = PastixFloat;
+        sB->b         = malloc( (size_t)dB->ld * dB->n * sizeof( float ) );
+        if ( sB->b == NULL ) return PASTIX_ERR_ALLOC;
+        tofree = 1;
+    }
+    if ( sB->flttype != PastixFloat || sB->b == NULL || sB->m != dB->m || sB->n != dB->n || sB->ld < sB->m )
+        return PASTIX_ERR_BADPARAMETER;
+    for ( j = 0; j < dB->n; j++ ) {
+        const double *src = (const double *)dB->b + j * dB->ld;
+        float        *dst = (float *)sB->b + j * sB->ld;
+        for ( i = 0; i < dB->m; i++ ) {
+            if ( fabs( src[i] ) > (double)FLT_MAX ) ovf = 1;
+            dst[i] = (float)src[i];
+        }
+    }
+    if ( ovf ) {
+        if ( tofree ) {
+            free( sB->b );
+            sB->b         = NULL;
+            sB->allocated = -1;
+        }
+        return PASTIX_ERR_INTERNAL;
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* FP32 -> FP64 copy of an RHS (reference pastixRhsSingleToDouble,
+ * sopalin/pastix_rhs.c:242-316) */
+int
+pastixRhsSingleToDouble( const pastix_rhs_t sB, pastix_rhs_t dB )
+{
+    pastix_int_t i, j;
+    if ( sB == NULL || dB == NULL || sB->flttype != PastixFloat || sB->b == NULL ) {
+        pa_error( "pastixRhsSingleToDouble: invalid float type for mixed precision" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( dB->allocated == -1 ) {
+        *dB           = *sB;
+        dB->allocated = 1;
+        dB->flttype   = PastixDouble;
+        dB->b         = malloc( (size_t)sB->ld * sB->n * sizeof( double ) );
+        if ( dB->b == NULL ) return PASTIX_ERR_ALLOC;
+    }
+    if ( dB->flttype != PastixDouble || dB->b == NULL || dB->m != sB->m || dB->n != sB->n || dB->ld < dB->m )
+        return PASTIX_ERR_BADPARAMETER;
+    for ( j = 0; j < sB->n; j++ ) {
+        const float *src = (const float *)sB->b + j * sB->ld;
+        double      *dst = (double *)dB->b + j * dB->ld;
+        for ( i = 0; i < sB->m; i++ ) dst[i] = (double)src[i];
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* Host RHS of the caller's arithmetic (FP64, or FP32 for a single precision
+ * matrix) <-> device FP64 work array with leading dimension m. */
+static int
+pa_rhs_upload( pastix_data_t *pd, const void *B, int flttype, pastix_int_t m, pastix_int_t nrhs, pastix_int_t ldb,
+               double *dB )
+{
+    pastix_int_t i, j;
+    int rc = PASTIX_SUCCESS;
+    if ( flttype != PastixFloat ) {
+        for ( j = 0; j < nrhs && !rc; j++ )
+            rc = pa_dev_memcpy_h2d( pd, dB + j * m, (const double *)B + j * ldb, m * sizeof( double ) );
+        return rc;
+    }
+    {
+        double *tmp = malloc( (size_t)pa_imax( m, 1 ) * sizeof( double ) );
+        if ( tmp == NULL ) return PASTIX_ERR_ALLOC;
+        for ( j = 0; j < nrhs && !rc; j++ ) {
+            const float *src = (const float *)B + j * ldb;
+            for ( i = 0; i < m; i++ ) tmp[i] = (double)src[i];
+            rc = pa_dev_memcpy_h2d( pd, dB + j * m, tmp, m * sizeof( double ) );
+        }
+        free( tmp );
+    }
+    return rc;
+}
+
+static int
+pa_rhs_download( pastix_data_t *pd, void *B, int flttype, pastix_int_t m, pastix_int_t nrhs, pastix_int_t ldb,
+                 const double *dB )
+{
+    pastix_int_t i, j;
+    int rc = PASTIX_SUCCESS;
+    if ( flttype != PastixFloat ) {
+        for ( j = 0; j < nrhs && !rc; j++ )
+            rc = pa_dev_memcpy_d2h( pd, (double *)B + j * ldb, dB + j * m, m * sizeof( double ) );
+        return rc;
+    }
+    {
+        double *tmp = malloc( (size_t)pa_imax( m, 1 ) * sizeof( double ) );
+        if ( tmp == NULL ) return PASTIX_ERR_ALLOC;
+        for ( j = 0; j < nrhs && !rc; j++ ) {
+            float *dst = (float *)B + j * ldb;
+            rc = pa_dev_memcpy_d2h( pd, tmp, dB + j * m, m * sizeof( double ) );
+            for ( i = 0; !rc && i < m; i++ ) dst[i] = (float)tmp[i];
+        }
+        free( tmp );
+    }
+    return rc;
+}
+
+/* arithmetic of the caller's vectors: the one of the matrix given to the
+ * analysis (reference sopalin/pastix_task_solve.c:200, csc->flttype) */
+static int
+pa_user_flttype( const pastix_data_t *pd )
+{
+    return ( pd->iparm[IPARM_FLOAT] == SpmFloat ) ? PastixFloat : PastixDouble;
 }
 
 /* In-place device solve on an HBM buffer in the user numbering. */
@@ -610,7 +786,7 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrh
         return PASTIX_ERR_BADPARAMETER;
     }
     n = pd->solvmatr->nodenbr;
-    if ( nrhs < 1 || ldb < n ) return PASTIX_ERR_BADPARAMETER;
+    if ( nrhs < 1 || ldb < n || dB == NULL ) return PASTIX_ERR_BADPARAMETER;
     if ( transA != PastixNoTrans && transA != PastixTrans ) return PASTIX_ERR_BADPARAMETER;
     t0  = pa_wtime();
     tmp = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
@@ -626,44 +802,228 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrh
 }
 
 /* Host-buffer solve (reference sopalin/pastix_task_solve.c:758): B is
- * overwritten by the solution; data crosses PCIe both ways. */
+ * overwritten by the solution; data crosses PCIe both ways.  The
+ * permutations P b and P^T x run on the device. */
 int
 pastix_task_solve( pastix_data_t *pd, pastix_int_t m, pastix_int_t nrhs, void *B, pastix_int_t ldb )
 {
     double *dB;
-    pastix_int_t n, j;
-    int rc;
+    pastix_int_t n;
+    int rc, flt;
     if ( pd == NULL || !( pd->steps & STEP_NUMFACT ) ) {
         pa_error( "pastix_task_solve: all steps up to pastix_task_numfact() must be done first" );
         return PASTIX_ERR_BADPARAMETER;
     }
     n = pd->solvmatr->nodenbr;
-    if ( m != n || ldb < n || B == NULL ) return PASTIX_ERR_BADPARAMETER;
-    dB = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    if ( m != n || ldb < n || B == NULL || nrhs < 1 ) return PASTIX_ERR_BADPARAMETER;
+    flt = pa_user_flttype( pd );
+    dB  = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
     if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
-    for ( j = 0; j < nrhs; j++ ) pa_dev_memcpy_h2d( pd, dB + j * n, (double *)B + j * ldb, n * sizeof( double ) );
-    rc = pastix_amd_solve_dev( pd, (pastix_trans_t)pd->iparm[IPARM_TRANSPOSE_SOLVE], nrhs, dB, n );
-    for ( j = 0; !rc && j < nrhs; j++ ) pa_dev_memcpy_d2h( pd, (double *)B + j * ldb, dB + j * n, n * sizeof( double ) );
+    rc = pa_rhs_upload( pd, B, flt, n, nrhs, ldb, dB );
+    if ( !rc ) rc = pastix_amd_solve_dev( pd, (pastix_trans_t)pd->iparm[IPARM_TRANSPOSE_SOLVE], nrhs, dB, n );
+    if ( !rc ) rc = pa_rhs_download( pd, B, flt, n, nrhs, ldb, dB );
     pa_dev_wput( pd, dB );
     return rc;
 }
 
+/* Permutation of the right-hand sides (reference
+ * pastix_subtask_applyorder, sopalin/pastix_task_solve.c:169-290, and
+ * bvec_zlapmr's shared-memory case, bcsc/bvec_zlapmr.c:816-910): Forward
+ * permutes b in place (b[i] <- b[peritab[i]]) and records it in Bp; Backward
+ * restores the user numbering in place and resets Bp. */
 int
-pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t b )
+pastix_subtask_applyorder( pastix_data_t *pd, pastix_dir_t dir, pastix_int_t m, pastix_int_t n, void *b,
+                           pastix_int_t ldb, pastix_rhs_t Bp )
 {
-    if ( b == NULL || b->flttype != PastixDouble ) return PASTIX_ERR_BADPARAMETER;
-    {
-        double *dB = pa_dev_wget( pd, (size_t)b->m * b->n * sizeof( double ) );
-        int rc;
-        pastix_int_t j;
-        if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
-        for ( j = 0; j < b->n; j++ ) pa_dev_memcpy_h2d( pd, dB + j * b->m, (double *)b->b + j * b->ld, b->m * sizeof( double ) );
-        /* b is already in the factorization numbering */
-        rc = pa_dev_solve_perm( pd, transA, (int32_t)b->n, dB, b->m );
-        for ( j = 0; !rc && j < b->n; j++ ) pa_dev_memcpy_d2h( pd, (double *)b->b + j * b->ld, dB + j * b->m, b->m * sizeof( double ) );
-        pa_dev_wput( pd, dB );
-        return rc;
+    const pastix_order_t *o;
+    pastix_int_t i, j, N;
+    int          flt;
+    size_t       sz;
+    char        *tmp;
+    if ( pd == NULL ) {
+        pa_error( "pastix_subtask_applyorder: wrong pastix_data parameter" );
+        return PASTIX_ERR_BADPARAMETER;
     }
+    if ( Bp == NULL ) {
+        pa_error( "pastix_subtask_applyorder: wrong Bp parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    o = pd->ordemesh;
+    if ( o == NULL || !( pd->steps & STEP_ORDERING ) ) {
+        pa_error( "pastix_subtask_applyorder: the ordering step has to be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( o->baseval != 0 ) {
+        pa_error( "pastix_subtask_applyorder: ordermesh must be 0-based" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    N   = o->vertnbr;
+    flt = pa_user_flttype( pd );
+    if ( m != N || ldb < m || n < 0 || ( b == NULL && n > 0 ) ) return PASTIX_ERR_BADPARAMETER;
+    if ( dir == PastixDirForward ) {
+        Bp->flttype   = flt;
+        Bp->m         = N;
+        Bp->n         = n;
+        Bp->allocated = 0;
+        Bp->ld        = ldb;
+        Bp->b         = b;
+    }
+    else if ( dir == PastixDirBackward ) {
+        if ( Bp->allocated != 0 || Bp->b != b || Bp->m != N || Bp->n != n || Bp->ld != ldb || Bp->flttype != flt ) {
+            pa_error( "pastix_subtask_applyorder: Bp does not describe b" );
+            return PASTIX_ERR_BADPARAMETER;
+        }
+    }
+    else {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    sz  = pa_flt_size( flt );
+    tmp = malloc( (size_t)pa_imax( N, 1 ) * sz );
+    if ( tmp == NULL ) return PASTIX_ERR_ALLOC;
+    for ( j = 0; j < n; j++ ) {
+        char *col = (char *)b + (size_t)j * ldb * sz;
+        if ( dir == PastixDirForward )
+            for ( i = 0; i < N; i++ ) memcpy( tmp + i * sz, col + o->peritab[i] * sz, sz );
+        else
+            for ( i = 0; i < N; i++ ) memcpy( tmp + i * sz, col + o->permtab[i] * sz, sz );
+        memcpy( col, tmp, (size_t)N * sz );
+    }
+    free( tmp );
+    if ( dir == PastixDirBackward ) {
+        Bp->allocated = -1;
+        Bp->flttype   = PastixPattern;
+        Bp->m = Bp->n = Bp->ld = -1;
+        Bp->b         = NULL;
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* checks shared by the subtasks that work on a permuted RHS */
+static int
+pa_check_rhs( pastix_data_t *pd, const pastix_rhs_t Bp, const char *who )
+{
+    if ( pd == NULL ) {
+        pa_error( "%s: wrong pastix_data parameter", who );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( Bp == NULL || Bp->b == NULL ) {
+        pa_error( "%s: wrong Bp parameter", who );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !( pd->steps & STEP_NUMFACT ) ) {
+        pa_error( "%s: all steps up to pastix_task_numfact() have to be done first", who );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( Bp->m != pd->solvmatr->nodenbr || Bp->ld < Bp->m || Bp->n < 0 ||
+         ( Bp->flttype != PastixDouble && Bp->flttype != PastixFloat ) ) {
+        pa_error( "%s: Bp does not match the factorized matrix", who );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* the device steps on a permuted host RHS: upload, run, download */
+typedef int ( *pa_perm_op_t )( pastix_data_t *, void *, int32_t, double *, int64_t );
+
+static int
+pa_rhs_device_op( pastix_data_t *pd, pastix_rhs_t Bp, pa_perm_op_t op, void *arg )
+{
+    const pastix_int_t n = Bp->m, nrhs = Bp->n;
+    double *dB;
+    int rc;
+    if ( nrhs == 0 ) return PASTIX_SUCCESS;
+    dB = pa_dev_wget( pd, (size_t)n * nrhs * sizeof( double ) );
+    if ( !dB ) return PASTIX_ERR_OUTOFMEMORY;
+    rc = pa_rhs_upload( pd, Bp->b, Bp->flttype, n, nrhs, Bp->ld, dB );
+    if ( !rc ) rc = op( pd, arg, (int32_t)nrhs, dB, n );
+    if ( !rc ) rc = pa_dev_sync( pd );
+    if ( !rc ) rc = pa_rhs_download( pd, Bp->b, Bp->flttype, n, nrhs, Bp->ld, dB );
+    pa_dev_wput( pd, dB );
+    return rc;
+}
+
+typedef struct { int upper, transposed, unit; } pa_trsm_arg_t;
+
+static int
+pa_op_trsm( pastix_data_t *pd, void *arg, int32_t nrhs, double *dB, int64_t ld )
+{
+    const pa_trsm_arg_t *a = (const pa_trsm_arg_t *)arg;
+    return pa_dev_trsm_perm( pd, a->upper, a->transposed, a->unit, nrhs, dB, ld );
+}
+
+static int
+pa_op_diag( pastix_data_t *pd, void *arg, int32_t nrhs, double *dB, int64_t ld )
+{
+    (void)arg;
+    return pa_dev_diag_perm( pd, nrhs, dB, ld );
+}
+
+static int
+pa_op_solve( pastix_data_t *pd, void *arg, int32_t nrhs, double *dB, int64_t ld )
+{
+    return pa_dev_solve_perm( pd, *(const pastix_trans_t *)arg, nrhs, dB, ld );
+}
+
+/* Triangular solve on a permuted RHS (reference pastix_subtask_trsm,
+ * sopalin/pastix_task_solve.c:296-400).  Left side only: the reference does
+ * not handle the right side either (kernels/solve_ztrsmsp.c:326-340).  The
+ * Upper cases sweep the stored U^T and need an LU factorization. */
+int
+pastix_subtask_trsm( pastix_data_t *pd, pastix_side_t side, pastix_uplo_t uplo, pastix_trans_t trans,
+                     pastix_diag_t diag, pastix_rhs_t Bp )
+{
+    pa_trsm_arg_t a;
+    int rc;
+    if ( ( rc = pa_check_rhs( pd, Bp, "pastix_subtask_trsm" ) ) ) return rc;
+    if ( side != PastixLeft ) {
+        pa_error( "pastix_subtask_trsm: only the left side is handled" );
+        return PASTIX_ERR_NOTIMPLEMENTED;
+    }
+    if ( ( uplo != PastixLower && uplo != PastixUpper ) || ( diag != PastixUnit && diag != PastixNonUnit ) ||
+         ( trans != PastixNoTrans && trans != PastixTrans && trans != PastixConjTrans ) )
+        return PASTIX_ERR_BADPARAMETER;
+    if ( uplo == PastixUpper && pd->iparm[IPARM_FACTORIZATION] != PastixFactLU ) {
+        pa_error( "pastix_subtask_trsm: the upper factor is only stored by an LU factorization" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    a.upper      = ( uplo == PastixUpper );
+    a.transposed = ( trans != PastixNoTrans ); /* real arithmetic: ConjTrans == Trans */
+    a.unit       = ( diag == PastixUnit );
+    return pa_rhs_device_op( pd, Bp, pa_op_trsm, &a );
+}
+
+/* y <- D^{-1} y (reference pastix_subtask_diag, sopalin/pastix_task_solve.c:422-470) */
+int
+pastix_subtask_diag( pastix_data_t *pd, pastix_rhs_t Bp )
+{
+    int rc;
+    if ( ( rc = pa_check_rhs( pd, Bp, "pastix_subtask_diag" ) ) ) return rc;
+    return pa_rhs_device_op( pd, Bp, pa_op_diag, NULL );
+}
+
+/* Solve on an RHS already in the factorization numbering (reference
+ * sopalin/pastix_task_solve.c:512-707).  The FP32 sweeps of a mixed
+ * precision factorization run on an FP32 copy made on the device. */
+int
+pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t Bp )
+{
+    double t0;
+    int    rc;
+    if ( ( rc = pa_check_rhs( pd, Bp, "pastix_subtask_solve_adv" ) ) ) return rc;
+    if ( transA == PastixConjTrans ) transA = PastixTrans; /* real arithmetic */
+    if ( transA != PastixNoTrans && transA != PastixTrans ) return PASTIX_ERR_BADPARAMETER;
+    t0 = pa_wtime();
+    rc = pa_rhs_device_op( pd, Bp, pa_op_solve, &transA );
+    pd->dparm[DPARM_SOLV_TIME] = pa_wtime() - t0;
+    if ( !rc ) pd->steps |= STEP_SOLVE;
+    return rc;
+}
+
+int
+pastix_subtask_solve( pastix_data_t *pd, pastix_rhs_t Bp )
+{
+    if ( pd == NULL ) return PASTIX_ERR_BADPARAMETER;
+    return pastix_subtask_solve_adv( pd, (pastix_trans_t)pd->iparm[IPARM_TRANSPOSE_SOLVE], Bp );
 }
 
 int
@@ -746,67 +1106,160 @@ pastixDumpParam( const pastix_data_t *pd )
     for ( i = 0; i < DPARM_SIZE; i++ ) fprintf( stdout, "dparm[%d],%e\n", i, pd->dparm[i] );
 }
 
-/* Legacy one-call interface (reference sopalin/pastix.c:103-330): runs the
- * steps IPARM_START_TASK..IPARM_END_TASK on a CSC given by arrays. */
+/* Legacy one-call interface (reference sopalin/pastix.c:103-487): runs the
+ * steps IPARM_START_TASK..IPARM_END_TASK on a CSC given by arrays, advancing
+ * IPARM_START_TASK as each step completes.  The spm describing the caller's
+ * arrays belongs to the instance (no state shared between instances, as the
+ * reference's example/reentrant.c requires); b is kept between the solve
+ * and the refinement steps so that the refinement sees the original right
+ * hand side. */
 int
 pastix( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t n, pastix_int_t *colptr,
         pastix_int_t *rowptr, void *values, pastix_int_t *perm, pastix_int_t *invp, void *B,
         pastix_int_t nrhs, pastix_int_t *iparm, double *dparm )
 {
-    static spmatrix_t spm_store;
-    spmatrix_t *spm = &spm_store;
     pastix_data_t *pd;
-    int rc = PASTIX_SUCCESS;
-    pastix_int_t task;
+    spmatrix_t    *spm;
+    size_t         size;
+    int            rc;
 
     if ( iparm[IPARM_MODIFY_PARAMETER] == 0 ) {
         pastixInitParam( iparm, dparm );
         iparm[IPARM_MODIFY_PARAMETER] = 1;
         return PASTIX_SUCCESS;
     }
-    if ( iparm[IPARM_START_TASK] == PastixTaskInit || *pastix_data == NULL ) {
+    if ( iparm[IPARM_END_TASK] < PastixTaskInit ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskInit ) {
+        if ( *pastix_data != NULL ) pastixFinalize( pastix_data ); /* restart with new parameters */
         pastixInit( pastix_data, comm, iparm, dparm );
+        iparm[IPARM_START_TASK]++;
     }
+    if ( iparm[IPARM_END_TASK] < PastixTaskOrdering ) return PASTIX_SUCCESS;
     pd = *pastix_data;
-    memset( spm, 0, sizeof( *spm ) );
-    spm->mtxtype = ( iparm[IPARM_MTX_TYPE] == -1 ) ? SpmGeneral : (spm_mtxtype_t)iparm[IPARM_MTX_TYPE];
-    spm->flttype = ( iparm[IPARM_FLOAT] == -1 ) ? SpmDouble : (spm_coeftype_t)iparm[IPARM_FLOAT];
-    spm->fmttype = SpmCSC;
-    spm->baseval = colptr ? colptr[0] : 0;
-    spm->n = spm->gN = spm->nexp = spm->gNexp = n;
-    spm->nnz = spm->gnnz = colptr ? colptr[n] - colptr[0] : 0;
-    spm->dof     = 1;
-    spm->colptr  = colptr;
-    spm->rowptr  = rowptr;
-    spm->values  = values;
+    if ( pd == NULL ) {
+        pa_error( "pastix: the initialization step was not run before the following steps" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
 
-    for ( task = iparm[IPARM_START_TASK]; !rc && task <= iparm[IPARM_END_TASK]; task++ ) {
-        switch ( task ) {
-        case PastixTaskOrdering: {
-            pastix_order_t o;
+    /* the spm of the caller's arrays (rebuilt when they change) */
+    if ( iparm[IPARM_START_TASK] <= PastixTaskNumfact ) {
+        spm = pd->legacy_spm;
+        if ( spm != NULL && ( spm->n != n || spm->nnz != colptr[n] - colptr[0] || spm->colptr != colptr ||
+                              spm->rowptr != rowptr ) ) {
+            if ( pd->csc == spm ) pd->csc = NULL;
+            free( spm );
+            spm = pd->legacy_spm = NULL;
+        }
+        if ( spm == NULL ) {
+            if ( iparm[IPARM_FLOAT] == -1 ) {
+                pa_error( "pastix: iparm[IPARM_FLOAT] has to be set" );
+                return PASTIX_ERR_BADPARAMETER;
+            }
+            if ( iparm[IPARM_MTX_TYPE] == -1 ) {
+                pa_error( "pastix: iparm[IPARM_MTX_TYPE] has to be set" );
+                return PASTIX_ERR_BADPARAMETER;
+            }
+            if ( iparm[IPARM_DOF_NBR] < 1 ) {
+                pa_error( "pastix: variable dofs are not supported by the legacy interface" );
+                return PASTIX_ERR_BADPARAMETER;
+            }
+            spm = calloc( 1, sizeof( spmatrix_t ) );
+            if ( spm == NULL ) return PASTIX_ERR_ALLOC;
+            spm->mtxtype  = (spm_mtxtype_t)iparm[IPARM_MTX_TYPE];
+            spm->flttype  = (spm_coeftype_t)iparm[IPARM_FLOAT];
+            spm->fmttype  = SpmCSC;
+            spm->baseval  = colptr[0];
+            spm->n = spm->gN = spm->nexp = spm->gNexp = n;
+            spm->nnz = spm->gnnz = spm->nnzexp = spm->gnnzexp = colptr[n] - colptr[0];
+            spm->dof      = iparm[IPARM_DOF_NBR];
+            spm->layout   = SpmColMajor;
+            spm->colptr   = colptr;
+            spm->rowptr   = rowptr;
+            spm->values   = values;
+            spm->clustnbr = 1;
+            pd->legacy_spm = spm;
+        }
+        /* values given only at the numerical steps */
+        if ( spm->values == NULL ) spm->values = values;
+    }
+    else {
+        spm = pd->legacy_spm;
+    }
+    if ( spm == NULL ) return PASTIX_ERR_BADPARAMETER;
+    size = ( spm->flttype == SpmFloat ? sizeof( float ) : sizeof( double ) ) * (size_t)spm->n * ( nrhs > 0 ? nrhs : 1 );
+
+    if ( iparm[IPARM_START_TASK] == PastixTaskOrdering ) {
+        pastix_order_t o, *po = NULL;
+        if ( perm != NULL || invp != NULL ) {
             memset( &o, 0, sizeof( o ) );
             o.vertnbr = n;
             o.permtab = perm;
             o.peritab = invp;
-            rc = pastix_subtask_order( pd, spm, ( perm || invp ) ? &o : NULL );
-            if ( !rc && perm ) memcpy( perm, pd->ordemesh->permtab, n * sizeof( pastix_int_t ) );
-            if ( !rc && invp ) memcpy( invp, pd->ordemesh->peritab, n * sizeof( pastix_int_t ) );
-            break;
+            po        = &o;
         }
-        case PastixTaskSymbfact: rc = pastix_subtask_symbfact( pd ); break;
-        case PastixTaskAnalyze:
-            rc = pastix_subtask_reordering( pd );
-            if ( !rc ) rc = pastix_subtask_blend( pd );
-            break;
-        case PastixTaskNumfact: rc = pastix_task_numfact( pd, spm ); break;
-        case PastixTaskSolve:   rc = pastix_task_solve( pd, n, nrhs, B, n ); break;
-        case PastixTaskRefine:  break; /* refinement needs the original b, as in the reference */
-        case PastixTaskClean:   break;
-        default: break;
+        if ( ( rc = pastix_subtask_order( pd, spm, po ) ) ) return rc;
+        if ( perm ) memcpy( perm, pd->ordemesh->permtab, n * sizeof( pastix_int_t ) );
+        if ( invp ) memcpy( invp, pd->ordemesh->peritab, n * sizeof( pastix_int_t ) );
+        iparm[IPARM_START_TASK]++;
+    }
+    if ( iparm[IPARM_END_TASK] < PastixTaskSymbfact ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskSymbfact ) {
+        if ( ( rc = pastix_subtask_symbfact( pd ) ) ) return rc;
+        iparm[IPARM_START_TASK]++;
+    }
+    if ( iparm[IPARM_END_TASK] < PastixTaskAnalyze ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskAnalyze ) {
+        if ( ( rc = pastix_subtask_reordering( pd ) ) ) return rc;
+        if ( ( rc = pastix_subtask_blend( pd ) ) ) return rc;
+        iparm[IPARM_START_TASK]++;
+    }
+    if ( iparm[IPARM_END_TASK] < PastixTaskNumfact ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskNumfact ) {
+        if ( ( rc = pastix_task_numfact( pd, spm ) ) ) return rc;
+        iparm[IPARM_START_TASK]++;
+    }
+    if ( iparm[IPARM_END_TASK] < PastixTaskSolve ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskSolve ) {
+        free( pd->x0 ); pd->x0 = NULL;
+        free( pd->b );  pd->b  = NULL;
+        /* keep b for a refinement in this call */
+        if ( iparm[IPARM_END_TASK] > PastixTaskSolve ) {
+            if ( ( pd->b = malloc( size ) ) == NULL ) return PASTIX_ERR_ALLOC;
+            memcpy( pd->b, B, size );
+        }
+        if ( ( rc = pastix_task_solve( pd, spm->n, nrhs, B, spm->n ) ) ) return rc;
+        iparm[IPARM_START_TASK]++;
+        /* keep x0 for a refinement in a later call */
+        if ( iparm[IPARM_END_TASK] == PastixTaskSolve ) {
+            if ( ( pd->x0 = malloc( size ) ) == NULL ) return PASTIX_ERR_ALLOC;
+            memcpy( pd->x0, B, size );
         }
     }
-    if ( !rc && iparm[IPARM_END_TASK] >= PastixTaskClean ) {
+    if ( iparm[IPARM_END_TASK] < PastixTaskRefine ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskRefine ) {
+        void *rb = pd->b, *rx = pd->x0;
+        if ( rb == NULL && rx == NULL ) {
+            pa_error( "pastix: neither b nor x0 was kept for the refinement" );
+            return PASTIX_ERR_UNKNOWN;
+        }
+        if ( rb != NULL && rx != NULL ) {
+            pa_error( "pastix: both b and x0 were kept, which cannot happen" );
+            return PASTIX_ERR_UNKNOWN;
+        }
+        /* x0 kept from an earlier call: B holds the original b; b kept in
+         * this call: B holds x0 */
+        if ( rb == NULL ) rb = B;
+        else              rx = B;
+        if ( ( rc = pastix_task_refine( pd, spm->n, nrhs, rb, spm->n, rx, spm->n ) ) ) return rc;
+        iparm[IPARM_START_TASK]++;
+        if ( B != rx ) memcpy( B, rx, size );
+    }
+    free( pd->x0 ); pd->x0 = NULL;
+    free( pd->b );  pd->b  = NULL;
+    if ( iparm[IPARM_END_TASK] < PastixTaskClean ) return PASTIX_SUCCESS;
+    if ( iparm[IPARM_START_TASK] == PastixTaskClean ) {
         pastixFinalize( pastix_data );
+        iparm[IPARM_START_TASK]++;
     }
-    return rc;
+    return PASTIX_SUCCESS;
 }

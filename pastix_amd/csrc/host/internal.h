@@ -69,10 +69,14 @@ void pa_symbol_exit( symbol_matrix_t *symb );
 /* Grid / personal orderings (order.c) */
 int  pa_order_apply_level_order( pastix_order_t *order, pastix_int_t level_tasks2d,
                                  pastix_int_t width_tasks2d );
-int  pa_order_tree_from_rangtab( pastix_order_t *order, pastix_int_t n, const pastix_int_t *colptr,
-                                 const pastix_int_t *rows );
-int  pa_order_from_etree( pastix_order_t *order, pastix_int_t n,
-                          const pastix_int_t *colptr, const pastix_int_t *rows );
+/* Supernode partition of a bare permutation (amalgamate.c): fundamental
+ * supernodes of the postordered elimination tree (perm/invp postordered in
+ * place), then amalgamation under the fill budgets lvl_cblk / lvl_blas (in
+ * percent of nnz(L)); colptr/rows: the symmetric graph without diagonal. */
+int  pa_order_find_supernodes( pastix_order_t *order, pastix_int_t n, const pastix_int_t *colptr,
+                               const pastix_int_t *rows );
+int  pa_order_amalgamate( pastix_order_t *order, pastix_int_t n, const pastix_int_t *colptr,
+                          const pastix_int_t *rows, int ilu, pastix_int_t lvl_cblk, pastix_int_t lvl_blas );
 
 /* Symmetrised adjacency graph without diagonal, 0-based CSC. */
 typedef struct pa_graph_s {
@@ -136,6 +140,9 @@ void pa_valmap_exit( pa_valmap_t *map );
 /* -------------------------------------------------------------------------- */
 /* Main instance structure.                                                   */
 /* -------------------------------------------------------------------------- */
+/* Right-hand sides (reference common/pastixdata.h:155-168): allocated is -1
+ * until the vector is set, 0 when b is the caller's array (the permutation
+ * is applied in place), 1 when the library owns it. */
 struct pastix_rhs_s {
     int           allocated;
     int           flttype;
@@ -161,6 +168,16 @@ struct pastix_data_s {
     SolverMatrix      *solvmatr;
     struct pa_device_s *dev;
     int                valmap_ready;   /* the device holds the value map of csc */
+    /* op(A) of the running refinement: IPARM_TRANSPOSE_SOLVE for the API
+     * entry points (reference bcsc/bcsc_zspmv.c:638), or the transposition
+     * given to pastix_amd_refine_dev_adv (the adjoint solve) */
+    pastix_trans_t     refine_trans;
+    /* legacy pastix() interface: the spm built from the caller's arrays and
+     * the backups of b / x0 kept between its solve and refinement steps
+     * (reference sopalin/pastix.c:193-250, 363-420) */
+    spmatrix_t        *legacy_spm;
+    void              *b;
+    void              *x0;
 };
 
 #ifdef __cplusplus

@@ -448,47 +448,6 @@ pa_order_apply_level_order( pastix_order_t *order, pastix_int_t level_tasks2d,
     return PASTIX_SUCCESS;
 }
 
-/*
- * Partition for a permutation given without supernodes: the elimination tree
- * of the permuted graph (Liu's algorithm with path compression), one column
- * per supernode.  This stands in for the reference's orderFindSupernodes +
- * orderAmalgamate (order/pastix_subtask_order.c:368-381), which are not
- * restated; the resulting structure is valid but not amalgamated.
- */
-int
-pa_order_from_etree( pastix_order_t *o, pastix_int_t n, const pastix_int_t *colptr,
-                     const pastix_int_t *rows )
-{
-    pastix_int_t *parent, *ancestor, i, k;
-    free( o->rangtab );
-    free( o->treetab );
-    o->cblknbr = n;
-    o->rangtab = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    o->treetab = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    parent   = o->treetab;
-    ancestor = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    for ( k = 0; k < n; k++ ) {
-        pastix_int_t ok = o->peritab[k], e;
-        parent[k]   = -1;
-        ancestor[k] = -1;
-        for ( e = colptr[ok]; e < colptr[ok + 1]; e++ ) {
-            pastix_int_t r = o->permtab[rows[e]];
-            /* walk from r (< k) up to its root, compressing the path to k */
-            while ( r != -1 && r < k ) {
-                pastix_int_t next = ancestor[r];
-                ancestor[r] = k;
-                if ( next == -1 ) {
-                    parent[r] = k;
-                }
-                r = next;
-            }
-        }
-    }
-    for ( i = 0; i <= n; i++ ) o->rangtab[i] = i;
-    free( ancestor );
-    return PASTIX_SUCCESS;
-}
-
 /* -------------------------------------------------------------------------- */
 /* Ordering files (the reference's text format, order/order_io.c:30-255):      */
 /*   version (0 or 1), cblknbr, vertnbr, rangtab[cblknbr+1], permtab[vertnbr], */
@@ -504,43 +463,6 @@ pa_read_int( FILE *f, pastix_int_t *v )
     if ( fscanf( f, "%lld", &x ) != 1 ) return 0;
     *v = (pastix_int_t)x;
     return 1;
-}
-
-/* Supernode tree of a partition given without one: the father of cblk k is
- * the cblk holding the elimination-tree parent of k's last column. */
-int
-pa_order_tree_from_rangtab( pastix_order_t *o, pastix_int_t n, const pastix_int_t *colptr, const pastix_int_t *rows )
-{
-    pastix_int_t *parent, *ancestor, *col2cblk, i, k, b = o->baseval;
-    parent   = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    ancestor = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    col2cblk = malloc( ( n + 1 ) * sizeof( pastix_int_t ) );
-    for ( k = 0; k < n; k++ ) {
-        pastix_int_t ok = o->peritab[k] - b, e;
-        parent[k]   = -1;
-        ancestor[k] = -1;
-        for ( e = colptr[ok]; e < colptr[ok + 1]; e++ ) {
-            pastix_int_t r = o->permtab[rows[e]] - b;
-            while ( r != -1 && r < k ) {
-                pastix_int_t next = ancestor[r];
-                ancestor[r] = k;
-                if ( next == -1 ) parent[r] = k;
-                r = next;
-            }
-        }
-    }
-    for ( k = 0; k < o->cblknbr; k++ )
-        for ( i = o->rangtab[k] - b; i < o->rangtab[k + 1] - b; i++ ) col2cblk[i] = k;
-    free( o->treetab );
-    o->treetab = malloc( ( o->cblknbr + 1 ) * sizeof( pastix_int_t ) );
-    for ( k = 0; k < o->cblknbr; k++ ) {
-        pastix_int_t last = o->rangtab[k + 1] - b - 1, p = parent[last];
-        o->treetab[k] = ( p < 0 ) ? -1 : col2cblk[p] + b;
-    }
-    free( parent );
-    free( ancestor );
-    free( col2cblk );
-    return PASTIX_SUCCESS;
 }
 
 int

@@ -295,7 +295,12 @@ pa_symbol_nnz( const symbol_matrix_t *s )
     ( ( ( m ) < ( n ) ) ? ( 0.5 * (double)( m ) * ( (double)( m ) * ( (double)( n ) - ( 1. / 3. ) * ( m ) ) - (double)( n ) ) + ( 1. / 6. ) * ( m ) ) \
                         : ( 0.5 * (double)( n ) * ( (double)( n ) * ( (double)( m ) - ( 1. / 3. ) * ( n ) ) - (double)( m ) ) + ( 1. / 6. ) * ( n ) ) )
 
+/* FMULS_SYTRF of the reference (common/flops.h:103): FMULS_POTRF with
+ * 10/3 n in place of n/3 (the scaling by D), FADDS as POTRF */
+#define PA_FMULS_SYTRF( n ) ( (double)( n ) * ( ( ( 1. / 6. ) * (double)( n ) + 0.5 ) * (double)( n ) + ( 10. / 3. ) ) )
+
 double pa_flops_potrf( double n ) { return PA_FMULS_POTRF( n ) + PA_FADDS_POTRF( n ); }
+double pa_flops_sytrf( double n ) { return PA_FMULS_SYTRF( n ) + PA_FADDS_POTRF( n ); }
 double pa_flops_getrf( double n ) { return PA_FMULS_GETRF( n, n ) + PA_FADDS_GETRF( n, n ); }
 /* right-side TRSM of an M x N block: M*N*(N+1) (FMULS_TRSM counted twice,
  * as in the reference's FLOPS_DTRSM) */
@@ -321,6 +326,13 @@ pa_symbol_flops( const symbol_matrix_t *s, int factotype )
         if ( factotype == PastixFactLU ) {
             total += pa_flops_getrf( N );
             if ( M > 0 ) total += 2. * pa_flops_trsm( M, N ) + pa_flops_gemm( M, M, N );
+        }
+        else if ( factotype == PastixFactLDLT || factotype == PastixFactLDLH ) {
+            /* reference flops_dsytrf_{diag,trsm,update}
+             * (symbol/symbol_cost_flops.c:56-158): the D scaling adds N M
+             * to the TRSM and M^2 to the update */
+            total += pa_flops_sytrf( N );
+            if ( M > 0 ) total += pa_flops_trsm( M, N ) + N * M + pa_flops_syrk( N, M ) + M * M;
         }
         else {
             total += pa_flops_potrf( N );

@@ -171,3 +171,25 @@ class uplo:
 class diag:
     NonUnit = 131
     Unit = 132
+
+
+class dir:
+    """pastix_dir_t (reference include/pastix/api.h)."""
+    Forward = 391
+    Backward = 392
+
+
+class error:
+    """pastix_error_t return codes (reference include/pastix/api.h:375-388)."""
+    SUCCESS = 0
+    UNKNOWN = 1
+    ALLOC = 2
+    NOTIMPLEMENTED = 3
+    OUTOFMEMORY = 4
+    THREAD = 5
+    INTERNAL = 6
+    BADPARAMETER = 7
+    FILE = 8
+    INTEGER_TYPE = 9
+    IO = 10
+    MPI = 11

@@ -105,6 +105,24 @@ class order(object):
         it[perm] = np.arange(n)
         return o
 
+    @staticmethod
+    def from_partition(perm, rangtab, treetab):
+        """A personal ordering with its supernode partition and tree
+        (perm[old] = new, rangtab[cblknbr + 1], treetab[cblknbr], -1 for a
+        root): the analysis uses the partition as given."""
+        perm = np.asarray(perm, dtype=np.int64)
+        rangtab = np.asarray(rangtab, dtype=np.int64)
+        treetab = np.asarray(treetab, dtype=np.int64)
+        n, c = len(perm), len(rangtab) - 1
+        o = order()
+        lib = _lib.load()
+        _check(lib.pastixOrderAlloc(C.byref(o.o), n, c), "pastixOrderAlloc")
+        np.ctypeslib.as_array(o.o.permtab, (n,))[:] = perm
+        np.ctypeslib.as_array(o.o.peritab, (n,))[perm] = np.arange(n)
+        np.ctypeslib.as_array(o.o.rangtab, (c + 1,))[:] = rangtab
+        np.ctypeslib.as_array(o.o.treetab, (c,))[:] = treetab
+        return o
+
     def arrays(self):
         n, c = self.o.vertnbr, self.o.cblknbr
         return (np.ctypeslib.as_array(self.o.permtab, (n,)).copy(),
@@ -158,16 +176,27 @@ def task_numfact(handle, spm):
     _check(_lib.load().pastix_task_numfact(handle, spm.ref), "pastix_task_numfact")
 
 
+def _rhs_dtype(spm):
+    """Arithmetic of the right-hand sides: that of the matrix (FP32 for a
+    single precision spm, FP64 otherwise), as the reference's solve reads it."""
+    return np.float32 if spm.values.dtype == np.float32 else np.float64
+
+
 def task_solve(handle, spm, x):
-    """In-place solve of the host array x (n or n x nrhs, FP64, column major)."""
-    x = np.asfortranarray(x)
+    """In-place solve of the host array x (n or n x nrhs, column major, in
+    the matrix's arithmetic); returns x (a converted copy if x had another
+    type or layout)."""
+    x = np.asfortranarray(x, dtype=_rhs_dtype(spm))
     nrhs = 1 if x.ndim == 1 else x.shape[1]
     _check(_lib.load().pastix_task_solve(handle, spm.n, nrhs, _ptr(x), spm.n), "pastix_task_solve")
     return x
 
 
 def task_refine(handle, spm, b, x):
-    b = np.asfortranarray(b, dtype=np.float64)
+    """Refine x (in place, same arithmetic and layout as task_solve) for b."""
+    b = np.asfortranarray(b, dtype=_rhs_dtype(spm))
+    if x.dtype != b.dtype or not x.flags.f_contiguous:
+        raise ValueError("x must be a column-major array of the matrix's arithmetic")
     nrhs = 1 if b.ndim == 1 else b.shape[1]
     _check(_lib.load().pastix_task_refine(handle, spm.n, nrhs, _ptr(b), spm.n, _ptr(x), spm.n),
            "pastix_task_refine")

@@ -8,7 +8,7 @@ from oracle import pyoracle as orc
 
 
 class Problem(object):
-    def __init__(self, A, grid, factotype, mixed=False, blocksize=None, analyze_only=False):
+    def __init__(self, A, grid, factotype, mixed=False, blocksize=None, analyze_only=False, order=None):
         self.A = sp.csc_matrix(A)
         self.n = A.shape[0]
         self.ft = factotype
@@ -25,7 +25,7 @@ class Problem(object):
         self.h = pa.init(self.iparm, self.dparm)
         mt = pa.mtxtype.General if self.lu else pa.mtxtype.Symmetric
         self.spm = pa.spmatrix(self.A, mtxtype=mt)
-        self.order = pa.order.grid(*grid) if grid is not None else None
+        self.order = pa.order.grid(*grid) if grid is not None else order
         pa.task_analyze(self.h, self.spm, self.order)
         self.cblk, self.blok, self.perm, self.invp, self.dims = pa.get_structure(self.h)
         self.S = orc.Structure(self.cblk, self.blok)

@@ -1,0 +1,410 @@
+"""The drop-in boundary on the GPU: the reference's solve and refinement
+subtasks, the legacy one-call interface, parameter changes between the
+analysis and the factorization, every refinement method, a single precision
+matrix, and the 2D-task GEMM entry point gpublok_{d,s}gemmsp against a dense
+brute-force update.  Everything goes through the C-ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import pastix_amd as pa
+from pastix_amd import _lib, problems
+from oracle import pyoracle as orc
+from helpers import Problem
+
+pytestmark = pytest.mark.gpu
+
+LEFT = 141
+UPPER, LOWER = 121, 122
+NONUNIT, UNIT = 131, 132
+NOTRANS, TRANS = 111, 112
+
+
+def _rhs_handle():
+    Bp = C.c_void_p()
+    assert _lib.load().pastixRhsInit(C.byref(Bp)) == 0
+    return Bp
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("case", ["llt", "lu", "lu-trans", "llt-mixed", "lu-mixed"])
+def test_subtasks_compose_the_solve(case):
+    """applyorder(Forward) + trsm(first sweep) + trsm(second sweep) +
+    applyorder(Backward) gives the solution of pastix_task_solve, and so do
+    pastix_subtask_solve / solve_adv (reference sopalin/pastix_task_solve.c:
+    543-649 chooses uplo/trans/diag per factorization the same way)."""
+    lu = case.startswith("lu")
+    tr = case.endswith("trans")
+    mixed = case.endswith("mixed")
+    grid = (10, 9, 8)
+    print("case", case, flush=True)
+    A = problems.convection_diffusion_27(*grid) if lu else problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LU if lu else pa.factotype.LLT, mixed=mixed)
+    lib = _lib.load()
+    try:
+        P.iparm[pa.iparm.transpose_solve] = TRANS if tr else NOTRANS
+        n, nrhs = P.n, 3
+        rng = np.random.default_rng(2)
+        b = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+        xref = pa.task_solve(P.h, P.spm, b.copy(order="F"))
+        Aop = sp.csc_matrix(A.T if tr else A)
+        tol = 1e-12 if not mixed else 1e-4
+        assert _rel(xref, spla.spsolve(Aop, b)) < tol
+        # step by step
+        x = b.copy(order="F")
+        Bp = _rhs_handle()
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Forward, n, nrhs, C.c_void_p(x.ctypes.data), n, Bp) == 0
+        if lu and tr:
+            steps = [(UPPER, TRANS, NONUNIT), (LOWER, TRANS, UNIT)]
+        elif lu:
+            steps = [(LOWER, NOTRANS, UNIT), (UPPER, NOTRANS, NONUNIT)]
+        else:
+            steps = [(LOWER, NOTRANS, NONUNIT), (LOWER, TRANS, NONUNIT)]
+        for uplo, trans, diag in steps:
+            assert lib.pastix_subtask_trsm(P.h, LEFT, uplo, trans, diag, Bp) == 0
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Backward, n, nrhs, C.c_void_p(x.ctypes.data), n, Bp) == 0
+        assert _rel(x, xref) < (1e-13 if not mixed else 1e-6)
+        # subtask_solve on the permuted vector
+        y = b.copy(order="F")
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Forward, n, nrhs, C.c_void_p(y.ctypes.data), n, Bp) == 0
+        assert lib.pastix_subtask_solve(P.h, Bp) == 0
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Backward, n, nrhs, C.c_void_p(y.ctypes.data), n, Bp) == 0
+        assert _rel(y, xref) < (1e-13 if not mixed else 1e-6)
+        # refinement subtask on permuted b and x
+        bb = b.copy(order="F")
+        xx = y.copy(order="F")
+        Xp = _rhs_handle()
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Forward, n, nrhs, C.c_void_p(bb.ctypes.data), n, Bp) == 0
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Forward, n, nrhs, C.c_void_p(xx.ctypes.data), n, Xp) == 0
+        assert lib.pastix_subtask_refine(P.h, Bp, Xp) == 0
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Backward, n, nrhs, C.c_void_p(bb.ctypes.data), n, Bp) == 0
+        assert lib.pastix_subtask_applyorder(P.h, pa.dir.Backward, n, nrhs, C.c_void_p(xx.ctypes.data), n, Xp) == 0
+        assert np.array_equal(bb, b)
+        for r in range(nrhs):
+            assert problems.backward_error(Aop, xx[:, r], b[:, r]) <= 1e-12
+        # wrong sizes are refused: a handle describing fewer rows
+        h = C.cast(Bp, C.POINTER(_lib.pastix_rhs_s)).contents
+        h.allocated, h.flttype, h.m, h.n, h.ld, h.b = 0, 3, n - 1, 1, n - 1, x.ctypes.data
+        assert lib.pastix_subtask_solve_adv(P.h, NOTRANS, Bp) == pa.error.BADPARAMETER
+        assert lib.pastix_subtask_trsm(P.h, LEFT, LOWER, NOTRANS, NONUNIT, Bp) == pa.error.BADPARAMETER
+        h.b = None
+        if not lu:
+            # the upper factor exists for LU only
+            h.m, h.ld, h.b = n, n, x.ctypes.data
+            assert lib.pastix_subtask_trsm(P.h, LEFT, UPPER, NOTRANS, NONUNIT, Bp) == pa.error.BADPARAMETER
+            h.b = None
+        assert lib.pastix_subtask_trsm(P.h, 142, LOWER, NOTRANS, NONUNIT, Xp) == pa.error.NOTIMPLEMENTED
+        lib.pastixRhsFinalize(Bp)
+        lib.pastixRhsFinalize(Xp)
+    finally:
+        P.close()
+
+
+def test_subtask_diag_divides_by_the_diagonal_of_the_factor():
+    """pastix_subtask_diag divides row j by the diagonal entry of its L
+    diagonal block (reference kernels/solve_ztrsmsp.c:664-716)."""
+    grid = (8, 7, 6)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT)
+    lib = _lib.load()
+    try:
+        n = P.n
+        L = pa.get_coeftab(P.h, 0)
+        d = np.empty(n)
+        for k in range(P.S.cblknbr):
+            f, l, fb, st, off = P.cblk[k]
+            nc = l - f + 1
+            d[f:l + 1] = L[off + np.arange(nc) * (nc + 1)]
+        rng = np.random.default_rng(5)
+        y = np.asfortranarray(rng.uniform(-1, 1, (n, 2)))
+        y0 = y.copy()
+        Bp = _rhs_handle()
+        h = C.cast(Bp, C.POINTER(_lib.pastix_rhs_s)).contents
+        h.allocated, h.flttype, h.m, h.n, h.ld, h.b = 0, 3, n, 2, n, y.ctypes.data
+        assert lib.pastix_subtask_diag(P.h, Bp) == 0
+        assert _rel(y, y0 / d[:, None]) < 1e-15
+        h.b = None
+        lib.pastixRhsFinalize(Bp)
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("refine", [pa.refine.GMRES, pa.refine.CG, pa.refine.SR, pa.refine.BiCGSTAB],
+                         ids=["gmres", "cg", "sr", "bicgstab"])
+@pytest.mark.parametrize("lu", [False, True], ids=["llt", "lu"])
+def test_every_refinement_method_reaches_1e12_from_fp32_factors(refine, lu):
+    """GMRES, CG (SPD only), simple refinement and BiCGSTAB (reference
+    refinement/z_refine_bicgstab.c:49-230) on FP32 factors."""
+    if lu and refine == pa.refine.CG:
+        pytest.skip("CG needs an SPD matrix")
+    grid = (14, 13, 12)
+    A = problems.convection_diffusion_27(*grid) if lu else problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LU if lu else pa.factotype.LLT, mixed=True)
+    try:
+        P.iparm[pa.iparm.refinement] = refine
+        xt = problems.uniform(P.n, seed=0)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        assert problems.backward_error(A, x, b) > 1e-10
+        pa.task_refine(P.h, P.spm, b, x)
+        assert P.dparm[pa.dparm.relative_error] <= 1e-12
+        assert problems.backward_error(A, x, b) <= 1e-12
+        assert P.iparm[pa.iparm.nbiter] >= 1
+    finally:
+        P.close()
+
+
+def test_transposed_solve_and_refinement():
+    """IPARM_TRANSPOSE_SOLVE = Trans on an LU factorization in FP32: solve
+    and refinement both use A^T (reference bcsc/bcsc_zspmv.c:638)."""
+    grid = (12, 11, 10)
+    A = problems.convection_diffusion_27(*grid)
+    P = Problem(A, grid, pa.factotype.LU, mixed=True)
+    try:
+        P.iparm[pa.iparm.transpose_solve] = TRANS
+        At = sp.csc_matrix(A.T)
+        xt = problems.uniform(P.n, seed=4)
+        b = At @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        pa.task_refine(P.h, P.spm, b, x)
+        assert problems.backward_error(At, x, b) <= 1e-12
+        assert problems.backward_error(sp.csc_matrix(A), x, b) > 1e-6
+    finally:
+        P.close()
+
+
+def _csc_arrays(A):
+    A = sp.csc_matrix(A)
+    A.sort_indices()
+    return (np.ascontiguousarray(A.indptr, dtype=np.int64), np.ascontiguousarray(A.indices, dtype=np.int64),
+            np.ascontiguousarray(A.data, dtype=np.float64))
+
+
+def test_legacy_pastix_one_call_mixed_precision_with_refinement():
+    """pastix() from the ordering to the cleaning step, FP32 factorization,
+    refinement of the kept b (reference sopalin/pastix.c:363-460), with a
+    personal permutation; IPARM_START_TASK advances past every step."""
+    grid = (12, 12, 12)
+    A = problems.laplacian(*grid)
+    lower = sp.tril(A).tocsc()
+    colptr, rows, vals = _csc_arrays(lower)
+    n = A.shape[0]
+    lib = _lib.load()
+    ip = np.zeros(_lib.IPARM_SIZE, dtype=np.int64)
+    dp = np.zeros(_lib.DPARM_SIZE, dtype=np.float64)
+    pd = C.c_void_p()
+    P = lambda a: C.c_void_p(a.ctypes.data) if a is not None else None
+    # first call: defaults only
+    assert lib.pastix(C.byref(pd), 0, n, P(colptr), P(rows), P(vals), None, None, None, 1, P(ip), P(dp)) == 0
+    assert ip[pa.iparm.modify_parameter] == 1 and not pd.value
+    ip[pa.iparm.factorization] = pa.factotype.LLT
+    ip[pa.iparm.mixed] = 1
+    ip[pa.iparm.float] = 3
+    ip[pa.iparm.mtx_type] = pa.mtxtype.Symmetric
+    ip[pa.iparm.verbose] = pa.verbose.Not
+    ip[pa.iparm.start_task] = 0
+    ip[pa.iparm.end_task] = 6   # up to the refinement, keep the instance
+    perm = np.ascontiguousarray(orc_grid_perm(grid), dtype=np.int64)
+    invp = np.empty_like(perm)
+    invp[perm] = np.arange(n)
+    xt = problems.uniform(n, seed=0)
+    b = A @ xt
+    x = b.copy()
+    rc = lib.pastix(C.byref(pd), 0, n, P(colptr), P(rows), P(vals), P(perm), P(invp), P(x), 1, P(ip), P(dp))
+    assert rc == 0
+    assert ip[pa.iparm.start_task] == 7
+    assert problems.backward_error(A, x, b) <= 1e-12
+    assert dp[pa.dparm.relative_error] <= 1e-12
+    # second instance in two calls: solve only, then refine (x0 kept)
+    pd2 = C.c_void_p()
+    ip2 = ip.copy()
+    dp2 = dp.copy()
+    ip2[pa.iparm.start_task], ip2[pa.iparm.end_task] = 0, 5
+    x2 = b.copy()
+    assert lib.pastix(C.byref(pd2), 0, n, P(colptr), P(rows), P(vals), None, None, P(x2), 1, P(ip2), P(dp2)) == 0
+    assert ip2[pa.iparm.start_task] == 6
+    assert problems.backward_error(A, x2, b) > 1e-10   # FP32 factors alone
+    x3 = b.copy()
+    ip2[pa.iparm.end_task] = 7
+    assert lib.pastix(C.byref(pd2), 0, n, P(colptr), P(rows), P(vals), None, None, P(x3), 1, P(ip2), P(dp2)) == 0
+    assert problems.backward_error(A, x3, b) <= 1e-12
+    assert not pd2.value   # cleaned
+    # both instances held their own spm: the first one still solves
+    ip[pa.iparm.start_task], ip[pa.iparm.end_task] = 5, 7
+    x4 = b.copy()
+    assert lib.pastix(C.byref(pd), 0, n, P(colptr), P(rows), P(vals), None, None, P(x4), 1, P(ip), P(dp)) == 0
+    assert problems.backward_error(A, x4, b) <= 1e-12
+    assert not pd.value
+
+
+def orc_grid_perm(grid):
+    """perm[old] = new of the grid nested dissection (pastixOrderGrid)."""
+    o = pa.order.grid(*grid)
+    perm, invp, rang, tree = o.arrays()
+    return perm
+
+
+@pytest.mark.parametrize("first,second", [(pa.factotype.LLT, pa.factotype.LU), (pa.factotype.LU, pa.factotype.LLT)],
+                         ids=["llt-then-lu", "lu-then-llt"])
+def test_factorization_kind_changed_after_the_analysis(first, second):
+    """IPARM_FACTORIZATION is read again at each numerical factorization (as
+    the reference does, sopalin/pastix_task_sopalin.c:303,509): analysing for
+    one kind and factorizing with the other rebuilds the device schedule, and
+    the factors equal the oracle's for the new kind."""
+    grid = (9, 8, 7)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, first, analyze_only=True)
+    try:
+        P.iparm[pa.iparm.factorization] = second
+        P.lu = (second == pa.factotype.LU)
+        pa.task_numfact(P.h, P.spm)
+        Lo, Uo, npiv = P.oracle_factors()
+        Lg, Ug = P.gpu_factors()
+        if P.lu:
+            assert _rel(Lg, Lo) < 1e-12 and _rel(Ug, Uo) < 1e-12
+        else:
+            m = P.lower_mask()
+            assert _rel(Lg[m], Lo[m]) < 1e-12
+        xt = problems.uniform(P.n, seed=1)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        assert problems.backward_error(A, x, b) <= 1e-14
+        # and back, with the mixed precision switched on in between
+        P.iparm[pa.iparm.factorization] = first
+        P.iparm[pa.iparm.mixed] = 1
+        pa.task_numfact(P.h, P.spm)
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        pa.task_refine(P.h, P.spm, b, x)
+        assert problems.backward_error(A, x, b) <= 1e-12
+    finally:
+        P.close()
+
+
+def test_single_precision_matrix_solves_in_single_precision():
+    """An SpmFloat matrix: FP32 factors, FP32 right-hand sides through
+    pastix_task_solve / pastix_task_refine (the reference's solve takes B in
+    the matrix's arithmetic)."""
+    grid = (10, 10, 10)
+    A = problems.laplacian(*grid).astype(np.float32)
+    ip, dp = pa.initParam()
+    ip[pa.iparm.factorization] = pa.factotype.LLT
+    ip[pa.iparm.verbose] = pa.verbose.Not
+    h = pa.init(ip, dp)
+    try:
+        spm = pa.spmatrix(A, mtxtype=pa.mtxtype.Symmetric)
+        pa.task_analyze(h, spm, pa.order.grid(*grid))
+        pa.task_numfact(h, spm)
+        xt = problems.uniform(A.shape[0], seed=0).astype(np.float32)
+        b = (A @ xt).astype(np.float32)
+        x = pa.task_solve(h, spm, b.copy())
+        assert x.dtype == np.float32
+        A64 = sp.csc_matrix(A, dtype=np.float64)
+        assert problems.backward_error(A64, x.astype(np.float64), b.astype(np.float64)) < 1e-6
+        pa.task_refine(h, spm, b, x)
+        assert problems.backward_error(A64, x.astype(np.float64), b.astype(np.float64)) < 1e-6
+    finally:
+        pa.finalize(h)
+
+
+def _host_structs(P):
+    from test_gpu_abi import host_structs
+    return host_structs(P)
+
+
+def _dense_target_map(P, F):
+    """(row, col) -> coeftab index of cblk F's tiles."""
+    f, l, fb, stride, off = [int(v) for v in P.cblk[F]]
+    nc = l - f + 1
+    m = {}
+    for b in range(fb, int(P.S.cblkx[F + 1, 2])):
+        fr, lr, _, _, coef = [int(v) for v in P.blok[b]]
+        M = lr - fr + 1
+        for r in range(fr, lr + 1):
+            for c in range(nc):
+                m[(r, f + c)] = off + coef + (r - fr) + c * M
+    return m
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["d", "s"])
+def test_gpublok_gemmsp_matches_dense_update(dtype):
+    """gpublok_{d,s}gemmsp (reference kernels/gpu_zgemmsp.c:486, 2D tasks):
+    for a pair of block groups (blok_mk facing cblk_m, blok_nk facing cblk_n)
+    of a cblk, C -= A B^T lands in the blocks of cblk_n starting at blok_mn;
+    checked against the update computed densely from the rows of both groups."""
+    import torch
+    grid = (9, 8, 7)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT, blocksize=(6, 12), analyze_only=True)
+    lib = _lib.load()
+    fn = lib.gpublok_dgemmsp if dtype == np.float64 else lib.gpublok_sgemmsp
+    try:
+        cb, bl = _host_structs(P)
+        rng = np.random.default_rng(8)
+        L = rng.uniform(-1, 1, P.S.coefnbr).astype(dtype)
+        esz = np.dtype(dtype).itemsize
+        checked = 0
+        for k in range(P.S.cblknbr - 1, -1, -1):
+            f, l, fb, stride, off = [int(v) for v in P.cblk[k]]
+            nc = l - f + 1
+            lb = int(P.S.cblkx[k + 1, 2])
+            groups, b = [], fb + 1
+            while b < lb:
+                e = b
+                while e + 1 < lb and P.blok[e + 1, 3] == P.blok[b, 3]:
+                    e += 1
+                groups.append((b, e))
+                b = e + 1
+            if len(groups) < 2:
+                continue
+            for gi in range(len(groups)):
+                for gj in range(gi + 1):
+                    (mb, me), (nb, ne) = groups[gi], groups[gj]
+                    Fn = int(P.blok[nb, 3])
+                    # the block of Fn holding the first row of group gi
+                    r0 = int(P.blok[mb, 0])
+                    fbF, lbF = int(P.S.cblkx[Fn, 2]), int(P.S.cblkx[Fn + 1, 2])
+                    mn = next(c for c in range(fbF, lbF) if P.blok[c, 0] <= r0 <= P.blok[c, 1])
+                    dL = torch.from_numpy(L.copy()).cuda()
+                    base = dL.data_ptr()
+                    Aptr = base + (off + int(P.blok[mb, 4])) * esz
+                    Bptr = base + (off + int(P.blok[nb, 4])) * esz
+                    Cptr = base + (int(P.cblk[Fn, 4]) + int(P.blok[mn, 4])) * esz
+                    fl = fn(TRANS, C.c_void_p(C.addressof(cb) + k * C.sizeof(_lib.SolverCblk)),
+                            C.c_void_p(C.addressof(cb) + Fn * C.sizeof(_lib.SolverCblk)),
+                            mb - fb, nb - fb, mn - fbF, C.c_void_p(Aptr), C.c_void_p(Bptr), C.c_void_p(Cptr),
+                            None, None)
+                    torch.cuda.synchronize()
+                    assert fl > 0
+                    got = dL.cpu().numpy()
+                    # dense brute force over the rows of both groups
+                    ref = L.astype(np.float64)
+                    tmap = _dense_target_map(P, Fn)
+                    arows, brows = [], []
+                    for bb in range(mb, me + 1):
+                        fr, lr, _, _, coef = [int(v) for v in P.blok[bb]]
+                        M = lr - fr + 1
+                        for r in range(fr, lr + 1):
+                            arows.append((r, off + coef + (r - fr) + np.arange(nc) * M))
+                    for bb in range(nb, ne + 1):
+                        fr, lr, _, _, coef = [int(v) for v in P.blok[bb]]
+                        M = lr - fr + 1
+                        for r in range(fr, lr + 1):
+                            brows.append((r, off + coef + (r - fr) + np.arange(nc) * M))
+                    Lf = L.astype(np.float64)
+                    for (ra, ia) in arows:
+                        for (rb, ib) in brows:
+                            ref[tmap[(ra, rb)]] -= Lf[ia] @ Lf[ib]
+                    tol = 1e-13 if dtype == np.float64 else 3e-5
+                    assert np.abs(got - ref).max() / np.abs(ref).max() < tol, (k, gi, gj)
+                    checked += 1
+                    if checked >= 12:
+                        return
+        assert checked > 0
+    finally:
+        P.close()

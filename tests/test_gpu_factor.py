@@ -202,3 +202,42 @@ def test_fp64_factors_match_oracle_on_graded_anisotropic_coefficients(ft):
     finally:
         P.close()
 
+
+
+def test_orsirr_fixture_with_a_personal_ordering():
+    """The reference's LU test matrix (test/matrix/orsirr.rua, n = 1030) with
+    a reverse Cuthill-McKee permutation given as a bare personal ordering:
+    the analysis finds and amalgamates the supernodes (reference
+    order/pastix_subtask_order.c:362-381); the FP64 LU factors equal the
+    oracle's on that structure, and an FP32 factorization refined by FP64
+    GMRES reaches backward error 1e-12."""
+    import os
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    from test_cpu_oracle import read_rua
+    A = read_rua(os.path.join(os.path.dirname(__file__), "golden", "orsirr.rua"))
+    pat = (abs(A) + abs(A.T)).tocsr()
+    rcm = reverse_cuthill_mckee(pat, symmetric_mode=True).astype(np.int64)
+    perm = np.empty_like(rcm)
+    perm[rcm] = np.arange(len(rcm))
+    print("case orsirr LU FP64", flush=True)
+    P = Problem(A, None, pa.factotype.LU, order=pa.order.from_permutation(perm))
+    try:
+        assert P.S.cblknbr < P.n // 4   # real supernodes, not one column each
+        Lo, Uo, npiv = P.oracle_factors()
+        Lg, Ug = P.gpu_factors()
+        assert npiv == P.iparm[pa.iparm.static_pivoting]
+        assert np.abs(Lg - Lo).max() / np.abs(Lo).max() <= 1e-12
+        assert np.abs(Ug - Uo).max() / np.abs(Uo).max() <= 1e-12
+        b = A @ problems.uniform(P.n, seed=0)
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        assert problems.backward_error(A, x, b) <= 1e-14
+    finally:
+        P.close()
+    print("case orsirr LU FP32 + GMRES", flush=True)
+    P = Problem(A, None, pa.factotype.LU, mixed=True, order=pa.order.from_permutation(perm))
+    try:
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        pa.task_refine(P.h, P.spm, b, x)
+        assert problems.backward_error(A, x, b) <= 1e-12
+    finally:
+        P.close()

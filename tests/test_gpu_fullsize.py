@@ -98,3 +98,86 @@ def test_batch_of_256_uneven_systems_full_size():
     for i in (0, 100, 255):
         ref = spla.spsolve(sp.csc_matrix(mats[i]), rhs[i])
         assert np.abs(xs[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_laplacian_100_fp32_ir_reaches_1e12_and_fp32_factors_match_the_oracle():
+    """The headline configuration: 100^3 Laplacian, FP32 factorization on the
+    GPU (IPARM_MIXED, reference sopalin/pastix_task_sopalin.c:272) refined in
+    FP64 to ||r||/||b|| <= 1e-12 (bcsc/bvec_zcompute.c:1036-1058); the FP32
+    factors equal the FP32 oracle's (oracle_spotrf, level-parallel driver)
+    within the FP32 rounding of a 1e9-entry factorization, and both are
+    within the same bound of the FP64 oracle's factors."""
+    grid = (100, 100, 100)
+    print("case laplacian 100^3 LL^t FP32+IR", flush=True)
+    A = problems.laplacian(*grid)
+    P = Problem(A, grid, pa.factotype.LLT, mixed=True)
+    try:
+        n = P.n
+        xt = problems.uniform(n, seed=0)
+        b = A @ xt
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        berr0 = problems.backward_error(A, x, b)
+        pa.task_refine(P.h, P.spm, b, x)
+        berr = problems.backward_error(A, x, b)
+        print("fp32 solve berr %.2e -> refined %.2e in %d iterations" % (berr0, berr, P.iparm[pa.iparm.nbiter]),
+              flush=True)
+        assert berr0 > 1e-10 and berr <= 1e-12
+        assert np.abs(x - xt).max() <= 1e-9
+        Lg = pa.get_coeftab(P.h, 0)
+        mask = P.lower_mask()
+        Ls, _, npiv, sec = orc.factorize_parallel(P.S, P.A, P.perm, P.crit, _threads(), use_blas=True,
+                                                  dtype=np.float32)
+        assert npiv == P.iparm[pa.iparm.static_pivoting] == 0
+        Ls64 = Ls.astype(np.float64)
+        del Ls
+        err = _masked_relerr(Lg.copy(), Ls64, mask)
+        print("fp32 factors: max rel diff to the FP32 oracle %.2e" % err, flush=True)
+        assert err <= 2e-5
+        Ld, _, _, _ = orc.factorize_parallel(P.S, P.A, P.perm, P.crit, _threads(), use_blas=True)
+        e32 = _masked_relerr(Ls64, Ld, mask)
+        eg = _masked_relerr(Lg, Ld, mask)
+        print("to the FP64 oracle: oracle FP32 %.2e, GPU FP32 %.2e" % (e32, eg), flush=True)
+        assert eg <= 2e-5 and e32 <= 2e-5
+    finally:
+        P.close()
+
+
+def test_convection_diffusion_80_fp32_getrf_gmres_matches_the_fp64_solution():
+    """The precision configuration: 27-point convection-diffusion 80^3, LU
+    with static pivoting factored in FP32 and refined by FP64 GMRES to
+    backward error 1e-12; the solution equals the plain FP64 GPU solve's and
+    the FP32 factors equal the FP32 oracle's."""
+    grid = (80, 80, 80)
+    print("case convection-diffusion 80^3 LU FP32+GMRES", flush=True)
+    A = problems.convection_diffusion_27(*grid)
+    n = A.shape[0]
+    xt = problems.uniform(n, seed=0)
+    b = A @ xt
+    P64 = Problem(A, grid, pa.factotype.LU)
+    try:
+        x64 = pa.task_solve(P64.h, P64.spm, b.copy())
+        assert problems.backward_error(A, x64, b) <= 1e-14
+    finally:
+        P64.close()
+    P = Problem(A, grid, pa.factotype.LU, mixed=True)
+    try:
+        assert P.iparm[pa.iparm.refinement] == pa.refine.GMRES
+        x = pa.task_solve(P.h, P.spm, b.copy())
+        berr0 = problems.backward_error(A, x, b)
+        pa.task_refine(P.h, P.spm, b, x)
+        berr = problems.backward_error(A, x, b)
+        print("fp32 solve berr %.2e -> refined %.2e in %d iterations" % (berr0, berr, P.iparm[pa.iparm.nbiter]),
+              flush=True)
+        assert berr0 > 1e-10 and berr <= 1e-12
+        assert np.abs(x - x64).max() <= 1e-10 * np.abs(x64).max()
+        Lg, Ug = P.gpu_factors()
+        Ls, Us, npiv, sec = orc.factorize_parallel(P.S, P.A, P.perm, P.crit, _threads(), use_blas=True,
+                                                   factotype=2, dtype=np.float32)
+        assert npiv == P.iparm[pa.iparm.static_pivoting]
+        allm = np.ones(Lg.shape[0], dtype=bool)
+        eL = _masked_relerr(Lg, Ls.astype(np.float64), allm)
+        eU = _masked_relerr(Ug, Us.astype(np.float64), allm)
+        print("fp32 LU factors: max rel diff to the FP32 oracle L %.2e U %.2e" % (eL, eU), flush=True)
+        assert eL <= 2e-5 and eU <= 2e-5
+    finally:
+        P.close()
