@@ -19,6 +19,7 @@
 #include "pastix/datatypes.h"
 #include "pastix/api.h"
 #include "pastix/order.h"
+#include "pastix/solver.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -78,6 +79,16 @@ int pastixRhsSingleToDouble( const pastix_rhs_t sB, pastix_rhs_t dB );
 int  pastixCheckParam( const pastix_int_t *iparm, const double *dparm );
 void pastixDumpParam( const pastix_data_t *pastix_data );
 
+/* ---- factor I/O ----------------------------------------------------------- */
+/* Human-readable dump of the factors in HBM (reference coeftab_zdump,
+ * sopalin/coeftab_z.c:55, and cpucblk_zdump, sopalin/coeftab_zinit.c:54):
+ * one file <prefix>_<cblk>.txt per column block holding the lines
+ * "row col value [L]" and, for LU / LDL^t, "col row value [U]" of its nonzero
+ * coefficients (lower triangle of the diagonal block).  The d / s variant
+ * matches the factors' arithmetic; the prefix may hold a directory. */
+void coeftab_ddump( pastix_data_t *pastix_data, const SolverMatrix *solvmtx, const char *prefix );
+void coeftab_sdump( pastix_data_t *pastix_data, const SolverMatrix *solvmtx, const char *prefix );
+
 /* ---- extensions (no reference counterpart) -------------------------------- */
 
 /*
@@ -89,6 +100,36 @@ void pastixDumpParam( const pastix_data_t *pastix_data );
  */
 int pastix_amd_solve_dev( pastix_data_t *pastix_data, pastix_trans_t transA,
                           pastix_int_t nrhs, void *dB, pastix_int_t ldb );
+
+/*
+ * Factor checkpoints.  pastix_amd_factors_save writes the factors held in
+ * HBM (after pastix_task_numfact) to a binary file: a header with the
+ * factorization kind, arithmetic, static pivot count, threshold and a
+ * fingerprint of the block structure, then the coefficient arrays in the
+ * 2D tile layout.  pastix_amd_factors_load, on an instance where
+ * pastix_task_analyze (same matrix pattern, ordering and parameters) and
+ * pastix_subtask_spm2bcsc have run, puts them back in HBM in place of
+ * bcsc2ctab + sopalin: solves and refinements then run as after a
+ * factorization.  A file of another structure, factorization kind or
+ * precision is refused with PASTIX_ERR_BADPARAMETER.
+ */
+int pastix_amd_factors_save( pastix_data_t *pastix_data, const char *path );
+int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
+
+/*
+ * Factor checkpoints.  pastix_amd_factors_save writes the factors held in
+ * HBM (after pastix_task_numfact) to a binary file: a header with the
+ * factorization kind, arithmetic, static pivot count, threshold and a
+ * fingerprint of the block structure, then the coefficient arrays in the
+ * 2D tile layout.  pastix_amd_factors_load, on an instance where
+ * pastix_task_analyze (same matrix pattern, ordering and parameters) and
+ * pastix_subtask_spm2bcsc have run, puts them back in HBM in place of
+ * bcsc2ctab + sopalin: solves and refinements then run as after a
+ * factorization.  A file of another structure, factorization kind or
+ * precision is refused with PASTIX_ERR_BADPARAMETER.
+ */
+int pastix_amd_factors_save( pastix_data_t *pastix_data, const char *path );
+int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
 
 /*
  * Refine X on the device: FP64 GMRES/CG/SR (IPARM_REFINEMENT) with the
@@ -181,6 +222,16 @@ int pastix_amd_get_dims( const pastix_data_t *pastix_data, pastix_int_t *dims );
  */
 int pastix_amd_get_structure( const pastix_data_t *pastix_data, pastix_int_t *cblk,
                               pastix_int_t *blok, pastix_int_t *perm, pastix_int_t *invp );
+/* The instance's SolverMatrix (after the analysis), e.g. for solverSave. */
+const SolverMatrix *pastix_amd_get_solver( const pastix_data_t *pastix_data );
+/*
+ * The static schedule of the CPU factorization computed by the analysis
+ * (proportional mapping + simulation over IPARM_THREAD_NBR threads):
+ * *thrdnbr threads; when tptr is given (thrdnbr+1 entries), thread t runs
+ * the cblks ttsk[tptr[t] .. tptr[t+1]) (cblknbr entries in all) in order.
+ */
+int pastix_amd_get_schedule( const pastix_data_t *pastix_data, pastix_int_t *thrdnbr,
+                             pastix_int_t *tptr, pastix_int_t *ttsk );
 /*
  * Copies a factor (side 0 = L, 1 = U^T) from HBM into `host` (coefnbr
  * entries, FP64 even when the factorization ran in FP32).

@@ -13,6 +13,7 @@
 #ifndef PASTIX_AMD_SOLVER_H
 #define PASTIX_AMD_SOLVER_H
 
+#include <stdio.h>
 #include "pastix/datatypes.h"
 #include "pastix/api.h"
 
@@ -71,6 +72,16 @@ typedef struct solver_cblk_s {
     pastix_int_t         priority;
 } SolverCblk;
 
+/* A task of the static scheduler: one cblk factorization with its updates
+ * (reference blend/solver.h:121-130, task type COMP_1D). */
+typedef struct task_s {
+    pastix_int_t          taskid;   /* COMP_1D (0)                                */
+    pastix_int_t          prionum;  /* rank of the task in the simulated execution */
+    pastix_int_t          cblknum;  /* attached column block                       */
+    pastix_int_t          bloknum;  /* its diagonal block                          */
+    pastix_int_t volatile ctrbcnt;  /* number of incoming contributions            */
+} Task;
+
 typedef struct solver_matrix_s {
     int          restore;
     pastix_int_t baseval;
@@ -108,7 +119,28 @@ typedef struct solver_matrix_s {
     pastix_int_t gemmmax;
     pastix_int_t blokmax;
     pastix_int_t colmax;
+    /* static scheduling of the CPU factorization (reference blend/solver.h:
+     * 251-265): the proportional mapping and the simulation of the analysis
+     * give each of the thrdnbr threads the ordered list of its tasks */
+    int           clustnum;
+    int           clustnbr;
+    pastix_int_t  procnbr;
+    pastix_int_t  thrdnbr;
+    pastix_int_t  bublnbr;
+    Task         *tasktab;     /* tasknbr tasks                                 */
+    pastix_int_t  tasknbr;
+    pastix_int_t  tasknbr_1dp;
+    pastix_int_t **ttsktab;    /* thrdnbr lists of task indices                 */
+    pastix_int_t *ttsknbr;     /* their lengths                                 */
 } SolverMatrix;
+
+/* Structure I/O in the reference's text format (reference
+ * blend/solver_io.c:42-355).  solverSave writes version 2; solverLoad reads
+ * what solverSave writes and rebuilds the derived fields (coefficient
+ * counts, browtab, the cblks of each block). */
+int solverLoad( SolverMatrix *solvptr, FILE *stream );
+int solverSave( const SolverMatrix *solvptr, FILE *stream );
+void solverExit( SolverMatrix *solvptr );
 
 static inline pastix_int_t cblk_colnbr( const SolverCblk *cblk ) {
     return cblk->lcolnum - cblk->fcolnum + 1;

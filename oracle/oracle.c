@@ -36,10 +36,13 @@
  *   blok[5*b]: frownum, lrownum, lcblknm, fcblknm, coefind
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <omp.h>
+#include <pthread.h>
+#include <sched.h>
 
 #if defined( ORACLE_SINGLE )
 #define REAL float
@@ -631,6 +634,129 @@ PFX( getrf_par )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REA
     *npiv = piv;
     free( lev ); free( order ); free( locks ); free( lptr ); free( pairs );
     return 0;
+}
+
+/* ---- static scheduling (CPU-baseline leg) -------------------------------- */
+
+/* Threads that follow a static schedule: thread t runs the cblk tasks
+ * ttsk[tptr[t] .. tptr[t+1]) in that order (reference thread_zpotrf_static,
+ * sopalin/sequential_zpotrf.c:104-145, and its getrf twin).  A task first
+ * waits until all the block updates facing its cblk have landed (the cblk's
+ * counter, initialised to its number of incoming blocks, reaches zero:
+ * cpucblk_zincoming_deps, kernels/cpucblk_zmpi_coeftab.c:543-600), factors
+ * the cblk, then applies each block's update to the facing cblk under that
+ * cblk's lock and decrements its counter (cpucblk_zrelease_deps, :602-624;
+ * the lock is the reference's pastix_cblk_lock of core_zgemmsp.c).  One
+ * POSIX thread per list, so every list has its own thread. */
+typedef struct {
+    oint             cblknbr;
+    const oint      *cblk, *blok, *tptr, *ttsk;
+    REAL            *L, *U;
+    double           crit;
+    int              lu;
+    volatile oint   *ctrb;
+    pthread_mutex_t *locks;
+    oint             npiv, nneg;
+    int              tid;
+} PFX( static_arg );
+
+static void *
+PFX( static_thread )( void *argp )
+{
+    PFX( static_arg ) *a = (PFX( static_arg ) *)argp;
+    const oint *cblk = a->cblk, *blok = a->blok;
+    REAL       *L = a->L, *U = a->U;
+    oint        t, i, j, b;
+    a->npiv = 0;
+    a->nneg = 0;
+    for ( t = a->tptr[a->tid]; t < a->tptr[a->tid + 1]; t++ ) {
+        oint  k = a->ttsk[t], nc = C_NCOL( k ), lb = C_FBLK( k + 1 );
+        REAL *D = L + C_COEF( k );
+        while ( __atomic_load_n( &a->ctrb[k], __ATOMIC_ACQUIRE ) > 0 ) sched_yield();
+        if ( a->lu ) {
+            REAL *Du = U + C_COEF( k );
+            for ( j = 0; j < nc; j++ )
+                for ( i = 0; i < nc; i++ ) D[i + j * nc] += Du[j + i * nc];
+            getrfsp( nc, D, nc, &a->npiv, a->crit );
+            for ( j = 0; j < nc; j++ )
+                for ( i = 0; i < nc; i++ ) Du[j + i * nc] = D[i + j * nc];
+            for ( b = C_FBLK( k ) + 1; b < lb; b++ ) {
+                trsm_run( B_NROW( b ), nc, D, nc, 0, L + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+                trsm_run( B_NROW( b ), nc, Du, nc, 1, U + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+            }
+        }
+        else {
+            potrfsp( nc, D, nc, &a->npiv, &a->nneg, a->crit );
+            for ( b = C_FBLK( k ) + 1; b < lb; b++ )
+                trsm_rlt( B_NROW( b ), nc, D, nc, L + C_COEF( k ) + B_COEF( b ), B_NROW( b ) );
+        }
+        for ( b = C_FBLK( k ) + 1; b < lb; b++ ) {
+            oint f = B_FCBK( b );
+            pthread_mutex_lock( a->locks + f );
+            if ( a->lu ) {
+                gemmsp_2d2d( cblk, blok, k, b, 0, L, U, L );
+                if ( b + 1 < lb ) gemmsp_2d2d( cblk, blok, k, b, 1, U, L, U );
+            }
+            else {
+                gemmsp_2d2d( cblk, blok, k, b, 0, L, L, L );
+            }
+            pthread_mutex_unlock( a->locks + f );
+            __atomic_fetch_sub( &a->ctrb[f], 1, __ATOMIC_RELEASE );
+        }
+    }
+    return NULL;
+}
+
+/* lu = 0: LL^t in L; lu = 1: LU in L and U.  Returns -1 when a list does not
+ * hold every cblk exactly once. */
+int
+PFX( factor_static )( oint cblknbr, const oint *cblk, const oint *blok, REAL *L, REAL *U, int lu, double crit,
+                      oint *npiv, oint *nneg, int nthreads, const oint *tptr, const oint *ttsk )
+{
+    PFX( static_arg ) *args;
+    pthread_t       *th;
+    pthread_mutex_t *locks;
+    volatile oint   *ctrb;
+    char            *seen;
+    oint             k, b;
+    int              t, rc = 0;
+    if ( nthreads < 1 || tptr[0] != 0 || tptr[nthreads] != cblknbr ) return -1;
+    seen = calloc( cblknbr + 1, 1 );
+    for ( k = 0; k < cblknbr; k++ ) {
+        if ( ttsk[k] < 0 || ttsk[k] >= cblknbr || seen[ttsk[k]] ) {
+            free( seen );
+            return -1;
+        }
+        seen[ttsk[k]] = 1;
+    }
+    free( seen );
+    args  = calloc( nthreads, sizeof( *args ) );
+    th    = calloc( nthreads, sizeof( pthread_t ) );
+    locks = malloc( ( cblknbr + 1 ) * sizeof( pthread_mutex_t ) );
+    ctrb  = calloc( cblknbr + 1, sizeof( oint ) );
+    for ( k = 0; k < cblknbr; k++ ) pthread_mutex_init( locks + k, NULL );
+    for ( k = 0; k < cblknbr; k++ )
+        for ( b = C_FBLK( k ) + 1; b < C_FBLK( k + 1 ); b++ ) ctrb[B_FCBK( b )]++;
+    for ( t = 0; t < nthreads; t++ ) {
+        args[t].cblknbr = cblknbr; args[t].cblk = cblk; args[t].blok = blok;
+        args[t].tptr = tptr; args[t].ttsk = ttsk; args[t].L = L; args[t].U = U;
+        args[t].crit = crit; args[t].lu = lu; args[t].ctrb = ctrb; args[t].locks = locks; args[t].tid = t;
+        if ( pthread_create( th + t, NULL, PFX( static_thread ), args + t ) != 0 ) {
+            /* a missing thread would leave its tasks' dependents waiting */
+            fprintf( stderr, "oracle factor_static: cannot start thread %d\n", t );
+            abort();
+        }
+    }
+    *npiv = 0;
+    if ( nneg ) *nneg = 0;
+    for ( t = 0; t < nthreads; t++ ) {
+        pthread_join( th[t], NULL );
+        *npiv += args[t].npiv;
+        if ( nneg ) *nneg += args[t].nneg;
+    }
+    for ( k = 0; k < cblknbr; k++ ) pthread_mutex_destroy( locks + k );
+    free( args ); free( th ); free( locks ); free( (void *)ctrb );
+    return rc;
 }
 
 /* Supernodal solve in the factorization numbering, x is n x nrhs (ld).

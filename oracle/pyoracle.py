@@ -39,6 +39,8 @@ def load():
             getattr(_lib, "oracle_%sset_blas" % pfx).argtypes = [P, P]
             getattr(_lib, "oracle_%spotrf_par" % pfx).argtypes = [I, P, P, P, D, P, P, C.c_int]
             getattr(_lib, "oracle_%sgetrf_par" % pfx).argtypes = [I, P, P, P, P, D, P, C.c_int]
+            getattr(_lib, "oracle_%sfactor_static" % pfx).argtypes = [I, P, P, P, P, C.c_int, D, P, P, C.c_int,
+                                                                       P, P]
     return _lib
 
 
@@ -156,6 +158,48 @@ def factorize_parallel(S, A, perm, crit, nthreads, use_blas=True, dtype=np.float
             limit.restore_original_limits()
     finally:
         setb(None, None)
+    return L, U, int(npiv[0]), sec
+
+
+def factorize_static(S, A, perm, crit, tptr, ttsk, use_blas=True, dtype=np.float64, factotype=0):
+    """Fill + LL^t (factotype 0) or LU (2) by the threads of a static
+    schedule: thread t runs the cblks ttsk[tptr[t]:tptr[t+1]] in order, each
+    waiting for its incoming updates (the dense kernels on SciPy's BLAS, one
+    BLAS thread per task, when use_blas); returns (L, U or None, npiv,
+    seconds of the factorization alone)."""
+    import time
+    lib = load()
+    pfx = "d" if dtype == np.float64 else "s"
+    tptr = np.ascontiguousarray(tptr, dtype=np.int64)
+    ttsk = np.ascontiguousarray(ttsk, dtype=np.int64)
+    colptr, rowtab, Lv, Uv = permuted_csc(A, perm)
+    L = np.zeros(S.coefnbr, dtype=dtype)
+    U = np.zeros(S.coefnbr, dtype=dtype) if factotype == 2 else None
+    getattr(lib, "oracle_%sfill" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(colptr), _p(rowtab),
+                                        _p(Lv), _p(Uv), _p(L), _p(U))
+    npiv = np.zeros(1, dtype=np.int64)
+    nneg = np.zeros(1, dtype=np.int64)
+    setb = getattr(lib, "oracle_%sset_blas" % pfx)
+    try:
+        if use_blas:
+            g, t = blas_pointers(dtype)
+            setb(C.c_void_p(g), C.c_void_p(t))
+        try:
+            from threadpoolctl import threadpool_limits
+            limit = threadpool_limits(limits=1, user_api="blas")
+        except Exception:
+            limit = None
+        t0 = time.perf_counter()
+        rc = getattr(lib, "oracle_%sfactor_static" % pfx)(S.cblknbr, _p(S.cblkx), _p(S.blok), _p(L), _p(U),
+                                                          1 if factotype == 2 else 0, crit, _p(npiv), _p(nneg),
+                                                          len(tptr) - 1, _p(tptr), _p(ttsk))
+        sec = time.perf_counter() - t0
+        if limit is not None:
+            limit.restore_original_limits()
+    finally:
+        setb(None, None)
+    if rc != 0:
+        raise ValueError("the static schedule does not hold every cblk exactly once")
     return L, U, int(npiv[0]), sec
 
 

@@ -10,12 +10,12 @@ binding, the reference-style Python wrapper and the differentiable solve.
 from . import enum
 from .enum import iparm, dparm, factotype, refine, verbose, trans, mtxtype, ordering, io, dir, error, side, uplo, diag
 from .solver import (solver, spmatrix, order, initParam, init, finalize, task_analyze,
-                     task_numfact, task_solve, task_refine, get_structure, get_coeftab,
+                     task_numfact, task_solve, task_refine, get_structure, get_schedule, get_coeftab,
                      tune, set_tuning, PastixError)
 from . import problems
 
 __all__ = ["enum", "iparm", "dparm", "factotype", "refine", "verbose", "trans", "mtxtype", "ordering", "io",
            "dir", "error", "side", "uplo", "diag",
            "solver", "spmatrix", "order", "initParam", "init", "finalize", "task_analyze",
-           "task_numfact", "task_solve", "task_refine", "get_structure", "get_coeftab",
+           "task_numfact", "task_solve", "task_refine", "get_structure", "get_schedule", "get_coeftab",
            "tune", "set_tuning", "PastixError", "problems"]

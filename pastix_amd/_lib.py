@@ -118,6 +118,15 @@ PROTOTYPES = {
     "pastixOrderGet": (_P, [_P]),
     "pastixOrderGrid": (C.c_int, [_P, _I, _I, _I]),
     "pastix_amd_solve_dev": (C.c_int, [_P, C.c_int, _I, _P, _I]),
+    "pastix_amd_factors_save": (C.c_int, [_P, C.c_char_p]),
+    "pastix_amd_factors_load": (C.c_int, [_P, C.c_char_p]),
+    "coeftab_ddump": (None, [_P, _P, C.c_char_p]),
+    "coeftab_sdump": (None, [_P, _P, C.c_char_p]),
+    "solverSave": (C.c_int, [_P, _P]),
+    "solverLoad": (C.c_int, [_P, _P]),
+    "solverExit": (None, [_P]),
+    "pastix_amd_get_solver": (_P, [_P]),
+    "pastix_amd_get_schedule": (C.c_int, [_P, _P, _P, _P]),
     "pastix_amd_refine_dev": (C.c_int, [_P, _I, _P, _I, _P, _I]),
     "pastix_amd_refine_dev_adv": (C.c_int, [_P, C.c_int, _I, _P, _I, _P, _I]),
     "pastix_amd_sampled_outer": (C.c_int, [_P, _P, _P, _I, _P, _I, _P, _I, C.c_double, _P]),

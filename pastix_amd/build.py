@@ -18,6 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libpastix_amd.so"
 
 HOST_SOURCES = ["host/order.c", "host/amalgamate.c", "host/symbol.c", "host/split.c", "host/solver.c",
+                "host/solver_io.c", "host/factor_io.c", "host/simu.c",
                 "host/bcsc.c", "host/api.c", "host/refine.c"]
 HIP_SOURCES = ["hip/engine.hip", "hip/gpu_api.hip"]
 

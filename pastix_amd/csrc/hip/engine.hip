@@ -866,7 +866,7 @@ pa_dev_set_values( pastix_data_t *pd, const double *d_values, double *normF )
 
 template <typename T>
 static int
-pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
+pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow, bool fill )
 {
     size_t bytes = (size_t)std::max<int64_t>( dev->coefnbr, 1 ) * sizeof( T );
     if ( !dev->d_L ) PA_HIP( hipMalloc( &dev->d_L, bytes ) );
@@ -884,7 +884,7 @@ pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
     }
     PA_HIP( hipMemsetAsync( dev->d_counters, 0, 16 * sizeof( int32_t ), dev->stream ) );
     int32_t n = (int32_t)dev->n;
-    if ( n > 0 ) {
+    if ( n > 0 && fill ) {
         k_fill<T><<<( n + 255 ) / 256, 256, 0, dev->stream>>>(
             dev->S, n, dev->d_colptr, dev->d_rowtab, dev->d_Lv, dev->d_Uv, (T *)dev->d_L,
             lu ? (T *)dev->d_U : nullptr, dev->d_col2cblk, dev->d_counters + 2 );
@@ -897,8 +897,8 @@ pa_coeftab_init_t( pa_device_s *dev, int lu, int32_t *overflow )
     return PASTIX_SUCCESS;
 }
 
-extern "C" int
-pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
+static int
+pa_coeftab_setup( pastix_data_t *pd, int prec, int lu, int32_t *overflow, bool fill )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
@@ -923,8 +923,88 @@ pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
     if ( dev->prec != prec || dev->lu != lu || ( lu && !dev->d_U ) ) pa_free_factors( dev );
     dev->prec = prec;
     dev->lu   = lu;
-    if ( prec == PA_PREC_FLOAT ) return pa_coeftab_init_t<float>( dev, lu, overflow );
-    return pa_coeftab_init_t<double>( dev, lu, overflow );
+    if ( prec == PA_PREC_FLOAT ) return pa_coeftab_init_t<float>( dev, lu, overflow, fill );
+    return pa_coeftab_init_t<double>( dev, lu, overflow, fill );
+}
+
+extern "C" int
+pa_dev_coeftab_init( pastix_data_t *pd, int prec, int lu, int32_t *overflow )
+{
+    return pa_coeftab_setup( pd, prec, lu, overflow, true );
+}
+
+/* ------------------------------------------------------------------------- */
+/* factor arrays to and from host memory (checkpoints)                       */
+/* ------------------------------------------------------------------------- */
+/* the factors held in HBM: arithmetic, two arrays, factorization kind and
+ * coefficients per array; PASTIX_ERR_BADPARAMETER when there are none */
+extern "C" int
+pa_dev_factor_info( const pastix_data_t *pd, int *prec, int *two, int *fact, int64_t *coefnbr )
+{
+    const pa_device_s *dev = DEV( pd );
+    if ( !dev || !dev->d_L || dev->fact_kind < 0 ) return PASTIX_ERR_BADPARAMETER;
+    *prec    = dev->prec;
+    *two     = dev->lu;
+    *fact    = dev->fact_kind;
+    *coefnbr = dev->coefnbr;
+    return PASTIX_SUCCESS;
+}
+
+/* copy count coefficients from offset off of factor array `side` (0: L,
+ * 1: the second array) to host memory, in the factors' arithmetic */
+extern "C" int
+pa_dev_factor_get( const pastix_data_t *pd, int side, int64_t off, int64_t count, void *host )
+{
+    pa_device_s *dev = DEV( pd );
+    const void  *src = side ? dev->d_U : dev->d_L;
+    const size_t es  = dev->prec == PA_PREC_DOUBLE ? sizeof( double ) : sizeof( float );
+    if ( !src || off < 0 || count < 0 || off + count > dev->coefnbr ) return PASTIX_ERR_BADPARAMETER;
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    PA_HIP( hipMemcpy( host, (const char *)src + off * es, count * es, hipMemcpyDeviceToHost ) );
+    return PASTIX_SUCCESS;
+}
+
+/* allocate (or keep) the coeftab for factors of kind IPARM_FACTORIZATION in
+ * arithmetic prec that will be copied in with pa_dev_factor_put, without
+ * filling it from the matrix */
+extern "C" int
+pa_dev_factor_begin( pastix_data_t *pd, int prec, int lu )
+{
+    return pa_coeftab_setup( pd, prec, lu, nullptr, false );
+}
+
+extern "C" int
+pa_dev_factor_put( pastix_data_t *pd, int side, int64_t off, int64_t count, const void *host )
+{
+    pa_device_s *dev = DEV( pd );
+    void        *dst = side ? dev->d_U : dev->d_L;
+    const size_t es  = dev->prec == PA_PREC_DOUBLE ? sizeof( double ) : sizeof( float );
+    if ( !dst || off < 0 || count < 0 || off + count > dev->coefnbr ) return PASTIX_ERR_BADPARAMETER;
+    PA_HIP( hipSetDevice( dev->device ) );
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    PA_HIP( hipMemcpy( (char *)dst + off * es, host, count * es, hipMemcpyHostToDevice ) );
+    return PASTIX_SUCCESS;
+}
+
+/* after the factor arrays are in place: the transposed diagonal blocks of
+ * the LL^t solve, as at the end of a factorization */
+extern "C" int
+pa_dev_factor_end( pastix_data_t *pd )
+{
+    pa_device_s *dev = DEV( pd );
+    PA_HIP( hipSetDevice( dev->device ) );
+    if ( !dev->lu && dev->S.cblknbr > 0 ) {
+        if ( dev->prec == PA_PREC_DOUBLE )
+            k_diag_transpose<double><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, (const double *)dev->d_L,
+                                                                               (double *)dev->d_DT );
+        else
+            k_diag_transpose<float><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, (const float *)dev->d_L,
+                                                                              (float *)dev->d_DT );
+        PA_HIP( hipGetLastError() );
+    }
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    return PASTIX_SUCCESS;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1114,7 +1194,9 @@ pa_collect_profile( pa_device_s *dev )
                      lev.flops[2] / ( t[2] * 1e9 ), lev.flops[1] / ( t[1] * 1e9 ) );
         }
         dev->prof_launch[0] += 1;
-        dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lev.trsm_inv ? 1 : lu_launches( dev ) ) : 0;
+        /* inverted-tile TRSM: k_tile_inv + k_trsm_inv (both sides in one
+         * launch each); substitution TRSM: one launch per side */
+        dev->prof_launch[1] += ( lev.trsm_end > lev.trsm_beg ) ? ( lev.trsm_inv ? 2 : lu_launches( dev ) ) : 0;
         for ( int c = 0; c < PA_NGC; c++ ) dev->prof_launch[2] += ( lev.gc_end[c] > lev.gc_beg[c] ) ? 1 : 0;
     }
     return PASTIX_SUCCESS;
@@ -1339,19 +1421,41 @@ pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t 
     return PASTIX_SUCCESS;
 }
 
-/* Run a sequence of solve steps in place on a device FP64 buffer in the
- * factorization numbering; FP32 factors work on an FP32 copy (down- and
- * up-conversion, reference sopalin/pastix_task_solve.c:583-592,
- * sopalin/pastix_rhs.c:162-242).  The launch sequence is replayed from a
- * cached hipGraph per (buffer, nrhs, ld, steps). */
+/* Launch a sequence of solve steps in place on a device FP64 buffer in the
+ * factorization numbering; FP32 factors work on the FP32 copy xs (n*nrhs,
+ * leading dimension n; down- and up-conversion, reference
+ * sopalin/pastix_task_solve.c:583-592, sopalin/pastix_rhs.c:162-242). */
+static int
+pa_launch_ops( pa_device_s *dev, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld, float *xs )
+{
+    if ( dev->prec == PA_PREC_DOUBLE ) return pa_run_ops_t<double>( dev, ops, nops, d_x, nrhs, ld );
+    for ( int32_t r = 0; r < nrhs; r++ )
+        k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, xs + r * dev->n );
+    int rc = pa_run_ops_t<float>( dev, ops, nops, xs, nrhs, dev->n );
+    if ( rc ) return rc;
+    for ( int32_t r = 0; r < nrhs; r++ )
+        k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, xs + r * dev->n, d_x + r * ld );
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+static int
+pa_check_ops( const pa_device_s *dev, const pa_sweep_op *ops, int nops )
+{
+    if ( !dev->d_L ) return PASTIX_ERR_BADPARAMETER;
+    for ( int o = 0; o < nops; o++ )
+        if ( ops[o].side && !dev->d_U ) return PASTIX_ERR_BADPARAMETER;
+    return PASTIX_SUCCESS;
+}
+
+/* Run a sequence of solve steps (pa_launch_ops) with the handle's FP32
+ * buffer, replayed from a cached hipGraph per (buffer, nrhs, ld, steps). */
 static int
 pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
-    if ( !dev->d_L ) return PASTIX_ERR_BADPARAMETER;
-    for ( int o = 0; o < nops; o++ )
-        if ( ops[o].side && !dev->d_U ) return PASTIX_ERR_BADPARAMETER;
+    if ( pa_check_ops( dev, ops, nops ) ) return PASTIX_ERR_BADPARAMETER;
     int64_t len = dev->n * (int64_t)nrhs;
     if ( dev->prec != PA_PREC_DOUBLE && dev->xs_len < len ) {
         hipFree( dev->d_xs );
@@ -1361,17 +1465,7 @@ pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrh
     }
     int64_t code = 0;
     for ( int o = 0; o < nops; o++ ) code = code * 16 + ops[o].kind * 4 + ops[o].side * 2 + ops[o].unit;
-    auto launch = [&]() -> int {
-        if ( dev->prec == PA_PREC_DOUBLE ) return pa_run_ops_t<double>( dev, ops, nops, d_x, nrhs, ld );
-        for ( int32_t r = 0; r < nrhs; r++ )
-            k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, dev->d_xs + r * dev->n );
-        int rc = pa_run_ops_t<float>( dev, ops, nops, dev->d_xs, nrhs, dev->n );
-        if ( rc ) return rc;
-        for ( int32_t r = 0; r < nrhs; r++ )
-            k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, dev->d_xs + r * dev->n, d_x + r * ld );
-        PA_HIP( hipGetLastError() );
-        return PASTIX_SUCCESS;
-    };
+    auto launch = [&]() -> int { return pa_launch_ops( dev, ops, nops, nrhs, d_x, ld, dev->d_xs ); };
     if ( !dev->use_graphs ) return launch();
     /* cached graph per (buffer, nrhs, ld, steps) among the most recent ones */
     pa_graph_entry *slot = nullptr, *oldest = &dev->solve_graphs[0];
@@ -1400,14 +1494,11 @@ pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrh
  *   LU:          L y = b (unit),      U x = y (U^T tiles, non-unit)
  *   LU, trans:   U^t y = b (U^T tiles, non-unit), L^t x = y (unit)
  */
-extern "C" int
-pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_x, int64_t ld )
+static int
+pa_solve_ops( int fact, pastix_trans_t trans, pa_sweep_op *ops )
 {
-    pa_device_s *dev = DEV( pd );
-    const int    fact = dev->fact_kind;
-    const bool   tr   = ( trans != PastixNoTrans );
-    pa_sweep_op  ops[3];
-    int          nops = 0;
+    const bool tr   = ( trans != PastixNoTrans );
+    int        nops = 0;
     if ( fact == PastixFactLU ) {
         ops[nops++] = { PA_OP_FWD, tr ? 1 : 0, tr ? 0 : 1 };
         ops[nops++] = { PA_OP_BWD, tr ? 0 : 1, tr ? 1 : 0 };
@@ -1421,6 +1512,14 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
         ops[nops++] = { PA_OP_FWD, 0, 0 };
         ops[nops++] = { PA_OP_BWD, 0, 0 };
     }
+    return nops;
+}
+
+extern "C" int
+pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_x, int64_t ld )
+{
+    pa_sweep_op ops[3];
+    const int   nops = pa_solve_ops( DEV( pd )->fact_kind, trans, ops );
     return pa_dev_run_ops( pd, ops, nops, nrhs, d_x, ld );
 }
 

@@ -47,6 +47,17 @@ int    pa_dev_factorize( pastix_data_t *pastix_data, double criterion, int32_t *
  * numbering (nrhs columns, leading dimension ld) */
 int    pa_dev_solve_perm( pastix_data_t *pastix_data, pastix_trans_t trans, int32_t nrhs,
                           double *d_x, int64_t ld );
+/* factor arrays to and from host memory: what HBM holds (arithmetic, two
+ * arrays, factorization kind, coefficients per array); raw copies of count
+ * coefficients at offset off of array side (0: L, 1: second array) in the
+ * factors' arithmetic; begin allocates the coeftab for the current
+ * IPARM_FACTORIZATION without filling it, end builds what the solve derives
+ * from the factors */
+int    pa_dev_factor_info( const pastix_data_t *pastix_data, int *prec, int *two, int *fact, int64_t *coefnbr );
+int    pa_dev_factor_get( const pastix_data_t *pastix_data, int side, int64_t off, int64_t count, void *host );
+int    pa_dev_factor_begin( pastix_data_t *pastix_data, int prec, int lu );
+int    pa_dev_factor_put( pastix_data_t *pastix_data, int side, int64_t off, int64_t count, const void *host );
+int    pa_dev_factor_end( pastix_data_t *pastix_data );
 /* one triangular sweep on a device FP64 buffer in the factorization
  * numbering: upper = 0 sweeps L, upper = 1 the stored U^T; transposed
  * selects op(factor) = factor^T; unit ignores the stored diagonal */

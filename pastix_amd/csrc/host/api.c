@@ -10,6 +10,7 @@
 #include <stdarg.h>
 #include <time.h>
 #include <float.h>
+#include <unistd.h>
 #include "internal.h"
 #include "../hip/pa_device.h"
 
@@ -370,6 +371,18 @@ pastix_subtask_blend( pastix_data_t *pd )
     pd->solvmatr = calloc( 1, sizeof( SolverMatrix ) );
     flt = pd->iparm[IPARM_MIXED] ? PastixFloat : PastixDouble;
     if ( ( rc = pa_solver_gen( pd->solvmatr, pd->symbmtx, (int)pd->iparm[IPARM_FACTORIZATION], flt ) ) ) return rc;
+    /* static schedule of the CPU factorization over IPARM_THREAD_NBR threads
+     * (the host's online processors when it is not set): proportional
+     * mapping + simulation, as the reference's blend */
+    {
+        pastix_int_t thr = pd->iparm[IPARM_THREAD_NBR];
+        double       pred = 0.;
+        if ( thr < 1 ) thr = (pastix_int_t)sysconf( _SC_NPROCESSORS_ONLN );
+        if ( ( rc = pa_blend_static( pd->solvmatr, (int)pd->iparm[IPARM_FACTORIZATION], thr,
+                                     (int)pd->iparm[IPARM_ALLCAND], &pred ) ) )
+            return rc;
+        pd->dparm[DPARM_PRED_FACT_TIME] = pred;
+    }
     /* the analysis runs without a GPU; the structure is uploaded as soon as
      * one is available (here, or at the first numerical step) */
     if ( pd->dev != NULL ) {
@@ -1051,6 +1064,32 @@ pastix_amd_get_dims( const pastix_data_t *pd, pastix_int_t *dims )
     dims[5] = pa_dev_nlevels( pd );
     dims[6] = pd->iparm[IPARM_FACTORIZATION];
     dims[7] = pd->solvmatr->flttype;
+    return PASTIX_SUCCESS;
+}
+
+const SolverMatrix *
+pastix_amd_get_solver( const pastix_data_t *pd )
+{
+    return pd ? pd->solvmatr : NULL;
+}
+
+int
+pastix_amd_get_schedule( const pastix_data_t *pd, pastix_int_t *thrdnbr, pastix_int_t *tptr, pastix_int_t *ttsk )
+{
+    const SolverMatrix *s;
+    pastix_int_t        t, i, n = 0;
+    if ( pd == NULL || pd->solvmatr == NULL || thrdnbr == NULL ) return PASTIX_ERR_BADPARAMETER;
+    s        = pd->solvmatr;
+    *thrdnbr = s->thrdnbr;
+    if ( tptr == NULL ) return PASTIX_SUCCESS;
+    tptr[0] = 0;
+    for ( t = 0; t < s->thrdnbr; t++ ) {
+        for ( i = 0; i < s->ttsknbr[t]; i++ ) {
+            if ( ttsk ) ttsk[n] = s->tasktab[s->ttsktab[t][i]].cblknum;
+            n++;
+        }
+        tptr[t + 1] = n;
+    }
     return PASTIX_SUCCESS;
 }
 

@@ -101,6 +101,11 @@ int pa_split_symbol( symbol_matrix_t *symb, pastix_int_t blcolmin, pastix_int_t 
 /* SolverMatrix generation (solver.c) */
 int  pa_solver_gen( SolverMatrix *solv, const symbol_matrix_t *symb, int factotype, int flttype );
 void pa_solver_exit( SolverMatrix *solv );
+int  pa_solver_build_browtab( SolverMatrix *solv );
+/* Static schedule of the CPU factorization over thrdnbr threads (simu.c):
+ * tasktab, ttsktab / ttsknbr and each cblk's thread; *pred_time gets the
+ * simulated factorization time (seconds of the fitted kernel model) */
+int  pa_blend_static( SolverMatrix *solv, int fact, pastix_int_t thrdnbr, int allcand, double *pred_time );
 /* Level (height in the elimination tree of cblks) of each cblk: leaves 0. */
 pastix_int_t pa_solver_levels( const SolverMatrix *solv, pastix_int_t *level );
 

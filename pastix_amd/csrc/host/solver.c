@@ -18,7 +18,7 @@ int
 pa_solver_gen( SolverMatrix *solv, const symbol_matrix_t *symb, int factotype, int flttype )
 {
     pastix_int_t k, b, cblknbr = symb->cblknbr, bloknbr = symb->bloknbr;
-    pastix_int_t coefnbr = 0, *pos;
+    pastix_int_t coefnbr = 0;
 
     memset( solv, 0, sizeof( *solv ) );
     solv->nodenbr   = symb->nodenbr;
@@ -73,7 +73,16 @@ pa_solver_gen( SolverMatrix *solv, const symbol_matrix_t *symb, int factotype, i
     solv->cblktab[cblknbr].lcolidx  = symb->nodenbr;
     solv->coefnbr = coefnbr;
 
-    /* browtab: incoming blocks of each cblk, by increasing source cblk */
+    return pa_solver_build_browtab( solv );
+}
+
+/* browtab: the incoming blocks of each cblk by increasing source cblk, the
+ * blocks' browind and the cblks' contribution counts (reference
+ * blend/solver_matrix_gen.c:571-620) */
+int
+pa_solver_build_browtab( SolverMatrix *solv )
+{
+    pastix_int_t k, b, cblknbr = solv->cblknbr, bloknbr = solv->bloknbr, *pos;
     {
         pastix_int_t *cnt = calloc( cblknbr + 1, sizeof( pastix_int_t ) );
         for ( b = 0; b < bloknbr; b++ ) {
@@ -107,12 +116,24 @@ pa_solver_gen( SolverMatrix *solv, const symbol_matrix_t *symb, int factotype, i
 void
 pa_solver_exit( SolverMatrix *solv )
 {
+    pastix_int_t t;
     if ( solv ) {
         free( solv->cblktab );
         free( solv->bloktab );
         free( solv->browtab );
+        if ( solv->ttsktab )
+            for ( t = 0; t < solv->thrdnbr; t++ ) free( solv->ttsktab[t] );
+        free( solv->ttsktab );
+        free( solv->ttsknbr );
+        free( (void *)solv->tasktab );
         memset( solv, 0, sizeof( *solv ) );
     }
+}
+
+void
+solverExit( SolverMatrix *solv )
+{
+    pa_solver_exit( solv );
 }
 
 /*

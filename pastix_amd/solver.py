@@ -203,6 +203,20 @@ def task_refine(handle, spm, b, x):
     return x
 
 
+def get_schedule(handle):
+    """(tptr, ttsk) of the analysis' static CPU schedule: thread t runs the
+    cblks ttsk[tptr[t]:tptr[t+1]] in order."""
+    lib = _lib.load()
+    nt = np.zeros(1, dtype=np.int64)
+    _check(lib.pastix_amd_get_schedule(handle, _ptr(nt), None, None), "pastix_amd_get_schedule")
+    dims = np.zeros(8, dtype=np.int64)
+    _check(lib.pastix_amd_get_dims(handle, _ptr(dims)), "pastix_amd_get_dims")
+    tptr = np.zeros(int(nt[0]) + 1, dtype=np.int64)
+    ttsk = np.zeros(int(dims[1]), dtype=np.int64)
+    _check(lib.pastix_amd_get_schedule(handle, _ptr(nt), _ptr(tptr), _ptr(ttsk)), "pastix_amd_get_schedule")
+    return tptr, ttsk
+
+
 def get_structure(handle):
     """(cblk[cblknbr,5], blok[bloknbr,5], perm, invp, dims) of the analysed
     problem; cblk rows: fcolnum, lcolnum, first block, stride, coefficient
