@@ -117,21 +117,6 @@ int pastix_amd_factors_save( pastix_data_t *pastix_data, const char *path );
 int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
 
 /*
- * Factor checkpoints.  pastix_amd_factors_save writes the factors held in
- * HBM (after pastix_task_numfact) to a binary file: a header with the
- * factorization kind, arithmetic, static pivot count, threshold and a
- * fingerprint of the block structure, then the coefficient arrays in the
- * 2D tile layout.  pastix_amd_factors_load, on an instance where
- * pastix_task_analyze (same matrix pattern, ordering and parameters) and
- * pastix_subtask_spm2bcsc have run, puts them back in HBM in place of
- * bcsc2ctab + sopalin: solves and refinements then run as after a
- * factorization.  A file of another structure, factorization kind or
- * precision is refused with PASTIX_ERR_BADPARAMETER.
- */
-int pastix_amd_factors_save( pastix_data_t *pastix_data, const char *path );
-int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
-
-/*
  * Refine X on the device: FP64 GMRES/CG/SR (IPARM_REFINEMENT) with the
  * factorization as preconditioner, device SpMV on the permuted CSC and device
  * norms.  dB and dX are device FP64 buffers in the user numbering.
