@@ -232,6 +232,8 @@ pa_set_lds_limits( int bytes )
     pa_lds_attr( k_diag<float, true, 1024>, bytes );  pa_lds_attr( k_diag<float, false, 1024>, bytes );
     pa_lds_attr( k_diag<double, true, 1024>, bytes ); pa_lds_attr( k_diag<double, false, 1024>, bytes );
     pa_lds_attr( k_diag<float, true, 64>, bytes );   pa_lds_attr( k_diag<float, false, 64>, bytes );
+    pa_lds_attr( k_diag<double, true, 64, true>, bytes );   pa_lds_attr( k_diag<double, true, 1024, true>, bytes );
+    pa_lds_attr( k_diag<float, true, 64, true>, bytes );    pa_lds_attr( k_diag<float, true, 1024, true>, bytes );
 }
 
 extern "C" int
@@ -1081,13 +1083,17 @@ pa_factorize_t( pa_device_s *dev )
         }
         else if ( lev.ncmax_fac > 64 ) {
             /* 1024 threads fit without spilling (FP64 with 2 trailing tiles
-             * per wave and round): the waves of the trailing updates */
-            if ( lu ) k_diag<T, true, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else      k_diag<T, false, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+             * per wave and round): the waves of the trailing updates; LDL^t
+             * runs the LU kernel on the block symmetrised from its lower
+             * triangle */
+            if ( dev->ldlt ) k_diag<T, true, 1024, true><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else if ( lu )   k_diag<T, true, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else             k_diag<T, false, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
         }
         else {
-            if ( lu ) k_diag<T, true, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else      k_diag<T, false, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            if ( dev->ldlt ) k_diag<T, true, 64, true><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else if ( lu )   k_diag<T, true, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            else             k_diag<T, false, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
         }
         PA_HIP( hipGetLastError() );
         PA_PROF_MARK( 1 );

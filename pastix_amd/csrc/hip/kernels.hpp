@@ -203,11 +203,15 @@ template <> struct pa_mfma<float> {
 /* Transposes between the two diagonal blocks of an LU cblk:
  * ADD: D(i, j) += Du(j, i) for i < j (the strict upper part of A, stored
  * transposed in the U^T block, merged into the block being factored);
+ * ADD and MIRROR: D(i, j) = D(j, i) for i < j (LDL^t: the block is symmetric
+ * and only its lower triangle holds the matrix and its updates);
  * otherwise Du = D^T (the factored block's U^T copy).  Wide blocks go through
  * LDS tiles of 32 x 32 (buf: at least 32 nc elements, the panel area of the
  * diagonal kernel) so that reads and writes are both coalesced; each thread
- * issues all of its loads of a batch of tiles before storing any. */
-template <typename T, int TH, bool ADD>
+ * issues all of its loads of a batch of tiles before storing any.  In the
+ * MIRROR case a strict upper entry is written from a strict lower one, which
+ * no tile writes, so the order of the tiles does not matter. */
+template <typename T, int TH, bool ADD, bool MIRROR = false>
 __device__ __forceinline__ void
 pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
 {
@@ -239,7 +243,7 @@ pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
                 for ( int tl = 0; tl < MB; tl++ ) {
                     int32_t a = 0, b = 0;
                     if ( tl < nb ) pair( base + tl, a, b );
-                    const T *src = ADD ? Du : D;
+                    const T *src = ( ADD && !MIRROR ) ? Du : D;
 #pragma unroll
                     for ( int u = 0; u < EPT; u++ ) {
                         const int32_t e = t + u * TH, r = e & 31, c = e >> 5;
@@ -266,7 +270,10 @@ pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
                         const int32_t dr = ( ADD ? a : b ) * 32 + r, dc = ( ADD ? b : a ) * 32 + c;
                         if ( tl < nb ) {
                             const T x = buf[tl * 1056 + c * 33 + r];
-                            if ( ADD ) {
+                            if ( ADD && MIRROR ) {
+                                if ( dr < dc && dc < nc ) D[dr + (int64_t)dc * nc] = x;
+                            }
+                            else if ( ADD ) {
                                 if ( dr < dc && dc < nc ) D[dr + (int64_t)dc * nc] += x;
                             }
                             else if ( dr < nc && dc < nc ) Du[dr + (int64_t)dc * nc] = x;
@@ -280,7 +287,10 @@ pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
     }
     for ( int32_t idx = t; idx < nc * nc; idx += TH ) {
         const int32_t i = idx % nc, j = idx / nc;
-        if ( ADD ) {
+        if ( ADD && MIRROR ) {
+            if ( i < j ) D[i + (int64_t)j * nc] = D[j + (int64_t)i * nc];
+        }
+        else if ( ADD ) {
             if ( i < j ) D[i + (int64_t)j * nc] += Du[j + (int64_t)i * nc];
         }
         else Du[j + (int64_t)i * nc] = D[i + (int64_t)j * nc];
@@ -290,7 +300,7 @@ pa_lu_transpose( T *D, T *Du, int32_t nc, T *buf )
 
 /* TH threads per workgroup: 1024 for wide diagonal blocks, 64 for the
  * small blocks of the lower levels */
-template <typename T, bool LU, int TH>
+template <typename T, bool LU, int TH, bool SYM = false>
 __device__ __forceinline__ void
 pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nneg, T *P )
 {
@@ -301,7 +311,15 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
     int32_t   mypiv = 0, myneg = 0;
     const T   tcrit = (T)crit;
 
-    if ( LU ) {
+    if ( LU && SYM ) {
+        /* LDL^t (reference core_zsytrfsp, kernels/core_zsytrfsp.c): the
+         * lower triangle holds the block with all its updates; the no-pivot
+         * LU of the symmetric block then leaves L below the diagonal, D on
+         * it and D L^t above it, the layout of the reference's factored
+         * diagonal block */
+        pa_lu_transpose<T, TH, true, true>( D, Du, nc, P );
+    }
+    else if ( LU ) {
         /* L += U^T on the full square: the strict lower part of U^T holds the
          * strict upper part of A with the updates of the descendant cblks
          * (U-side scatter-GEMMs), L holds the rest */
@@ -523,15 +541,15 @@ pa_diag_factor( T *D, T *Du, int32_t nc, double crit, int32_t *npiv, int32_t *nn
 /* one workgroup per cblk of a level; the pivoting criterion is read from
  * device memory so that a recorded launch sequence stays valid when it
  * changes with the matrix values */
-template <typename T, bool LU, int TH>
+template <typename T, bool LU, int TH, bool SYM = false>
 __global__ __launch_bounds__( TH ) void
 k_diag( pa_dstruct S, const int32_t *cblks, T *Lc, T *Uc, const double *critp, int32_t *npiv,
         int32_t *nneg )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     int32_t k = cblks[blockIdx.x];
-    pa_diag_factor<T, LU, TH>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], *critp, npiv,
-                               nneg, reinterpret_cast<T *>( pa_smem_raw ) );
+    pa_diag_factor<T, LU, TH, SYM>( Lc + S.c_coef[k], LU ? Uc + S.c_coef[k] : nullptr, S.c_ncol[k], *critp, npiv,
+                                    nneg, reinterpret_cast<T *>( pa_smem_raw ) );
 }
 
 /* out[w] = sum of vals[src[k]] for k in [ptr[w], ptr[w+1]): the permuted

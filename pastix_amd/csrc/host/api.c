@@ -987,11 +987,17 @@ pastix_subtask_trsm( pastix_data_t *pd, pastix_side_t side, pastix_uplo_t uplo, 
 {
     pa_trsm_arg_t a;
     int rc;
-    if ( ( rc = pa_check_rhs( pd, Bp, "pastix_subtask_trsm" ) ) ) return rc;
+    /* the instance and the handle must exist (reference :310-318); the side
+     * is checked before the handle's contents */
+    if ( pd == NULL || Bp == NULL ) {
+        pa_error( "pastix_subtask_trsm: wrong pastix_data or Bp parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
     if ( side != PastixLeft ) {
         pa_error( "pastix_subtask_trsm: only the left side is handled" );
         return PASTIX_ERR_NOTIMPLEMENTED;
     }
+    if ( ( rc = pa_check_rhs( pd, Bp, "pastix_subtask_trsm" ) ) ) return rc;
     if ( ( uplo != PastixLower && uplo != PastixUpper ) || ( diag != PastixUnit && diag != PastixNonUnit ) ||
          ( trans != PastixNoTrans && trans != PastixTrans && trans != PastixConjTrans ) )
         return PASTIX_ERR_BADPARAMETER;

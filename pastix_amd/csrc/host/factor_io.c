@@ -43,7 +43,7 @@ pa_coeftab_dump( pastix_data_t *pd, const SolverMatrix *solv, const char *prefix
     int          prec, two, fact, side;
     int64_t      coefnbr;
     double      *coef[2] = { NULL, NULL };
-    pastix_int_t k;
+    pastix_int_t k, base;
     char         fname[4096];
     if ( pd == NULL || solv == NULL || prefix == NULL || pa_dev_factor_info( pd, &prec, &two, &fact, &coefnbr ) ) {
         pa_error( "%s: no factors to dump", name );
@@ -62,7 +62,7 @@ pa_coeftab_dump( pastix_data_t *pd, const SolverMatrix *solv, const char *prefix
             return;
         }
     }
-    for ( k = 0; k < solv->cblknbr; k++ ) {
+    for ( k = 0, base = 0; k < solv->cblknbr; base += solv->cblktab[k].stride * cblk_colnbr( solv->cblktab + k ), k++ ) {
         const SolverCblk *c = solv->cblktab + k;
         FILE             *f;
         if ( c->cblktype & ( CBLK_FANIN | CBLK_RECV ) ) continue;
@@ -72,8 +72,9 @@ pa_coeftab_dump( pastix_data_t *pd, const SolverMatrix *solv, const char *prefix
             pa_error( "%s: cannot open %s", name, fname );
             continue;
         }
-        pa_cblk_dump( 0, c, coef[0], f );
-        if ( two ) pa_cblk_dump( 1, c, coef[1], f );
+        /* the cblk's panel starts after those of the cblks before it */
+        pa_cblk_dump( 0, c, coef[0] + base, f );
+        if ( two ) pa_cblk_dump( 1, c, coef[1] + base, f );
         fclose( f );
     }
     free( coef[0] );
