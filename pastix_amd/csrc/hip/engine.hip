@@ -59,6 +59,8 @@ struct pa_level_t {
     int64_t tinv_beg, tinv_end;   /* into d_tinv_tasks (diagonal tiles)     */
     bool    trsm_inv;             /* TRSM by inverted tiles (k_trsm_inv)    */
     int64_t gc_beg[PA_NGC], gc_end[PA_NGC]; /* into d_gemm_tasks, per tile class */
+    int64_t gc_mid[PA_NGC];       /* [gc_beg, gc_mid): tasks whose target is a
+                                     cblk of the next level (look-ahead)   */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
     int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
     int32_t ncmax;                /* widest cblk of the level               */
@@ -98,6 +100,7 @@ struct pa_wblock {
 struct pa_device_s {
     int         device;
     hipStream_t stream;
+    hipStream_t stream2;          /* look-ahead panels of the next level     */
     int         lds_limit;
     /* structure */
     int32_t *d_c_fcol, *d_c_ncol, *d_c_stride, *d_c_fblok;
@@ -138,6 +141,8 @@ struct pa_device_s {
     /* hipGraphs of the factorization and of recent solves (the launch
      * sequences are long: ~5 launches per level) */
     int            use_graphs = 1;
+    int            lookahead  = 1;     /* PASTIX_AMD_LOOKAHEAD=0 disables      */
+    std::vector<hipEvent_t> la_ev;     /* fork / join events, 2 per level    */
     pa_graph_entry fact_graph;
     pa_graph_entry solve_graphs[PA_SOLVE_GRAPHS];
     uint64_t       graph_clock = 0;
@@ -254,6 +259,11 @@ pa_dev_open( pastix_data_t *pd )
     if ( env ) dev->device = atoi( env );
     PA_HIP( hipSetDevice( dev->device ) );
     PA_HIP( hipStreamCreate( &dev->stream ) );
+    PA_HIP( hipStreamCreate( &dev->stream2 ) );
+    {
+        const char *la = getenv( "PASTIX_AMD_LOOKAHEAD" );
+        if ( la && atoi( la ) == 0 ) dev->lookahead = 0;
+    }
     PA_HIP( hipEventCreate( &dev->ev0 ) );
     PA_HIP( hipEventCreate( &dev->ev1 ) );
     int lds = 65536;
@@ -377,6 +387,8 @@ pa_dev_close( pastix_data_t *pd )
     hipEventDestroy( dev->ev0 );
     hipEventDestroy( dev->ev1 );
     hipStreamDestroy( dev->stream );
+    hipStreamDestroy( dev->stream2 );
+    for ( hipEvent_t e : dev->la_ev ) hipEventDestroy( e );
     delete dev;
     pd->dev = NULL;
 }
@@ -705,6 +717,13 @@ pa_build_schedule( pastix_data_t *pd )
         L.tinv_end  = tinv_tasks.size();
         L.trsm_inv  = tr_inv && L.ncmax_fac <= PA_TI_MAXNC;
         for ( int c = 0; c < PA_NGC; c++ ) L.gc_end[c] = gtasks[c].size();
+        /* look-ahead order: the tasks updating cblks of the next level
+         * first (the next level's panels can start once they are done) */
+        for ( int c = 0; c < PA_NGC; c++ ) {
+            auto b0  = gtasks[c].begin() + L.gc_beg[c], b1 = gtasks[c].begin() + L.gc_end[c];
+            auto mid = std::stable_partition( b0, b1, [&]( const pa_gemm_task &t ) { return level[t.F] == l + 1; } );
+            L.gc_mid[c] = mid - gtasks[c].begin();
+        }
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
     }
@@ -724,6 +743,7 @@ pa_build_schedule( pastix_data_t *pd )
         const int64_t base = gemm_tasks.size();
         for ( pa_level_t &L : dev->levels ) {
             L.gc_beg[c] += base;
+            L.gc_mid[c] += base;
             L.gc_end[c] += base;
         }
         gemm_tasks.insert( gemm_tasks.end(), gtasks[c].begin(), gtasks[c].end() );
@@ -1054,117 +1074,193 @@ pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x
     }
 }
 
+/* Diagonal factorization, tile inverses and panel TRSM of the general
+ * cblks of one level, on stream st */
+template <typename T>
+static int
+pa_launch_panel( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
+{
+    const double *crit = dev->d_crit;
+    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
+    const int lu = dev->lu;
+    const int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg );
+    if ( ncb == 0 ) return PASTIX_SUCCESS;
+    /* the panel P, and for LU also the row panel U12 */
+    size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax_fac * sizeof( T ) * ( lu ? 2 : 1 );
+    const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
+    int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
+    const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax_fac > 64 && lev.ncmax_fac <= 16 * PA_DREG_NT;
+    if ( regdiag ) {
+        k_diag_potrf_reg<<<ncb, 1024, 0, st>>>( dev->S, cb, (float *)L, crit, c0, c1 );
+    }
+    else if ( lev.ncmax_fac > 64 ) {
+        /* 1024 threads fit without spilling (FP64 with 2 trailing tiles
+         * per wave and round): the waves of the trailing updates; LDL^t
+         * runs the LU kernel on the block symmetrised from its lower
+         * triangle */
+        if ( dev->ldlt ) k_diag<T, true, 1024, true><<<ncb, 1024, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+        else if ( lu )   k_diag<T, true, 1024><<<ncb, 1024, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+        else             k_diag<T, false, 1024><<<ncb, 1024, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+    }
+    else {
+        if ( dev->ldlt ) k_diag<T, true, 64, true><<<ncb, 64, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+        else if ( lu )   k_diag<T, true, 64><<<ncb, 64, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+        else             k_diag<T, false, 64><<<ncb, 64, lds_diag, st>>>( dev->S, cb, L, U, crit, c0, c1 );
+    }
+    PA_HIP( hipGetLastError() );
+    return PASTIX_SUCCESS;
+}
+
+template <typename T>
+static int
+pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
+{
+    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
+    const int lu = dev->lu;
+    int64_t ntr = lev.trsm_end - lev.trsm_beg;
+    if ( ntr > 0 && lev.trsm_inv ) {
+        const int32_t ntl = (int32_t)( lev.tinv_end - lev.tinv_beg );
+        const unsigned gti = (unsigned)( ( ntl + 3 ) / 4 );
+        const int2   *tt   = dev->d_tinv_tasks + lev.tinv_beg;
+        const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
+        T            *Ti   = (T *)dev->d_tinv;
+        const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
+        if ( dev->ldlt ) {
+            /* L panel against D L^t, with its copy L D into the second array */
+            k_tile_inv<T, 4><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+            k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0 );
+        }
+        else if ( lu ) {
+            /* both LU modes (L and U^T panels) in one launch each */
+            k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
+            k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2 );
+        }
+        else {
+            k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+            k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0 );
+        }
+        PA_HIP( hipGetLastError() );
+    }
+    else if ( ntr > 0 ) {
+        size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
+        if ( dev->ldlt ) {
+            k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            k_ldlt_scalo<T><<<(unsigned)ntr, 256, 0, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+        }
+        else if ( lu ) {
+            k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+            k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+        }
+        else {
+            k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
+        }
+        PA_HIP( hipGetLastError() );
+    }
+    return PASTIX_SUCCESS;
+}
+
+/* the scatter-GEMM tasks [from[c], to[c]) of every tile class, on the
+ * library stream, the widest tiles first (they hold most of the work) */
+template <typename T>
+static int
+pa_launch_gemm( pa_device_s *dev, const int64_t *from, const int64_t *to )
+{
+    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
+    T *Up = dev->lu ? U : nullptr;
+    int32_t *nerr = dev->d_counters + 3;
+    for ( int c = PA_NGC - 1; c >= 0; c-- ) {
+        const int64_t ng = to[c] - from[c];
+        if ( ng <= 0 ) continue;
+        const pa_gemm_task *tk = dev->d_gemm_tasks + from[c];
+        const unsigned g = (unsigned)ng;
+        switch ( c ) {
+        case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+        case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+        case 3: k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+        case 4: k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
+        default:
+            if constexpr ( sizeof( T ) == 4 ) {
+                k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr );
+            }
+            else {
+                return PASTIX_ERR_INTERNAL; /* 128 x 128 tiles are FP32 only */
+            }
+        }
+        PA_HIP( hipGetLastError() );
+    }
+    return PASTIX_SUCCESS;
+}
+
+/*
+ * The level-ordered launch sequence.  Per level: the narrow cblks' fused
+ * kernel, the diagonal factorizations, the tile inverses and TRSMs, and the
+ * scatter-GEMMs.  With look-ahead (the default, off while profiling the
+ * kernel classes), the GEMM tasks of level l whose target cblk belongs to
+ * level l + 1 run first; then the diagonal factorizations and TRSMs of
+ * level l + 1 start on the second stream, concurrently with the rest of
+ * level l's GEMM, whose targets lie above level l + 1 and are disjoint from
+ * what they read and write; level l + 1's GEMMs wait for both.  Every other
+ * input of level l + 1's panels comes from updates ordered before the fork
+ * on the library stream.
+ */
 template <typename T>
 static int
 pa_factorize_t( pa_device_s *dev )
 {
     const double *crit = dev->d_crit;
-    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
-    const int lu = dev->lu;
-    for ( size_t li = 0; li < dev->levels.size(); li++ ) {
+    T *L = (T *)dev->d_L;
+    const size_t nl = dev->levels.size();
+    const bool   la = dev->lookahead && !dev->profiling;
+    int rc;
+    if ( la && dev->la_ev.size() < 2 * nl ) {
+        size_t old = dev->la_ev.size();
+        dev->la_ev.resize( 2 * nl );
+        for ( size_t i = old; i < dev->la_ev.size(); i++ )
+            PA_HIP( hipEventCreateWithFlags( &dev->la_ev[i], hipEventDisableTiming ) );
+    }
+    bool ahead = false; /* the current level's panels were launched ahead */
+    for ( size_t li = 0; li < nl; li++ ) {
         const pa_level_t &lev = dev->levels[li];
         int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg ), nsm = (int32_t)( lev.cblk_end - lev.fac_end );
         if ( ncb == 0 && nsm == 0 ) continue;
         PA_PROF_MARK( 0 );
-        /* the panel P, and for LU also the row panel U12 */
-        size_t lds_diag = (size_t)pa_diag_nb<T>::value * lev.ncmax_fac * sizeof( T ) * ( lu ? 2 : 1 );
-        const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
         int32_t *c0 = dev->d_counters, *c1 = dev->d_counters + 1;
         if ( nsm > 0 ) {
             pa_small_llt_launch<T>( dev, lev, L, crit, c0, c1 );
             PA_HIP( hipGetLastError() );
         }
-        const bool regdiag = !lu && sizeof( T ) == 4 && lev.ncmax_fac > 64 && lev.ncmax_fac <= 16 * PA_DREG_NT;
-        if ( ncb == 0 ) {
-            /* only narrow cblks at this level */
-        }
-        else if ( regdiag ) {
-            k_diag_potrf_reg<<<ncb, 1024, 0, dev->stream>>>( dev->S, cb, (float *)L, crit, c0, c1 );
-        }
-        else if ( lev.ncmax_fac > 64 ) {
-            /* 1024 threads fit without spilling (FP64 with 2 trailing tiles
-             * per wave and round): the waves of the trailing updates; LDL^t
-             * runs the LU kernel on the block symmetrised from its lower
-             * triangle */
-            if ( dev->ldlt ) k_diag<T, true, 1024, true><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else if ( lu )   k_diag<T, true, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else             k_diag<T, false, 1024><<<ncb, 1024, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+        if ( ahead ) {
+            /* join: this level's panels were factored on the second stream */
+            PA_HIP( hipStreamWaitEvent( dev->stream, dev->la_ev[2 * li + 1], 0 ) );
         }
         else {
-            if ( dev->ldlt ) k_diag<T, true, 64, true><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else if ( lu )   k_diag<T, true, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
-            else             k_diag<T, false, 64><<<ncb, 64, lds_diag, dev->stream>>>( dev->S, cb, L, U, crit, c0, c1 );
+            if ( ( rc = pa_launch_panel<T>( dev, lev, dev->stream ) ) ) return rc;
         }
-        PA_HIP( hipGetLastError() );
         PA_PROF_MARK( 1 );
-        int64_t ntr = lev.trsm_end - lev.trsm_beg;
         PA_PROF_MARK( 2 );
-        if ( ntr > 0 && lev.trsm_inv ) {
-            const int32_t ntl = (int32_t)( lev.tinv_end - lev.tinv_beg );
-            const unsigned gti = (unsigned)( ( ntl + 3 ) / 4 );
-            const int2   *tt   = dev->d_tinv_tasks + lev.tinv_beg;
-            const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
-            T            *Ti   = (T *)dev->d_tinv;
-            const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
-            if ( dev->ldlt ) {
-                /* L panel against D L^t, with its copy L D into the second array */
-                k_tile_inv<T, 4><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-                k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, 0 );
-            }
-            else if ( lu ) {
-                /* both LU modes (L and U^T panels) in one launch each */
-                k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
-                k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, ti2 );
-            }
-            else {
-                k_tile_inv<T, 0><<<gti, 256, 0, dev->stream>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-                k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, tr, L, U, Ti, 0 );
-            }
-            PA_HIP( hipGetLastError() );
-        }
-        else if ( ntr > 0 ) {
-            size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
-            if ( dev->ldlt ) {
-                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                k_ldlt_scalo<T><<<(unsigned)ntr, 256, 0, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-            }
-            else if ( lu ) {
-                k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-                k_trsm<T, 2><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-            }
-            else {
-                k_trsm<T, 0><<<(unsigned)ntr, 256, lds_trsm, dev->stream>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
-            }
-            PA_HIP( hipGetLastError() );
-        }
+        if ( !ahead && ( rc = pa_launch_trsm<T>( dev, lev, dev->stream ) ) ) return rc;
         PA_PROF_MARK( 3 );
         PA_PROF_MARK( 4 );
-        /* the widest tiles first: they hold most of the level's work */
-        for ( int c = PA_NGC - 1; c >= 0; c-- ) {
-            const int64_t ng = lev.gc_end[c] - lev.gc_beg[c];
-            if ( ng <= 0 ) continue;
-            const pa_gemm_task *tk = dev->d_gemm_tasks + lev.gc_beg[c];
-            T *Up = lu ? U : nullptr;
-            int32_t *nerr = dev->d_counters + 3;
-            const unsigned g = (unsigned)ng;
-            switch ( c ) {
-            case 0: k_gemm_level<T, 64, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
-            case 1: k_gemm_level<T, 64, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
-            case 3: k_gemm_level<T, 128, 64><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
-            case 4: k_gemm_level<T, 128, 32><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr ); break;
-            default:
-                if constexpr ( sizeof( T ) == 4 ) {
-                    k_gemm_level<T, 128, 128><<<g, 256, 0, dev->stream>>>( dev->S, tk, L, Up, nerr );
-                }
-                else {
-                    return PASTIX_ERR_INTERNAL; /* 128 x 128 tiles are FP32 only */
-                }
-            }
-            PA_HIP( hipGetLastError() );
+        /* the next level with general cblks, if it directly follows */
+        const bool fork = la && li + 1 < nl && dev->levels[li + 1].fac_end > dev->levels[li + 1].cblk_beg;
+        if ( fork ) {
+            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_beg, lev.gc_mid ) ) ) return rc;
+            const pa_level_t &nx = dev->levels[li + 1];
+            PA_HIP( hipEventRecord( dev->la_ev[2 * ( li + 1 )], dev->stream ) );
+            PA_HIP( hipStreamWaitEvent( dev->stream2, dev->la_ev[2 * ( li + 1 )], 0 ) );
+            if ( ( rc = pa_launch_panel<T>( dev, nx, dev->stream2 ) ) ) return rc;
+            if ( ( rc = pa_launch_trsm<T>( dev, nx, dev->stream2 ) ) ) return rc;
+            PA_HIP( hipEventRecord( dev->la_ev[2 * ( li + 1 ) + 1], dev->stream2 ) );
+            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_mid, lev.gc_end ) ) ) return rc;
         }
+        else {
+            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_beg, lev.gc_end ) ) ) return rc;
+        }
+        ahead = fork;
         PA_PROF_MARK( 5 );
     }
-    if ( !lu && dev->S.cblknbr > 0 ) {
+    if ( ahead ) return PASTIX_ERR_INTERNAL; /* a fork past the last level: not possible */
+    if ( dev->lu == 0 && dev->S.cblknbr > 0 ) {
         /* transposed diagonal blocks for the backward solve sweep */
         k_diag_transpose<T><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, L, (T *)dev->d_DT );
         PA_HIP( hipGetLastError() );

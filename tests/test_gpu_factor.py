@@ -241,3 +241,43 @@ def test_orsirr_fixture_with_a_personal_ordering():
         assert problems.backward_error(A, x, b) <= 1e-12
     finally:
         P.close()
+
+
+@pytest.mark.parametrize("ft,mixed", [(pa.factotype.LLT, False), (pa.factotype.LU, False), (pa.factotype.LLT, True),
+                                      (pa.factotype.LDLT, False)], ids=["llt", "lu", "llt-fp32", "ldlt"])
+def test_lookahead_gives_the_factors_of_the_level_by_level_schedule(ft, mixed, monkeypatch):
+    """With look-ahead (the default) the next level's diagonal blocks and
+    TRSMs run on a second stream while the current level's other GEMMs run;
+    the factors equal those of the level-by-level order
+    (PASTIX_AMD_LOOKAHEAD=0) within the rounding of reordered atomic
+    updates (1e-13 FP64, 1e-6 FP32), through the captured graph and eager."""
+    grid = (22, 20, 18)
+    gen = problems.convection_diffusion_27 if ft == pa.factotype.LU else problems.laplacian
+    A = gen(*grid)
+    if ft == pa.factotype.LDLT:
+        # symmetric indefinite, strictly diagonally dominant (+-7 diagonal):
+        # no pivot growth to amplify the reordered rounding
+        import scipy.sparse as sp
+        sign = np.where(np.random.default_rng(4).random(A.shape[0]) < 0.5, -1.0, 1.0)
+        A = (A - sp.diags(A.diagonal()) + sp.diags(7.0 * sign)).tocsc()
+    out = {}
+    for mode, env in (("seq", {"PASTIX_AMD_LOOKAHEAD": "0"}), ("la", {}),
+                      ("la-eager", {"PASTIX_AMD_GRAPHS": "0"})):
+        for k in ("PASTIX_AMD_LOOKAHEAD", "PASTIX_AMD_GRAPHS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        print("case lookahead", ft, mixed, mode, flush=True)
+        P = Problem(A, grid, ft, mixed=mixed)
+        try:
+            # a second factorization replays the captured launch sequence
+            pa.task_numfact(P.h, P.spm)
+            out[mode] = P.gpu_factors()
+            mask = None if P.lu or ft == pa.factotype.LDLT else P.lower_mask()
+        finally:
+            P.close()
+    tol = 1e-6 if mixed else 1e-13
+    for mode in ("la", "la-eager"):
+        assert _relerr(out[mode][0], out["seq"][0], mask) < tol
+        if out["seq"][1] is not None:
+            assert _relerr(out[mode][1], out["seq"][1]) < tol

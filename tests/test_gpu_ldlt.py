@@ -26,8 +26,25 @@ NOTRANS, TRANS = 111, 112
 
 
 def _indefinite(grid, shift):
+    """The Laplacian shifted into indefiniteness: some Schur complements of
+    the elimination come close to singular, so pivots are small and the
+    factors grow (max |L| ~ 300 times the matrix entries at shift 2.5):
+    rounding differences are amplified accordingly."""
     n = int(np.prod(grid))
     return (problems.laplacian(*grid) - shift * sp.eye(n)).tocsc()
+
+
+def _indefinite_dd(grid, seed):
+    """Symmetric, indefinite and strictly diagonally dominant: the 7-point
+    stencil's -1 couplings with a diagonal of +7 or -7 per cell (random
+    signs).  Every Schur complement of a no-pivot elimination stays strictly
+    diagonally dominant, so the pivots keep a magnitude of at least 1, D has
+    both signs (LL^t would fail) and the factors do not grow: two correct
+    LDL^t's agree to the rounding of their operations."""
+    n = int(np.prod(grid))
+    L = problems.laplacian(*grid)
+    sign = np.where(np.random.default_rng(seed).random(n) < 0.5, -1.0, 1.0)
+    return (L - sp.diags(L.diagonal()) + sp.diags(7.0 * sign)).tocsc()
 
 
 def _offdiag_mask(P):
@@ -59,7 +76,7 @@ def test_ldlt_factors_and_solve_match_the_oracle(fact, trsm, monkeypatch):
         monkeypatch.setenv("PASTIX_AMD_TRSM", "0")
     grid = (12, 11, 10)
     print("case LDL^t", fact, trsm, flush=True)
-    A = _indefinite(grid, 2.5)
+    A = _indefinite_dd(grid, 1)
     P = LdltProblem(A, grid, fact)
     try:
         Lo, Do, npiv = P.oracle_factors()
@@ -85,7 +102,7 @@ def test_ldlt_solve_by_subtasks_with_the_diagonal_step():
     applyorder gives pastix_task_solve's solution (reference
     sopalin/pastix_task_solve.c:596-640 for LDL^t)."""
     grid = (10, 9, 8)
-    A = _indefinite(grid, 1.5)
+    A = _indefinite_dd(grid, 2)
     P = LdltProblem(A, grid, pa.factotype.LDLT)
     lib = _lib.load()
     try:
@@ -109,20 +126,25 @@ def test_ldlt_solve_by_subtasks_with_the_diagonal_step():
 
 @pytest.mark.parametrize("refine", [pa.refine.GMRES, pa.refine.BiCGSTAB], ids=["gmres", "bicgstab"])
 def test_ldlt_fp32_factors_refined_to_1e12(refine):
-    """Mixed precision LDL^t: FP32 factors on the GPU (and the FP32 oracle's
-    within the FP32 rounding), FP64 refinement to backward error 1e-12."""
+    """Mixed precision LDL^t: FP32 factors on the GPU equal the FP32
+    oracle's within the FP32 rounding (diagonally dominant indefinite
+    matrix), and FP64 refinement reaches backward error 1e-12 on it and on
+    the shifted Laplacian, whose small pivots leave the FP32 solve far from
+    the solution."""
     grid = (14, 13, 12)
-    A = _indefinite(grid, 0.7)
-    P = LdltProblem(A, grid, pa.factotype.LDLT, mixed=True)
-    try:
-        P.iparm[pa.iparm.refinement] = refine
-        Lg, Dg = P.gpu_factors()
-        Lo, Do, npiv = P.oracle_factors(dtype=np.float32)
-        assert np.abs(Lg - Lo).max() / np.abs(Lo).max() <= 1e-4
-        b = A @ problems.uniform(P.n, seed=5)
-        x = pa.task_solve(P.h, P.spm, b.copy())
-        assert problems.backward_error(A, x, b) > 1e-10
-        pa.task_refine(P.h, P.spm, b, x)
-        assert problems.backward_error(A, x, b) <= 1e-12
-    finally:
-        P.close()
+    for name, A in (("dd", _indefinite_dd(grid, 3)), ("shifted", _indefinite(grid, 0.7))):
+        print("case", name, flush=True)
+        P = LdltProblem(A, grid, pa.factotype.LDLT, mixed=True)
+        try:
+            P.iparm[pa.iparm.refinement] = refine
+            if name == "dd":
+                Lg, Dg = P.gpu_factors()
+                Lo, Do, npiv = P.oracle_factors(dtype=np.float32)
+                assert np.abs(Lg - Lo).max() / np.abs(Lo).max() <= 1e-5
+            b = A @ problems.uniform(P.n, seed=5)
+            x = pa.task_solve(P.h, P.spm, b.copy())
+            assert problems.backward_error(A, x, b) > 1e-10
+            pa.task_refine(P.h, P.spm, b, x)
+            assert problems.backward_error(A, x, b) <= 1e-12
+        finally:
+            P.close()
