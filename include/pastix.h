@@ -117,6 +117,21 @@ int pastix_amd_factors_save( pastix_data_t *pastix_data, const char *path );
 int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
 
 /*
+ * pastix_amd_solve_dev without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, same kernels and same results; the permuted right-hand
+ * sides (and their FP32 copy for FP32 factors) live in a static 64 MiB
+ * device array per GPU, so n * nrhs may be at most 8,388,576 for FP64
+ * factors and 5,592,384 for FP32 factors; larger calls return
+ * PASTIX_ERR_BADPARAMETER before any launch.  Launches are eager (no graph)
+ * and the call returns after the solve has finished.  The array is shared by
+ * every handle on a GPU: calls that overlap on one GPU race.
+ */
+int pastix_amd_solve_dev_noalloc( pastix_data_t *pastix_data, pastix_trans_t transA,
+                                  pastix_int_t nrhs, void *dB, pastix_int_t ldb );
+
+/*
  * Refine X on the device: FP64 GMRES/CG/SR (IPARM_REFINEMENT) with the
  * factorization as preconditioner, device SpMV on the permuted CSC and device
  * norms.  dB and dX are device FP64 buffers in the user numbering.

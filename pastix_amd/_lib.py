@@ -118,6 +118,7 @@ PROTOTYPES = {
     "pastixOrderGet": (_P, [_P]),
     "pastixOrderGrid": (C.c_int, [_P, _I, _I, _I]),
     "pastix_amd_solve_dev": (C.c_int, [_P, C.c_int, _I, _P, _I]),
+    "pastix_amd_solve_dev_noalloc": (C.c_int, [_P, C.c_int, _I, _P, _I]),
     "pastix_amd_factors_save": (C.c_int, [_P, C.c_char_p]),
     "pastix_amd_factors_load": (C.c_int, [_P, C.c_char_p]),
     "coeftab_ddump": (None, [_P, _P, C.c_char_p]),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include "kernels.hpp"
 
 extern "C" {
@@ -40,6 +41,17 @@ double pa_flops_sytrf( double n );
             return PASTIX_ERR_INTERNAL;                                                      \
         }                                                                                    \
     } while ( 0 )
+
+/* Scratch of the allocation-free solve (pa_dev_solve_noalloc): the
+ * right-hand sides in the factorization numbering (FP64, n*nrhs, padded to
+ * 256 bytes) followed by their FP32 copy when the factors are FP32.  One copy
+ * per device, held until the process ends and shared by every handle on that
+ * device; its address on each device is resolved once (pa_scratch_resolve)
+ * and only that address is passed to kernels and copies. */
+#define PA_SCRATCH_BYTES ( (size_t)64 << 20 )
+#define PA_SCRATCH_MAXDEV 64
+static __device__ __align__( 256 ) unsigned char scratch[PA_SCRATCH_BYTES];
+static void *pa_scratch_addr[PA_SCRATCH_MAXDEV];
 
 /* Scatter-GEMM tile classes (rho rows x sigma columns of the output tile):
  * a column group narrower than the tile would pad the MFMA work, so the
@@ -155,6 +167,7 @@ struct pa_device_s {
     int64_t xs_len;
     double *d_xtmp;    /* permuted rhs buffer */
     int64_t xtmp_len;
+    void   *d_scratch = nullptr;   /* this device's `scratch` array */
     std::vector<pa_wblock> wpool;   /* pa_dev_wget / pa_dev_wput */
     /* host copy of the structure the schedule is built from */
     std::vector<int32_t> h_c_ncol, h_c_stride, h_c_fblok, h_b_frow, h_b_lrow, h_b_fcblk, h_b_rowoff;
@@ -241,6 +254,34 @@ pa_set_lds_limits( int bytes )
     pa_lds_attr( k_diag<float, true, 64, true>, bytes );    pa_lds_attr( k_diag<float, true, 1024, true>, bytes );
 }
 
+/* Device address of `scratch` on `device` (the current device), looked up
+ * once per device and kept: each device holds its own copy of the array, and
+ * the host-side name is not a device address. */
+static int
+pa_scratch_resolve( int device, void **addr )
+{
+    static std::mutex lock;
+    if ( device < 0 || device >= PA_SCRATCH_MAXDEV ) {
+        fprintf( stderr, "pastix_amd: device %d is beyond the %d devices the solve scratch serves\n", device,
+                 PA_SCRATCH_MAXDEV );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    std::lock_guard<std::mutex> g( lock );
+    if ( !pa_scratch_addr[device] ) {
+        void      *p = nullptr;
+        hipError_t e = hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) );
+        if ( e != hipSuccess || p == nullptr ) {
+            fprintf( stderr, "pastix_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+                     hipGetErrorName( e ) );
+            (void)hipGetLastError();
+            return PASTIX_ERR_INTERNAL;
+        }
+        pa_scratch_addr[device] = p;
+    }
+    *addr = pa_scratch_addr[device];
+    return PASTIX_SUCCESS;
+}
+
 extern "C" int
 pa_dev_open( pastix_data_t *pd )
 {
@@ -258,6 +299,10 @@ pa_dev_open( pastix_data_t *pd )
     if ( genv && atoi( genv ) == 0 ) dev->use_graphs = 0;
     if ( env ) dev->device = atoi( env );
     PA_HIP( hipSetDevice( dev->device ) );
+    if ( pa_scratch_resolve( dev->device, &dev->d_scratch ) != PASTIX_SUCCESS ) {
+        delete dev;
+        return PASTIX_ERR_INTERNAL;
+    }
     PA_HIP( hipStreamCreate( &dev->stream ) );
     PA_HIP( hipStreamCreate( &dev->stream2 ) );
     {
@@ -1623,6 +1668,51 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
     pa_sweep_op ops[3];
     const int   nops = pa_solve_ops( DEV( pd )->fact_kind, trans, ops );
     return pa_dev_run_ops( pd, ops, nops, nrhs, d_x, ld );
+}
+
+/* Largest n * nrhs the allocation-free solve takes for FP32 factors (FP64
+ * and FP32 copies in the scratch) and for FP64 factors */
+extern "C" int64_t
+pa_dev_noalloc_max( pastix_data_t *pd )
+{
+    const pa_device_s *dev = DEV( pd );
+    const size_t       per = sizeof( double ) + ( dev->prec != PA_PREC_DOUBLE ? sizeof( float ) : 0 );
+    return (int64_t)( ( PA_SCRATCH_BYTES - 256 ) / per );
+}
+
+/* The full solve of pa_dev_solve_perm plus the permutations P b and P^T x,
+ * on d_b in the user numbering, with no device memory allocated: the
+ * permuted right-hand sides and their FP32 copy live in this device's
+ * `scratch` array (each fully written before it is read: nothing needs
+ * clearing).  Eager launches on the handle's stream, then a stream
+ * synchronization.  Calls that overlap on one device share the array and
+ * race. */
+extern "C" int
+pa_dev_solve_noalloc( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double *d_b, int64_t ldb )
+{
+    pa_device_s *dev = DEV( pd );
+    PA_HIP( hipSetDevice( dev->device ) );
+    pa_sweep_op  ops[3];
+    const int    nops = pa_solve_ops( dev->fact_kind, trans, ops );
+    if ( pa_check_ops( dev, ops, nops ) ) return PASTIX_ERR_BADPARAMETER;
+    const int64_t n  = dev->n;
+    const size_t  xb = ( ( (size_t)n * nrhs * sizeof( double ) ) + 255 ) & ~(size_t)255;
+    const size_t  need = xb + ( dev->prec != PA_PREC_DOUBLE ? (size_t)n * nrhs * sizeof( float ) : 0 );
+    if ( nrhs < 1 || need > PA_SCRATCH_BYTES || !dev->d_scratch ) {
+        fprintf( stderr,
+                 "pastix_amd_solve_dev_noalloc: n * nrhs = %lld needs %zu bytes of scratch, the limit is %zu\n",
+                 (long long)( n * nrhs ), need, (size_t)PA_SCRATCH_BYTES );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( n == 0 ) return PASTIX_SUCCESS;
+    double *xp = (double *)dev->d_scratch;
+    float  *xs = (float *)( (char *)dev->d_scratch + xb );
+    int     rc = pa_dev_apply_perm( pd, 1, nrhs, d_b, ldb, xp, n );
+    if ( !rc ) rc = pa_launch_ops( dev, ops, nops, nrhs, xp, n, xs );
+    if ( !rc ) rc = pa_dev_apply_perm( pd, 0, nrhs, xp, n, d_b, ldb );
+    if ( rc ) return rc;
+    PA_HIP( hipStreamSynchronize( dev->stream ) );
+    return PASTIX_SUCCESS;
 }
 
 /* One triangular sweep in the factorization numbering (reference

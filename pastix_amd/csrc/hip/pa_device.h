@@ -47,6 +47,12 @@ int    pa_dev_factorize( pastix_data_t *pastix_data, double criterion, int32_t *
  * numbering (nrhs columns, leading dimension ld) */
 int    pa_dev_solve_perm( pastix_data_t *pastix_data, pastix_trans_t trans, int32_t nrhs,
                           double *d_x, int64_t ld );
+/* pa_dev_solve_perm plus the permutations, on a user-numbering device FP64
+ * buffer, with no device allocation (the device's static scratch array);
+ * pa_dev_noalloc_max gives the largest n * nrhs it takes */
+int     pa_dev_solve_noalloc( pastix_data_t *pastix_data, pastix_trans_t trans, int32_t nrhs, double *d_b,
+                              int64_t ldb );
+int64_t pa_dev_noalloc_max( pastix_data_t *pastix_data );
 /* factor arrays to and from host memory: what HBM holds (arithmetic, two
  * arrays, factorization kind, coefficients per array); raw copies of count
  * coefficients at offset off of array side (0: L, 1: second array) in the

@@ -814,6 +814,35 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrh
     return rc;
 }
 
+/* pastix_amd_solve_dev with no device allocation: the device layer's static
+ * scratch array holds the permuted right-hand sides (no counterpart in the
+ * reference; for callers that cannot allocate). */
+int
+pastix_amd_solve_dev_noalloc( pastix_data_t *pd, pastix_trans_t transA, pastix_int_t nrhs, void *dB,
+                              pastix_int_t ldb )
+{
+    pastix_int_t n;
+    int rc;
+    double t0;
+    if ( pd == NULL || !( pd->steps & STEP_NUMFACT ) ) {
+        pa_error( "pastix_amd_solve_dev_noalloc: all steps up to pastix_task_numfact() must be done first" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    n = pd->solvmatr->nodenbr;
+    if ( nrhs < 1 || ldb < n || dB == NULL ) return PASTIX_ERR_BADPARAMETER;
+    if ( transA != PastixNoTrans && transA != PastixTrans ) return PASTIX_ERR_BADPARAMETER;
+    if ( (int64_t)n * nrhs > pa_dev_noalloc_max( pd ) ) {
+        pa_error( "pastix_amd_solve_dev_noalloc: n * nrhs = %ld exceeds the scratch limit %ld", (long)( n * nrhs ),
+                  (long)pa_dev_noalloc_max( pd ) );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    t0 = pa_wtime();
+    rc = pa_dev_solve_noalloc( pd, transA, (int32_t)nrhs, dB, ldb );
+    pd->dparm[DPARM_SOLV_TIME] = pa_wtime() - t0;
+    if ( !rc ) pd->steps |= STEP_SOLVE;
+    return rc;
+}
+
 /* Host-buffer solve (reference sopalin/pastix_task_solve.c:758): B is
  * overwritten by the solution; data crosses PCIe both ways.  The
  * permutations P b and P^T x run on the device. */
