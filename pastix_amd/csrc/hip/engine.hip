@@ -68,8 +68,12 @@ struct pa_level_t {
     int64_t sm_end[3];            /* by row slots: [fac_end, sm_end[0]) of  */
                                   /* <= 64 rows, then <= 128, <= 192        */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
+    int64_t trsm_mid;             /* [trsm_mid, trsm_end): identity slabs of
+                                     the diagonal-block inverses            */
     int64_t tinv_beg, tinv_end;   /* into d_tinv_tasks (diagonal tiles)     */
     bool    trsm_inv;             /* TRSM by inverted tiles (k_trsm_inv)    */
+    bool    dinv;                 /* the TRSM launch also inverts the level's
+                                     diagonal blocks (solve by products)    */
     int64_t gc_beg[PA_NGC], gc_end[PA_NGC]; /* into d_gemm_tasks, per tile class */
     int64_t gc_mid[PA_NGC];       /* [gc_beg, gc_mid): tasks whose target is a
                                      cblk of the next level (look-ahead)   */
@@ -139,6 +143,11 @@ struct pa_device_s {
     int2         *d_tinv_tasks = nullptr;
     void         *d_tinv = nullptr;
     int64_t       tinv_tiles = 0;
+    /* inverses of the lower triangles of the diagonal blocks (offsets
+     * c_dtoff, column major), one region per factor array: L, and the U^T
+     * array of LU; written by the identity tasks of the TRSM launches */
+    void         *d_dinv = nullptr;
+    bool          all_dinv = false;   /* every level inverts its diagonal blocks */
     /* matrix */
     int64_t *d_colptr;
     int32_t *d_rowtab;
@@ -229,6 +238,12 @@ template <typename T, bool UNIT>
 static void
 pa_lds_attr_sd( int bytes )
 {
+    if ( !UNIT ) {
+        pa_lds_attr( k_solve_dinv<T, false, 64>, bytes );  pa_lds_attr( k_solve_dinv<T, true, 64>, bytes );
+        pa_lds_attr( k_solve_dinv<T, false, 128>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 128>, bytes );
+        pa_lds_attr( k_solve_dinv<T, false, 256>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 256>, bytes );
+        pa_lds_attr( k_solve_dinv<T, false, 512>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 512>, bytes );
+    }
     pa_lds_attr( k_solve_diag_fwd<T, UNIT, 64>, bytes );  pa_lds_attr( k_solve_diag_bwd<T, UNIT, 64>, bytes );
     pa_lds_attr( k_solve_diag_fwd<T, UNIT, 128>, bytes ); pa_lds_attr( k_solve_diag_bwd<T, UNIT, 128>, bytes );
     pa_lds_attr( k_solve_diag_fwd<T, UNIT, 256>, bytes ); pa_lds_attr( k_solve_diag_bwd<T, UNIT, 256>, bytes );
@@ -394,8 +409,8 @@ pa_free_structure( pa_device_s *dev )
                      dev->d_solve_tasks, dev->d_bsolve_tasks, dev->d_c_dtoff, dev->d_r_base, dev->d_r_info,
                      dev->d_r_blk };
     for ( void *p : ptrs ) hipFree( p );
-    hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv );
-    dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL;
+    hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv ); hipFree( dev->d_dinv );
+    dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL; dev->d_dinv = NULL;
     dev->tinv_tiles = 0;
     dev->S.c_tinv = NULL;
     dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
@@ -581,8 +596,8 @@ pa_build_schedule( pastix_data_t *pd )
         for ( void *p : ptrs ) hipFree( p );
         dev->d_lev_cblks = NULL; dev->d_trsm_tasks = NULL; dev->d_gemm_tasks = NULL; dev->d_solve_tasks = NULL;
         dev->d_bsolve_tasks = NULL;
-        hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv );
-        dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL;
+        hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv ); hipFree( dev->d_dinv );
+        dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL; dev->d_dinv = NULL;
         dev->tinv_tiles = 0;
         dev->levels.clear();
     }
@@ -611,6 +626,11 @@ pa_build_schedule( pastix_data_t *pd )
     /* PASTIX_AMD_TRSM=0 keeps the substitution TRSM (k_trsm) on every level */
     const char *tr_env  = getenv( "PASTIX_AMD_TRSM" );
     const bool  tr_inv  = !( tr_env && tr_env[0] == '0' );
+    /* PASTIX_AMD_DINV=0 keeps the substitution diagonal solves */
+    const char *di_env  = getenv( "PASTIX_AMD_DINV" );
+    const bool  dinv_on = !( di_env && di_env[0] == '0' );
+    bool        any_dinv = false, all_dinv = !ldlt && dinv_on;
+    std::vector<int2> ident_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gtasks[PA_NGC];
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
@@ -695,6 +715,11 @@ pa_build_schedule( pastix_data_t *pd )
                 }
             }
         }
+        /* TRSM by inverted tiles for the level; the same launch then also
+         * inverts the diagonal blocks for the solve (LL^t and LU; LDL^t
+         * solves by substitution) */
+        const bool lev_inv  = tr_inv && L.ncmax_fac <= PA_TI_MAXNC;
+        const bool lev_dinv = lev_inv && !ldlt && dinv_on;
         for ( int32_t k : bylev[l] ) {
             int32_t nc = c_ncol[k], R = c_stride[k] - nc;
             if ( is_small( k ) ) continue; /* k_small_llt / k_small_solve */
@@ -706,7 +731,11 @@ pa_build_schedule( pastix_data_t *pd )
             for ( int32_t r = 0; r < R; r += 1024 / ncg )
                 for ( int32_t c = 0; c < nc; c += 16 * ncg ) bsolve_tasks.push_back( make_int4( k, r, c, 0 ) );
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
-            if ( R > 0 ) {
+            /* identity rows r0.. of the diagonal block, as (k, -1 - r0),
+             * after the level's panel slabs */
+            if ( lev_dinv )
+                for ( int32_t r = 0; r < nc; r += PA_TRSM_TR ) ident_tasks.push_back( make_int2( k, -1 - r ) );
+            if ( R > 0 || lev_dinv ) {
                 c_tinv[k] = tinv_tiles;
                 for ( int32_t tl = 0; tl < ( nc + 15 ) / 16; tl++ ) tinv_tasks.push_back( make_int2( k, tl ) );
                 tinv_tiles += ( nc + 15 ) / 16;
@@ -758,9 +787,15 @@ pa_build_schedule( pastix_data_t *pd )
         }
         for ( int32_t F : touched ) { srccount[F] = 0; lastsrc[F] = -1; }
         L.cblk_end  = lev_cblks.size();
-        L.trsm_end  = trsm_tasks.size();
         L.tinv_end  = tinv_tasks.size();
-        L.trsm_inv  = tr_inv && L.ncmax_fac <= PA_TI_MAXNC;
+        L.trsm_mid  = trsm_tasks.size();
+        trsm_tasks.insert( trsm_tasks.end(), ident_tasks.begin(), ident_tasks.end() );
+        ident_tasks.clear();
+        L.trsm_end  = trsm_tasks.size();
+        L.trsm_inv  = lev_inv;
+        L.dinv      = lev_dinv;
+        any_dinv    = any_dinv || lev_dinv;
+        if ( L.fac_end > L.cblk_beg && !lev_dinv ) all_dinv = false;
         for ( int c = 0; c < PA_NGC; c++ ) L.gc_end[c] = gtasks[c].size();
         /* look-ahead order: the tasks updating cblks of the next level
          * first (the next level's panels can start once they are done) */
@@ -774,6 +809,7 @@ pa_build_schedule( pastix_data_t *pd )
     }
     dev->lu   = two;
     dev->ldlt = ldlt;
+    dev->all_dinv = all_dinv && any_dinv;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_tinv_tasks, tinv_tasks ) ) ) return rc;
@@ -781,6 +817,9 @@ pa_build_schedule( pastix_data_t *pd )
     dev->tinv_tiles = tinv_tiles;
     /* inverses in the widest precision, one region per TRSM mode */
     PA_HIP( hipMalloc( &dev->d_tinv, (size_t)std::max<int64_t>( tinv_tiles, 1 ) * 256 * sizeof( double ) * ( lu ? 2 : 1 ) ) );
+    /* inverted diagonal blocks, one region per factor array */
+    if ( any_dinv )
+        PA_HIP( hipMalloc( &dev->d_dinv, (size_t)std::max<int64_t>( dev->dtnbr, 1 ) * sizeof( double ) * ( lu ? 2 : 1 ) ) );
     dev->S.c_tinv = dev->d_c_tinv;
     /* the task lists of all classes in one array, class after class */
     std::vector<pa_gemm_task> gemm_tasks;
@@ -1056,11 +1095,19 @@ pa_dev_factor_put( pastix_data_t *pd, int side, int64_t off, int64_t count, cons
 
 /* after the factor arrays are in place: the transposed diagonal blocks of
  * the LL^t solve, as at the end of a factorization */
+template <typename T> static int pa_launch_dinv( pa_device_s *dev, const pa_level_t &lev, hipStream_t st );
+
 extern "C" int
 pa_dev_factor_end( pastix_data_t *pd )
 {
     pa_device_s *dev = DEV( pd );
     PA_HIP( hipSetDevice( dev->device ) );
+    /* the inverted diagonal blocks of the solve, as a factorization leaves them */
+    for ( const pa_level_t &lev : dev->levels ) {
+        int rc = ( dev->prec == PA_PREC_DOUBLE ) ? pa_launch_dinv<double>( dev, lev, dev->stream )
+                                                 : pa_launch_dinv<float>( dev, lev, dev->stream );
+        if ( rc ) return rc;
+    }
     if ( !dev->lu && dev->S.cblknbr > 0 ) {
         if ( dev->prec == PA_PREC_DOUBLE )
             k_diag_transpose<double><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, (const double *)dev->d_L,
@@ -1170,19 +1217,21 @@ pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
         const int2   *tr   = dev->d_trsm_tasks + lev.trsm_beg;
         T            *Ti   = (T *)dev->d_tinv;
         const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
+        T            *Di   = (T *)dev->d_dinv;      /* inverted diagonal blocks (identity tasks) */
+        const int64_t di2  = dev->dtnbr;             /* LU: the U^T array's inverses */
         if ( dev->ldlt ) {
             /* L panel against D L^t, with its copy L D into the second array */
             k_tile_inv<T, 4><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0 );
+            k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         else if ( lu ) {
             /* both LU modes (L and U^T panels) in one launch each */
             k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
-            k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2 );
+            k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2, Di, di2 );
         }
         else {
             k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0 );
+            k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         PA_HIP( hipGetLastError() );
     }
@@ -1201,6 +1250,36 @@ pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
         }
         PA_HIP( hipGetLastError() );
     }
+    return PASTIX_SUCCESS;
+}
+
+/* The tile inverses and the diagonal-block inverses of one level alone (the
+ * identity slabs of its TRSM task list), for factors that did not come from
+ * a factorization on this instance */
+template <typename T>
+static int
+pa_launch_dinv( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
+{
+    T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
+    const int64_t nid = lev.trsm_end - lev.trsm_mid;
+    if ( !lev.dinv || nid <= 0 ) return PASTIX_SUCCESS;
+    const int32_t ntl = (int32_t)( lev.tinv_end - lev.tinv_beg );
+    const unsigned gti = (unsigned)( ( ntl + 3 ) / 4 );
+    const int2   *tt   = dev->d_tinv_tasks + lev.tinv_beg;
+    const int2   *tr   = dev->d_trsm_tasks + lev.trsm_mid;
+    T            *Ti   = (T *)dev->d_tinv;
+    const int64_t ti2  = dev->tinv_tiles * 256;
+    T            *Di   = (T *)dev->d_dinv;
+    const int64_t di2  = dev->dtnbr;
+    if ( dev->lu ) {
+        k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
+        k_trsm_inv<T, 3><<<dim3( (unsigned)nid, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2, Di, di2 );
+    }
+    else {
+        k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+        k_trsm_inv<T, 0><<<(unsigned)nid, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
+    }
+    PA_HIP( hipGetLastError() );
     return PASTIX_SUCCESS;
 }
 
@@ -1305,8 +1384,10 @@ pa_factorize_t( pa_device_s *dev )
         PA_PROF_MARK( 5 );
     }
     if ( ahead ) return PASTIX_ERR_INTERNAL; /* a fork past the last level: not possible */
-    if ( dev->lu == 0 && dev->S.cblknbr > 0 ) {
-        /* transposed diagonal blocks for the backward solve sweep */
+    if ( dev->lu == 0 && dev->S.cblknbr > 0 && !dev->all_dinv ) {
+        /* transposed diagonal blocks for the substitution backward sweeps
+         * (with inverted diagonal blocks on every level, a solve that needs
+         * them makes them itself) */
         k_diag_transpose<T><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, L, (T *)dev->d_DT );
         PA_HIP( hipGetLastError() );
     }
@@ -1437,14 +1518,25 @@ pa_solve_diag_launch( pa_device_s *dev, dim3 grid, size_t lds, const int32_t *cb
 template <typename T, bool BWD, bool UNIT>
 static int
 pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off, T *x,
-                     int32_t nrhs, int64_t ld )
+                     int32_t nrhs, int64_t ld, const T *Dinv )
 {
     /* the general cblks of the level; the narrow ones go to k_small_solve */
     const int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg );
     const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
     const int32_t ncmax = lev.ncmax_fac;
     if ( ncb == 0 ) return PASTIX_SUCCESS;
-    if ( ncmax <= PA_SD_MAXCOL ) {
+    if ( Dinv && lev.dinv ) {
+        /* products with the inverted diagonal blocks */
+        const size_t  avail  = (size_t)dev->lds_limit - 4096;
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)ncmax * sizeof( T ) ) ) );
+        dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
+        size_t        lds = (size_t)ncmax * rchunk * sizeof( T );
+        if ( ncmax <= 64 )       k_solve_dinv<T, BWD, 64><<<grid, 64, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
+        else if ( ncmax <= 128 ) k_solve_dinv<T, BWD, 128><<<grid, 128, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
+        else if ( ncmax <= 256 ) k_solve_dinv<T, BWD, 256><<<grid, 256, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
+        else                     k_solve_dinv<T, BWD, 512><<<grid, 512, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
+    }
+    else if ( ncmax <= PA_SD_MAXCOL ) {
         const size_t  avail  = (size_t)dev->lds_limit - 4096;
         const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)ncmax * sizeof( T ) ) ) );
         dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
@@ -1468,15 +1560,15 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
  * transposes of C's diagonal blocks) over every level. */
 template <typename T>
 static int
-pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, int64_t ld )
+pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, int64_t ld, const T *Dinv )
 {
     const size_t nl = dev->levels.size();
     int rc;
     for ( size_t li = 0; li < nl; li++ ) {
         const pa_level_t &lev = dev->levels[li];
         if ( lev.cblk_end == lev.cblk_beg ) continue;
-        rc = unit_f ? pa_solve_diag_level<T, false, true>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld )
-                    : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld );
+        rc = unit_f ? pa_solve_diag_level<T, false, true>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld, Dinv )
+                    : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld, Dinv );
         if ( rc ) return rc;
         int64_t ns = lev.solve_end - lev.solve_beg;
         if ( ns > 0 && nrhs == 1 )
@@ -1495,7 +1587,7 @@ pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, 
 template <typename T>
 static int
 pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off, T *x, int32_t nrhs,
-                int64_t ld )
+                int64_t ld, const T *Dinv )
 {
     const size_t nl = dev->levels.size();
     int rc;
@@ -1512,8 +1604,8 @@ pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const 
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
         if ( nsm > 0 ) pa_small_solve_launch<T, true>( dev, lev, Cb, x, nrhs, ld );
         PA_HIP( hipGetLastError() );
-        rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld )
-                    : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld );
+        rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld, Dinv )
+                    : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld, Dinv );
         if ( rc ) return rc;
     }
     return PASTIX_SUCCESS;
@@ -1544,8 +1636,14 @@ pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t 
     for ( int o = 0; o < nops; o++ ) {
         const pa_sweep_op &op = ops[o];
         const T *C = (const T *)( op.side ? dev->d_U : dev->d_L );
+        /* the inverted diagonal blocks hold the inverse of the side's lower
+         * triangle with the diagonal the factorization gives it (LL^t: L
+         * non-unit; LU: L unit, U^T non-unit); other requests substitute */
+        const int natural_unit = ( dev->fact_kind == PastixFactLU && op.side == 0 ) ? 1 : 0;
+        const T  *Dinv = ( dev->d_dinv && !dev->ldlt && op.unit == natural_unit )
+                             ? (const T *)dev->d_dinv + ( op.side ? dev->dtnbr : 0 ) : nullptr;
         if ( op.kind == PA_OP_FWD ) {
-            if ( ( rc = pa_sweep_fwd_t<T>( dev, C, op.unit != 0, x, nrhs, ld ) ) ) return rc;
+            if ( ( rc = pa_sweep_fwd_t<T>( dev, C, op.unit != 0, x, nrhs, ld, Dinv ) ) ) return rc;
         }
         else if ( op.kind == PA_OP_BWD ) {
             /* transposed diagonal blocks of the swept factor: for two-array
@@ -1554,7 +1652,14 @@ pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t 
             const bool     two = dev->d_U != nullptr;
             const int64_t *ct_off = two ? dev->d_c_coef : dev->d_c_dtoff;
             const T       *ct     = (const T *)( two ? ( op.side ? dev->d_L : dev->d_U ) : dev->d_DT );
-            if ( ( rc = pa_sweep_bwd_t<T>( dev, C, op.unit != 0, ct, ct_off, x, nrhs, ld ) ) ) return rc;
+            if ( !two && dev->all_dinv && !Dinv && dev->S.cblknbr > 0 ) {
+                /* a substitution sweep where the factorization left only the
+                 * inverted blocks: transpose the diagonal blocks first */
+                k_diag_transpose<T><<<dev->S.cblknbr, 256, 0, dev->stream>>>( dev->S, (const T *)dev->d_L,
+                                                                             (T *)dev->d_DT );
+                PA_HIP( hipGetLastError() );
+            }
+            if ( ( rc = pa_sweep_bwd_t<T>( dev, C, op.unit != 0, ct, ct_off, x, nrhs, ld, Dinv ) ) ) return rc;
         }
         else {
             if ( dev->n > 0 ) {

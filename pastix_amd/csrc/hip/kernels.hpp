@@ -1222,7 +1222,7 @@ k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T
 
 template <typename T, int MODE>
 __global__ __launch_bounds__( 256 ) void
-k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_t tinv2 )
+k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_t tinv2, T *dinv, int64_t dinv2 )
 {
     /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y.  MODE 4
      * (LDL^t, reference cpucblk_zsytrfsp1d_panel, kernels/core_zsytrfsp.c:
@@ -1241,16 +1241,25 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_
     const int t = threadIdx.x, lane = t & 63, li = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane( t >> 6 );
 
-    const int2    task = tasks[blockIdx.x];
-    const int32_t k = task.x, r0 = task.y;
+    /* a task (k, -1 - r0) solves identity rows r0.. instead of panel rows:
+     * row r of T^{-1} is stored as column r of the cblk's inverse block, so
+     * that block is the column-major inverse of the lower triangle whose
+     * transpose is T (mode 1: the U^T array's, into the second region) */
+    const int2    task  = tasks[blockIdx.x];
+    const bool    ident = task.y < 0;
+    const int32_t k = task.x, r0 = ident ? -1 - task.y : task.y;
     const int32_t nc = S.c_ncol[k], R = S.c_stride[k] - nc;
-    const int32_t rows = R - r0 < SR ? R - r0 : SR;
+    const int32_t rows = ident ? min( nc - r0, SR ) : ( R - r0 < SR ? R - r0 : SR );
     const int32_t nt = ( nc + 15 ) >> 4;
-    T            *panel = ( md == 2 ? Uc : Lc ) + S.c_coef[k];
+    T            *panel = ident ? dinv + ( md == 1 ? dinv2 : 0 ) + S.c_dtoff[k] : ( md == 2 ? Uc : Lc ) + S.c_coef[k];
     const T      *Q     = ( md == 1 ? Uc : Lc ) + S.c_coef[k];
     const T      *Mi    = tinv + S.c_tinv[k] * 256;
 
-    if ( t < SR ) {
+    if ( t < SR && ident ) {
+        xoff[t] = (int64_t)( r0 + min( t, rows - 1 ) ) * nc;
+        xld[t]  = 1;
+    }
+    else if ( t < SR ) {
         if ( t < rows ) {
             int32_t x = nc + r0 + t;
             int32_t b = pa_find_rowblk( S.b_rowoff, S.c_fblok[k] + 1, S.c_fblok[k + 1] - 1, x );
@@ -1287,8 +1296,13 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_
             const int32_t cc  = min( col, nc - 1 );
 #pragma unroll
             for ( int rt = 0; rt < NRT; rt++ ) {
-                const T v = panel[ro[rt] + (int64_t)cc * rl[rt]];
-                acc[rt][jj][s] = ( rv[rt] && col < nc ) ? v : T( 0 );
+                if ( ident ) {
+                    acc[rt][jj][s] = ( rv[rt] && r0 + rt * 16 + li == col ) ? T( 1 ) : T( 0 );
+                }
+                else {
+                    const T v = panel[ro[rt] + (int64_t)cc * rl[rt]];
+                    acc[rt][jj][s] = ( rv[rt] && col < nc ) ? v : T( 0 );
+                }
             }
         }
     }
@@ -1364,7 +1378,7 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_
                 for ( int rt = 0; rt < NRT; rt++ )
                     if ( rv[rt] && col < nc ) {
                         panel[ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s];
-                        if ( MODE == 4 ) Uc[S.c_coef[k] + ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s] * d;
+                        if ( MODE == 4 && !ident ) Uc[S.c_coef[k] + ro[rt] + (int64_t)col * rl[rt]] = acc[rt][jj][s] * d;
                     }
             }
         }
@@ -2082,6 +2096,77 @@ k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64
     for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
         int32_t i = idx % nc, r = idx / nc;
         x[fcol + i + ( r0 + r ) * ldx] = xs[r * nc + i];
+    }
+}
+
+/* Diagonal-block solves by products with the inverted blocks that the TRSM
+ * launches write (M_k = inverse of the lower triangle of the swept factor's
+ * diagonal block, column major, c_dtoff offsets, zeros above the diagonal):
+ * forward x_k <- M_k x_k, backward x_k <- M_k^T x_k.  Every load is
+ * independent of the others (no dependent chain as in substitution).  One
+ * workgroup of TH threads per (cblk, chunk of rchunk right-hand sides); the
+ * cblk's right-hand sides are staged in LDS and the results go straight to
+ * x.  Forward: thread i accumulates row i over the columns up to its wave's
+ * last row (coalesced column reads).  Backward: a wave takes 16 columns at a
+ * time, its lanes run down the rows from the group's first column and the
+ * 16 column sums are reduced by the transposing butterfly. */
+template <typename T, bool BWD, int TH>
+__global__ __launch_bounds__( TH ) void
+k_solve_dinv( pa_dstruct S, const int32_t *cblks, const T *Dinv, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
+{
+    extern __shared__ unsigned char pa_smem_raw[];
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
+    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
+    const T      *M = Dinv + S.c_dtoff[k];
+    const int     t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        const int32_t i = idx % nc, r = idx / nc;
+        xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
+    }
+    __syncthreads();
+    if ( !BWD ) {
+        for ( int32_t i0 = 0; i0 < nc; i0 += TH ) {
+            const int32_t i    = i0 + t;
+            const int32_t ic   = min( i, nc - 1 );
+            const int32_t cend = min( i0 + 64 * wave + 64, nc ); /* columns past the wave's last row are zero */
+            for ( int32_t r = 0; r < nr; r++ ) {
+                const T *xr  = xs + r * nc;
+                T        acc = T( 0 );
+                int32_t  c   = 0;
+                for ( ; c + 4 <= cend; c += 4 ) {
+                    const T m0 = M[ic + (int64_t)c * nc], m1 = M[ic + (int64_t)( c + 1 ) * nc];
+                    const T m2 = M[ic + (int64_t)( c + 2 ) * nc], m3 = M[ic + (int64_t)( c + 3 ) * nc];
+                    acc += m0 * xr[c] + m1 * xr[c + 1] + m2 * xr[c + 2] + m3 * xr[c + 3];
+                }
+                for ( ; c < cend; c++ ) acc += M[ic + (int64_t)c * nc] * xr[c];
+                if ( i < nc ) x[fcol + i + ( r0 + r ) * ldx] = acc;
+            }
+        }
+    }
+    else {
+        constexpr int NW = TH / 64;
+        const int32_t ng = ( nc + 15 ) >> 4;
+        const int     col = lane >> 2;
+        for ( int32_t g = wave; g < ng; g += NW ) {
+            const int32_t c0 = 16 * g, ncols = min( 16, nc - c0 );
+            for ( int32_t r = 0; r < nr; r++ ) {
+                const T *xr = xs + r * nc;
+                T        v[16];
+#pragma unroll
+                for ( int cc = 0; cc < 16; cc++ ) v[cc] = T( 0 );
+                for ( int32_t i = c0 + lane; i < nc; i += 64 ) {
+                    const T yi = xr[i];
+#pragma unroll
+                    for ( int cc = 0; cc < 16; cc++ ) {
+                        const T m = M[i + (int64_t)min( c0 + cc, nc - 1 ) * nc];
+                        v[cc] += ( cc < ncols ? m : T( 0 ) ) * yi;
+                    }
+                }
+                const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c0 + c */
+                if ( ( lane & 3 ) == 0 && col < ncols ) x[fcol + c0 + col + ( r0 + r ) * ldx] = sum;
+            }
+        }
     }
 }
 

@@ -281,3 +281,53 @@ def test_lookahead_gives_the_factors_of_the_level_by_level_schedule(ft, mixed, m
         assert _relerr(out[mode][0], out["seq"][0], mask) < tol
         if out["seq"][1] is not None:
             assert _relerr(out[mode][1], out["seq"][1]) < tol
+
+
+@pytest.mark.parametrize("ft,trans", [(pa.factotype.LLT, False), (pa.factotype.LU, False), (pa.factotype.LU, True)],
+                         ids=["llt", "lu", "lu-trans"])
+def test_solve_by_inverted_diagonal_blocks_matches_substitution(ft, trans, monkeypatch):
+    """The diagonal-block solves by products with the inverted blocks that
+    the TRSM launches write (the default) give the solution of the
+    substitution kernels (PASTIX_AMD_DINV=0) within 1e-12, for both sweeps
+    and both factor arrays; also after the factors were saved and loaded
+    into a fresh instance (the load rebuilds the inverses)."""
+    import os
+    import tempfile
+    grid = (21, 19, 17)
+    gen = problems.convection_diffusion_27 if ft == pa.factotype.LU else problems.laplacian
+    A = gen(*grid)
+    rng = np.random.default_rng(8)
+    b = np.asfortranarray(rng.uniform(-1, 1, (A.shape[0], 3)))
+    xs = {}
+    for mode in ("subst", "dinv"):
+        monkeypatch.delenv("PASTIX_AMD_DINV", raising=False)
+        if mode == "subst":
+            monkeypatch.setenv("PASTIX_AMD_DINV", "0")
+        print("case dinv", ft, trans, mode, flush=True)
+        P = Problem(A, grid, ft)
+        try:
+            P.iparm[pa.iparm.transpose_solve] = pa.trans.Trans if trans else pa.trans.NoTrans
+            xs[mode] = pa.task_solve(P.h, P.spm, b.copy(order="F"))
+            if mode == "dinv":
+                path = os.path.join(tempfile.mkdtemp(), "f.bin").encode()
+                from pastix_amd import _lib
+                assert _lib.load().pastix_amd_factors_save(P.h, path) == 0
+                ip, dp = pa.initParam()
+                ip[pa.iparm.factorization] = ft
+                ip[pa.iparm.verbose] = pa.verbose.Not
+                ip[pa.iparm.transpose_solve] = P.iparm[pa.iparm.transpose_solve]
+                h2 = pa.init(ip, dp)
+                mt = pa.mtxtype.General if ft == pa.factotype.LU else pa.mtxtype.Symmetric
+                spm2 = pa.spmatrix(A, mtxtype=mt)
+                pa.task_analyze(h2, spm2, pa.order.grid(*grid))
+                assert _lib.load().pastix_subtask_spm2bcsc(h2, spm2.ref) == 0
+                assert _lib.load().pastix_amd_factors_load(h2, path) == 0
+                xs["loaded"] = pa.task_solve(h2, spm2, b.copy(order="F"))
+                pa.finalize(h2)
+        finally:
+            P.close()
+    Aop = A.T if trans else A
+    for mode in ("dinv", "loaded"):
+        assert np.abs(xs[mode] - xs["subst"]).max() <= 1e-12 * np.abs(xs["subst"]).max()
+        for r in range(3):
+            assert problems.backward_error(Aop, xs[mode][:, r], b[:, r]) <= 1e-14
