@@ -193,6 +193,9 @@ struct pa_device_s {
     double  prof_ms[3];
     int64_t prof_launch[3];
     double  class_flops[3];            /* algorithmic flops of a full factorization */
+    double  real_flops = 0.;           /* flops the kernels perform: the GEMM tiles
+                                          padded to their class, diagonal and TRSM
+                                          as the model */
 };
 
 static inline pa_device_s *DEV( const pastix_data_t *d ) { return (pa_device_s *)d->dev; }
@@ -670,6 +673,7 @@ pa_build_schedule( pastix_data_t *pd )
         }
     };
     dev->class_flops[0] = dev->class_flops[1] = dev->class_flops[2] = 0.;
+    double gemm_real = 0.;
     for ( pastix_int_t k = 0; k < cblknbr; k++ ) {
         double f[3];
         cblk_flops( k, f );
@@ -772,6 +776,7 @@ pa_build_schedule( pastix_data_t *pd )
                             cls = TN == 64 ? 0 : 1;
                         }
                         const int32_t slen = std::min( TN, rem );
+                        gemm_real += 2. * TM * TN * nc * (double)( ( R - g0 + TM - 1 ) / TM );
                         for ( int32_t r = g0; r < R; r += TM ) {
                             pa_gemm_task t;
                             t.k = k; t.F = F; t.s0 = s; t.r0 = r; t.slen = slen;
@@ -809,6 +814,9 @@ pa_build_schedule( pastix_data_t *pd )
     }
     dev->lu   = two;
     dev->ldlt = ldlt;
+    /* the diagonal class holds the narrow cblks' fused kernels at their
+     * model flops; the GEMM tiles count as executed, padding included */
+    dev->real_flops = dev->class_flops[0] + dev->class_flops[1] + gemm_real;
     dev->all_dinv = all_dinv && any_dinv;
     if ( ( rc = pa_upload( &dev->d_lev_cblks, lev_cblks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_trsm_tasks, trsm_tasks ) ) ) return rc;
@@ -907,6 +915,12 @@ pa_dev_get_tune( const pastix_data_t *pd, int32_t *params )
     const pa_device_s *dev = DEV( pd );
     const pa_tune_t   &t   = dev ? dev->tune : pa_tune_default;
     params[0] = t.tm_min; params[1] = t.narrow; params[2] = t.small; params[3] = t.tall64;
+}
+
+extern "C" double
+pa_dev_real_flops( const pastix_data_t *pd )
+{
+    return DEV( pd ) ? DEV( pd )->real_flops : 0.;
 }
 
 extern "C" pastix_int_t
@@ -1528,9 +1542,9 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
     if ( Dinv && lev.dinv ) {
         /* products with the inverted diagonal blocks */
         const size_t  avail  = (size_t)dev->lds_limit - 4096;
-        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)ncmax * sizeof( T ) ) ) );
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( 2 * (size_t)ncmax * sizeof( T ) ) ) );
         dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
-        size_t        lds = (size_t)ncmax * rchunk * sizeof( T );
+        size_t        lds = 2 * (size_t)ncmax * rchunk * sizeof( T );
         if ( ncmax <= 64 )       k_solve_dinv<T, BWD, 64><<<grid, 64, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
         else if ( ncmax <= 128 ) k_solve_dinv<T, BWD, 128><<<grid, 128, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
         else if ( ncmax <= 256 ) k_solve_dinv<T, BWD, 256><<<grid, 256, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );

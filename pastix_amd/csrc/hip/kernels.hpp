@@ -2102,71 +2102,81 @@ k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64
 /* Diagonal-block solves by products with the inverted blocks that the TRSM
  * launches write (M_k = inverse of the lower triangle of the swept factor's
  * diagonal block, column major, c_dtoff offsets, zeros above the diagonal):
- * forward x_k <- M_k x_k, backward x_k <- M_k^T x_k.  Every load is
- * independent of the others (no dependent chain as in substitution).  One
- * workgroup of TH threads per (cblk, chunk of rchunk right-hand sides); the
- * cblk's right-hand sides are staged in LDS and the results go straight to
- * x.  Forward: thread i accumulates row i over the columns up to its wave's
- * last row (coalesced column reads).  Backward: a wave takes 16 columns at a
- * time, its lanes run down the rows from the group's first column and the
- * 16 column sums are reduced by the transposing butterfly. */
+ * forward x_k <- M_k x_k, backward x_k <- M_k^T x_k.  One workgroup of TH
+ * threads per (cblk, chunk of rchunk right-hand sides); the right-hand
+ * sides are staged in LDS (xs) and the products accumulate in LDS (ys).
+ * The work is cut into wave tasks whose loads are all independent, so that
+ * each wave has many loads in flight at once instead of a chain:
+ *   forward:  (64 rows, 32 columns) tiles of the lower triangle; lane = row,
+ *             32 coalesced column loads, partial row sums added to ys;
+ *   backward: (16 columns, 64 rows) tiles from the group's first column
+ *             down; lane = row, 16 coalesced loads, the 16 column sums
+ *             reduced by the transposing butterfly and added to ys. */
 template <typename T, bool BWD, int TH>
 __global__ __launch_bounds__( TH ) void
 k_solve_dinv( pa_dstruct S, const int32_t *cblks, const T *Dinv, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
 {
     extern __shared__ unsigned char pa_smem_raw[];
-    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
     const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
     const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] inputs   */
+    T *ys = xs + (int64_t)rchunk * nc;            /* [nr][nc] products */
     const T      *M = Dinv + S.c_dtoff[k];
     const int     t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int NW = TH / 64;
     for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
         const int32_t i = idx % nc, r = idx / nc;
         xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
+        ys[r * nc + i] = T( 0 );
     }
     __syncthreads();
     if ( !BWD ) {
-        for ( int32_t i0 = 0; i0 < nc; i0 += TH ) {
-            const int32_t i    = i0 + t;
-            const int32_t ic   = min( i, nc - 1 );
-            const int32_t cend = min( i0 + 64 * wave + 64, nc ); /* columns past the wave's last row are zero */
+        const int32_t nrb = ( nc + 63 ) >> 6, ncb = ( nc + 31 ) >> 5;
+        for ( int32_t task = wave; task < nrb * ncb; task += NW ) {
+            const int32_t rb = task / ncb, cb = task % ncb;
+            const int32_t c0 = 32 * cb;
+            if ( c0 > 64 * rb + 63 ) continue; /* above the diagonal: zero */
+            const int32_t i = 64 * rb + lane, ic = min( i, nc - 1 ), cnt = min( 32, nc - c0 );
+            T m[32];
+#pragma unroll
+            for ( int j = 0; j < 32; j++ ) m[j] = M[ic + (int64_t)( c0 + min( j, cnt - 1 ) ) * nc];
             for ( int32_t r = 0; r < nr; r++ ) {
-                const T *xr  = xs + r * nc;
+                const T *xr  = xs + r * nc + c0;
                 T        acc = T( 0 );
-                int32_t  c   = 0;
-                for ( ; c + 4 <= cend; c += 4 ) {
-                    const T m0 = M[ic + (int64_t)c * nc], m1 = M[ic + (int64_t)( c + 1 ) * nc];
-                    const T m2 = M[ic + (int64_t)( c + 2 ) * nc], m3 = M[ic + (int64_t)( c + 3 ) * nc];
-                    acc += m0 * xr[c] + m1 * xr[c + 1] + m2 * xr[c + 2] + m3 * xr[c + 3];
-                }
-                for ( ; c < cend; c++ ) acc += M[ic + (int64_t)c * nc] * xr[c];
-                if ( i < nc ) x[fcol + i + ( r0 + r ) * ldx] = acc;
+#pragma unroll
+                for ( int j = 0; j < 32; j++ ) acc += ( j < cnt ? m[j] : T( 0 ) ) * xr[min( j, cnt - 1 )];
+                if ( i < nc ) atomicAdd( ys + r * nc + i, acc );
             }
         }
     }
     else {
-        constexpr int NW = TH / 64;
         const int32_t ng = ( nc + 15 ) >> 4;
         const int     col = lane >> 2;
-        for ( int32_t g = wave; g < ng; g += NW ) {
-            const int32_t c0 = 16 * g, ncols = min( 16, nc - c0 );
-            for ( int32_t r = 0; r < nr; r++ ) {
-                const T *xr = xs + r * nc;
-                T        v[16];
+        /* tasks: column group g, row chunk q from the group's first column */
+        int32_t task = 0;
+        for ( int32_t g = 0; g < ng; g++ ) {
+            const int32_t c0 = 16 * g, ncols = min( 16, nc - c0 ), nq = ( nc - c0 + 63 ) >> 6;
+            for ( int32_t q = 0; q < nq; q++, task++ ) {
+                if ( task % NW != wave ) continue;
+                const int32_t i = c0 + 64 * q + lane, ic = min( i, nc - 1 );
+                T m[16];
 #pragma unroll
-                for ( int cc = 0; cc < 16; cc++ ) v[cc] = T( 0 );
-                for ( int32_t i = c0 + lane; i < nc; i += 64 ) {
-                    const T yi = xr[i];
+                for ( int cc = 0; cc < 16; cc++ ) m[cc] = M[ic + (int64_t)min( c0 + cc, nc - 1 ) * nc];
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    const T yi = ( i < nc ) ? xs[r * nc + ic] : T( 0 );
+                    T       v[16];
 #pragma unroll
-                    for ( int cc = 0; cc < 16; cc++ ) {
-                        const T m = M[i + (int64_t)min( c0 + cc, nc - 1 ) * nc];
-                        v[cc] += ( cc < ncols ? m : T( 0 ) ) * yi;
-                    }
+                    for ( int cc = 0; cc < 16; cc++ ) v[cc] = ( cc < ncols ? m[cc] : T( 0 ) ) * yi;
+                    const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c0 + c */
+                    if ( ( lane & 3 ) == 0 && col < ncols ) atomicAdd( ys + r * nc + c0 + col, sum );
                 }
-                const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c0 + c */
-                if ( ( lane & 3 ) == 0 && col < ncols ) x[fcol + c0 + col + ( r0 + r ) * ldx] = sum;
             }
         }
+    }
+    __syncthreads();
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        const int32_t i = idx % nc, r = idx / nc;
+        x[fcol + i + ( r0 + r ) * ldx] = ys[r * nc + i];
     }
 }
 
@@ -2225,36 +2235,31 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     const int t = threadIdx.x;
     const int32_t v = tk.y + t;
     const bool    valid = v < R;
-    int4 ri = make_int4( 0, 1, 0, 0 );
+    int4 ri = make_int4( 0, 0, 0, 0 ); /* an empty row reads Coef[0] */
     if ( valid ) ri = S.r_info[S.r_base[k] + v];
     const T *A = Coef + pa_rinfo_off( ri ) + (int64_t)c0 * ri.y;
-    /* right-hand sides RC at a time: each coefficient is read once per chunk */
+    /* the row's PA_SF_COLS coefficients, all loads issued at once from
+     * in-range addresses (columns past the cblk repeat its last one and are
+     * masked below): one memory latency per task instead of a chain */
+    T a[PA_SF_COLS];
+#pragma unroll
+    for ( int w = 0; w < PA_SF_COLS; w++ ) a[w] = A[(int64_t)min( w, ncols - 1 ) * ri.y];
+    /* right-hand sides RC at a time */
     for ( int32_t r0 = 0; r0 < nrhs; r0 += RC ) {
         const int32_t nr = min( RC, nrhs - r0 );
-        for ( int32_t idx = t; idx < ncols * nr; idx += 256 ) {
-            const int32_t c = idx % ncols, u = idx / ncols;
-            xk[u][c] = x[fcol + c0 + c + (int64_t)( r0 + u ) * ldx];
+        for ( int32_t idx = t; idx < PA_SF_COLS * nr; idx += 256 ) {
+            const int32_t c = idx % PA_SF_COLS, u = idx / PA_SF_COLS;
+            xk[u][c] = ( c < ncols ) ? x[fcol + c0 + c + (int64_t)( r0 + u ) * ldx] : T( 0 );
         }
         __syncthreads();
         if ( valid ) {
             T acc[RC];
 #pragma unroll
             for ( int u = 0; u < RC; u++ ) acc[u] = T( 0 );
-            int32_t c = 0;
-            for ( ; c + 16 <= ncols; c += 16 ) {
-                T a[16];
 #pragma unroll
-                for ( int w = 0; w < 16; w++ ) a[w] = A[(int64_t)( c + w ) * ri.y];
+            for ( int w = 0; w < PA_SF_COLS; w++ )
 #pragma unroll
-                for ( int w = 0; w < 16; w++ )
-#pragma unroll
-                    for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xk[u][c + w];
-            }
-            for ( ; c < ncols; c++ ) {
-                const T aw = A[(int64_t)c * ri.y];
-#pragma unroll
-                for ( int u = 0; u < RC; u++ ) acc[u] += aw * xk[u][c];
-            }
+                for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xk[u][w];
 #pragma unroll
             for ( int u = 0; u < RC; u++ )
                 if ( u < nr ) pa_atomic_add( x + ri.x + (int64_t)( r0 + u ) * ldx, -acc[u] );

@@ -59,6 +59,8 @@ int64_t pa_dev_noalloc_max( pastix_data_t *pastix_data );
  * factors' arithmetic; begin allocates the coeftab for the current
  * IPARM_FACTORIZATION without filling it, end builds what the solve derives
  * from the factors */
+/* flops the factorization kernels perform (GEMM tiles with their padding) */
+double pa_dev_real_flops( const pastix_data_t *pastix_data );
 int    pa_dev_factor_info( const pastix_data_t *pastix_data, int *prec, int *two, int *fact, int64_t *coefnbr );
 int    pa_dev_factor_get( const pastix_data_t *pastix_data, int side, int64_t off, int64_t count, void *host );
 int    pa_dev_factor_begin( pastix_data_t *pastix_data, int prec, int lu );

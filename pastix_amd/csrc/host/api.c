@@ -542,7 +542,9 @@ pastix_subtask_sopalin( pastix_data_t *pd )
     pd->iparm[IPARM_STATIC_PIVOTING] = npiv;
     pd->dparm[DPARM_FACT_TIME]  = seconds;
     pd->dparm[DPARM_FACT_FLOPS] = seconds > 0. ? pd->dparm[DPARM_FACT_THFLOPS] / seconds / 1073741824. : 0.;
-    pd->dparm[DPARM_FACT_RLFLOPS] = pd->dparm[DPARM_FACT_THFLOPS];
+    /* performed flops (reference kernels/kernels_trace.c:349 accumulates the
+     * kernels' own counts): the device kernels' work, GEMM padding included */
+    pd->dparm[DPARM_FACT_RLFLOPS] = pa_dev_real_flops( pd );
     if ( pd->iparm[IPARM_VERBOSE] > PastixVerboseNot ) {
         fprintf( stdout, "    Time to factorize                      %e s\n", seconds );
         fprintf( stdout, "    Pivot thresholds: %d static pivot(s)\n", npiv );

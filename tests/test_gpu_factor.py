@@ -331,3 +331,20 @@ def test_solve_by_inverted_diagonal_blocks_matches_substitution(ft, trans, monke
         assert np.abs(xs[mode] - xs["subst"]).max() <= 1e-12 * np.abs(xs["subst"]).max()
         for r in range(3):
             assert problems.backward_error(Aop, xs[mode][:, r], b[:, r]) <= 1e-14
+
+
+@pytest.mark.parametrize("ft", [pa.factotype.LLT, pa.factotype.LU], ids=["llt", "lu"])
+def test_performed_flops_count_the_padded_gemm_tiles(ft):
+    """DPARM_FACT_RLFLOPS counts the flops the kernels perform (the reference
+    accumulates its kernels' own counts, kernels/kernels_trace.c:349): the
+    GEMM tiles padded to their class, so at least the model's THFLOPS and
+    less than twice it on a 3D stencil."""
+    grid = (20, 20, 20)
+    gen = problems.convection_diffusion_27 if ft == pa.factotype.LU else problems.laplacian
+    P = Problem(gen(*grid), grid, ft)
+    try:
+        th = float(P.dparm[pa.dparm.fact_thflops])
+        rl = float(P.dparm[pa.dparm.fact_rlflops])
+        assert th > 0 and th <= rl < 2 * th
+    finally:
+        P.close()
