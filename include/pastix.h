@@ -121,9 +121,9 @@ int pastix_amd_factors_load( pastix_data_t *pastix_data, const char *path );
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Same arguments, same kernels and same results; the permuted right-hand
- * sides (and their FP32 copy for FP32 factors) live in a static 64 MiB
- * device array per GPU, so n * nrhs may be at most 8,388,576 for FP64
- * factors and 5,592,384 for FP32 factors; larger calls return
+ * sides, their FP32 copy for FP32 factors and the second array of the
+ * triangular sweeps live in a static 128 MiB device array per GPU (16 bytes
+ * per entry), so n * nrhs may be at most 8,388,576; larger calls return
  * PASTIX_ERR_BADPARAMETER before any launch.  Launches are eager (no graph)
  * and the call returns after the solve has finished.  The array is shared by
  * every handle on a GPU: calls that overlap on one GPU race.

@@ -44,11 +44,12 @@ double pa_flops_sytrf( double n );
 
 /* Scratch of the allocation-free solve (pa_dev_solve_noalloc): the
  * right-hand sides in the factorization numbering (FP64, n*nrhs, padded to
- * 256 bytes) followed by their FP32 copy when the factors are FP32.  One copy
+ * 256 bytes), then the second array of the sweeps (FP64 factors), or the
+ * FP32 copy (padded to 256 bytes) and the second array in FP32.  One copy
  * per device, held until the process ends and shared by every handle on that
  * device; its address on each device is resolved once (pa_scratch_resolve)
  * and only that address is passed to kernels and copies. */
-#define PA_SCRATCH_BYTES ( (size_t)64 << 20 )
+#define PA_SCRATCH_BYTES ( (size_t)128 << 20 )
 #define PA_SCRATCH_MAXDEV 64
 static __device__ __align__( 256 ) unsigned char scratch[PA_SCRATCH_BYTES];
 static void *pa_scratch_addr[PA_SCRATCH_MAXDEV];
@@ -78,6 +79,9 @@ struct pa_level_t {
     int64_t gc_mid[PA_NGC];       /* [gc_beg, gc_mid): tasks whose target is a
                                      cblk of the next level (look-ahead)   */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
+    int64_t dvt_beg, dvt_end;     /* into d_dinv_tasks: 64-entry blocks of
+                                     the products with the inverted
+                                     diagonal blocks (dinv levels)          */
     int64_t bsolve_beg, bsolve_end; /* into d_bsolve_tasks (backward)       */
     int32_t ncmax;                /* widest cblk of the level               */
     int32_t ncmax_fac;            /* widest cblk of [cblk_beg, fac_end)     */
@@ -88,7 +92,7 @@ struct pa_level_t {
 struct pa_graph_entry {
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec  = nullptr;
-    const void    *key[4] = { nullptr, nullptr, nullptr, nullptr };
+    const void    *key[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
     int64_t        ikey[4] = { 0, 0, 0, 0 };
     uint64_t       stamp = 0;
 };
@@ -136,6 +140,7 @@ struct pa_device_s {
     int2         *d_trsm_tasks;
     pa_gemm_task *d_gemm_tasks;
     int4         *d_solve_tasks, *d_bsolve_tasks;
+    int2         *d_dinv_tasks = nullptr;
     /* TRSM by inverted diagonal tiles: first tile of each cblk (-1: none),
      * the (cblk, tile) list of every level and the inverses (one region per
      * TRSM mode: two for LU) */
@@ -174,6 +179,8 @@ struct pa_device_s {
     double *h_partial;
     float  *d_xs;      /* FP32 solve buffer of the mixed path */
     int64_t xs_len;
+    void   *d_alt = nullptr;   /* second array of the solve sweeps       */
+    size_t  alt_bytes = 0;
     double *d_xtmp;    /* permuted rhs buffer */
     int64_t xtmp_len;
     void   *d_scratch = nullptr;   /* this device's `scratch` array */
@@ -242,8 +249,6 @@ static void
 pa_lds_attr_sd( int bytes )
 {
     if ( !UNIT ) {
-        pa_lds_attr( k_solve_dinv<T, false, 64>, bytes );  pa_lds_attr( k_solve_dinv<T, true, 64>, bytes );
-        pa_lds_attr( k_solve_dinv<T, false, 128>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 128>, bytes );
         pa_lds_attr( k_solve_dinv<T, false, 256>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 256>, bytes );
         pa_lds_attr( k_solve_dinv<T, false, 512>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 512>, bytes );
     }
@@ -414,6 +419,8 @@ pa_free_structure( pa_device_s *dev )
     for ( void *p : ptrs ) hipFree( p );
     hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv ); hipFree( dev->d_dinv );
     dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL; dev->d_dinv = NULL;
+    hipFree( dev->d_dinv_tasks );
+    dev->d_dinv_tasks = NULL;
     dev->tinv_tiles = 0;
     dev->S.c_tinv = NULL;
     dev->d_c_fcol = dev->d_c_ncol = dev->d_c_stride = dev->d_c_fblok = NULL;
@@ -446,6 +453,7 @@ pa_dev_close( pastix_data_t *pd )
     hipFree( dev->d_partial );
     hipHostFree( dev->h_partial );
     hipFree( dev->d_xs );
+    hipFree( dev->d_alt );
     hipFree( dev->d_xtmp );
     hipEventDestroy( dev->ev0 );
     hipEventDestroy( dev->ev1 );
@@ -601,6 +609,8 @@ pa_build_schedule( pastix_data_t *pd )
         dev->d_bsolve_tasks = NULL;
         hipFree( dev->d_c_tinv ); hipFree( dev->d_tinv_tasks ); hipFree( dev->d_tinv ); hipFree( dev->d_dinv );
         dev->d_c_tinv = NULL; dev->d_tinv_tasks = NULL; dev->d_tinv = NULL; dev->d_dinv = NULL;
+        hipFree( dev->d_dinv_tasks );
+        dev->d_dinv_tasks = NULL;
         dev->tinv_tiles = 0;
         dev->levels.clear();
     }
@@ -633,7 +643,7 @@ pa_build_schedule( pastix_data_t *pd )
     const char *di_env  = getenv( "PASTIX_AMD_DINV" );
     const bool  dinv_on = !( di_env && di_env[0] == '0' );
     bool        any_dinv = false, all_dinv = !ldlt && dinv_on;
-    std::vector<int2> ident_tasks;
+    std::vector<int2> ident_tasks, dinv_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
     std::vector<pa_gemm_task> gtasks[PA_NGC];
     std::vector<int32_t>      srccount( cblknbr + 1, 0 ), lastsrc( cblknbr + 1, -1 );
@@ -687,6 +697,7 @@ pa_build_schedule( pastix_data_t *pd )
         for ( int c = 0; c < PA_NGC; c++ ) L.gc_beg[c] = gtasks[c].size();
         L.solve_beg = solve_tasks.size();
         L.bsolve_beg = bsolve_tasks.size();
+        L.dvt_beg = dinv_tasks.size();
         L.ncmax     = 0;
         L.ncmax_fac = 0;
         L.flops[0] = L.flops[1] = L.flops[2] = 0.;
@@ -737,8 +748,10 @@ pa_build_schedule( pastix_data_t *pd )
             for ( int32_t r = 0; r < R; r += PA_TRSM_TR ) trsm_tasks.push_back( make_int2( k, r ) );
             /* identity rows r0.. of the diagonal block, as (k, -1 - r0),
              * after the level's panel slabs */
-            if ( lev_dinv )
+            if ( lev_dinv ) {
                 for ( int32_t r = 0; r < nc; r += PA_TRSM_TR ) ident_tasks.push_back( make_int2( k, -1 - r ) );
+                for ( int32_t b = 0; 64 * b < nc; b++ ) dinv_tasks.push_back( make_int2( k, b ) );
+            }
             if ( R > 0 || lev_dinv ) {
                 c_tinv[k] = tinv_tiles;
                 for ( int32_t tl = 0; tl < ( nc + 15 ) / 16; tl++ ) tinv_tasks.push_back( make_int2( k, tl ) );
@@ -811,6 +824,7 @@ pa_build_schedule( pastix_data_t *pd )
         }
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
+        L.dvt_end = dinv_tasks.size();
     }
     dev->lu   = two;
     dev->ldlt = ldlt;
@@ -843,7 +857,16 @@ pa_build_schedule( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_gemm_tasks, gemm_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_solve_tasks, solve_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
+    if ( ( rc = pa_upload( &dev->d_dinv_tasks, dinv_tasks ) ) ) return rc;
 
+    /* the scatter-GEMM addresses its targets by 32-bit offsets inside the
+     * target cblk */
+    for ( pastix_int_t k = 0; k < cblknbr; k++ )
+        if ( (int64_t)c_stride[k] * c_ncol[k] >= ( (int64_t)1 << 32 ) ) {
+            fprintf( stderr, "pastix_amd: cblk %lld holds 2^32 or more coefficients; lower IPARM_MAX_BLOCKSIZE\n",
+                     (long long)k );
+            return PASTIX_ERR_BADPARAMETER;
+        }
     /* LDS needs of the diagonal, TRSM and solve kernels */
     size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 * 2 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {
@@ -1164,7 +1187,8 @@ pa_small_llt_launch( pa_device_s *dev, const pa_level_t &lev, T *L, const double
 
 template <typename T, bool BWD>
 static void
-pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x, int32_t nrhs, int64_t ld )
+pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x, int64_t ld, T *xo, int64_t ldo,
+                       int32_t nrhs )
 {
     int64_t beg = lev.fac_end;
     for ( int ns = 1; ns <= 3; ns++ ) {
@@ -1172,9 +1196,9 @@ pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x
         const int32_t *cb = dev->d_lev_cblks + beg;
         const unsigned g  = (unsigned)( ( n + 3 ) / 4 );
         if ( n > 0 ) {
-            if ( ns == 1 )      k_small_solve<T, BWD, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
-            else if ( ns == 2 ) k_small_solve<T, BWD, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
-            else                k_small_solve<T, BWD, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld );
+            if ( ns == 1 )      k_small_solve<T, BWD, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else if ( ns == 2 ) k_small_solve<T, BWD, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else                k_small_solve<T, BWD, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
         }
         beg = lev.sm_end[ns - 1];
     }
@@ -1520,19 +1544,22 @@ pa_dev_last_fact_event_seconds( const pastix_data_t *pd )
 template <typename T, bool BWD, bool UNIT, int TH>
 static void
 pa_solve_diag_launch( pa_device_s *dev, dim3 grid, size_t lds, const int32_t *cb, const T *C, const T *CT,
-                      const int64_t *ct_off, T *x, int32_t nrhs, int32_t rchunk, int64_t ld )
+                      const int64_t *ct_off, const T *x, int64_t ld, T *xo, int64_t ldo, int32_t nrhs, int32_t rchunk )
 {
-    if ( BWD ) k_solve_diag_bwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, CT, ct_off, x, nrhs, rchunk, ld );
-    else       k_solve_diag_fwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, rchunk, ld );
+    if ( BWD ) k_solve_diag_bwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, CT, ct_off, x, nrhs, rchunk, ld, xo, ldo );
+    else       k_solve_diag_fwd<T, UNIT, TH><<<grid, TH, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, rchunk, ld, xo, ldo );
 }
 
-/* diagonal solves of one level: the register-prefetching kernels (workgroup
- * of the level's widest cblk rounded up to 64/128/256/512 threads) for cblks
- * of up to PA_SD_MAXCOL columns, the LDS-blocked kernel above that */
+/* diagonal solves of one level, from x (the right-hand sides with every
+ * update of the other cblks applied) to xo: the products with the inverted
+ * diagonal blocks on dinv levels (64-entry blocks, 256 or 512 threads), else
+ * the register-prefetching kernels (workgroup of the level's widest cblk
+ * rounded up to 64/128/256/512 threads) for cblks of up to PA_SD_MAXCOL
+ * columns, the LDS-blocked kernel above that */
 template <typename T, bool BWD, bool UNIT>
 static int
-pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off, T *x,
-                     int32_t nrhs, int64_t ld, const T *Dinv )
+pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off,
+                     const T *x, int64_t ld, T *xo, int64_t ldo, int32_t nrhs, const T *Dinv )
 {
     /* the general cblks of the level; the narrow ones go to k_small_solve */
     const int32_t ncb = (int32_t)( lev.fac_end - lev.cblk_beg );
@@ -1541,29 +1568,32 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
     if ( ncb == 0 ) return PASTIX_SUCCESS;
     if ( Dinv && lev.dinv ) {
         /* products with the inverted diagonal blocks */
+        const int     th     = ncmax <= 128 ? 256 : 512;
+        const size_t  per    = ( (size_t)ncmax + ( th / 64 ) * 64 ) * sizeof( T );
         const size_t  avail  = (size_t)dev->lds_limit - 4096;
-        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( 2 * (size_t)ncmax * sizeof( T ) ) ) );
-        dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
-        size_t        lds = 2 * (size_t)ncmax * rchunk * sizeof( T );
-        if ( ncmax <= 64 )       k_solve_dinv<T, BWD, 64><<<grid, 64, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
-        else if ( ncmax <= 128 ) k_solve_dinv<T, BWD, 128><<<grid, 128, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
-        else if ( ncmax <= 256 ) k_solve_dinv<T, BWD, 256><<<grid, 256, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
-        else                     k_solve_dinv<T, BWD, 512><<<grid, 512, lds, dev->stream>>>( dev->S, cb, Dinv, x, nrhs, rchunk, ld );
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / per ) );
+        const int64_t nt     = lev.dvt_end - lev.dvt_beg;
+        dim3          grid( (unsigned)nt, ( nrhs + rchunk - 1 ) / rchunk );
+        const size_t  lds = per * rchunk;
+        const int2   *tk  = dev->d_dinv_tasks + lev.dvt_beg;
+        if ( nt <= 0 ) return PASTIX_SUCCESS;
+        if ( th == 256 ) k_solve_dinv<T, BWD, 256><<<grid, 256, lds, dev->stream>>>( dev->S, tk, Dinv, x, ld, xo, ldo, nrhs, rchunk );
+        else             k_solve_dinv<T, BWD, 512><<<grid, 512, lds, dev->stream>>>( dev->S, tk, Dinv, x, ld, xo, ldo, nrhs, rchunk );
     }
     else if ( ncmax <= PA_SD_MAXCOL ) {
         const size_t  avail  = (size_t)dev->lds_limit - 4096;
         const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( (size_t)ncmax * sizeof( T ) ) ) );
         dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
         size_t        lds = (size_t)ncmax * rchunk * sizeof( T );
-        if ( ncmax <= 64 )       pa_solve_diag_launch<T, BWD, UNIT, 64>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else if ( ncmax <= 128 ) pa_solve_diag_launch<T, BWD, UNIT, 128>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else if ( ncmax <= 256 ) pa_solve_diag_launch<T, BWD, UNIT, 256>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
-        else                     pa_solve_diag_launch<T, BWD, UNIT, 512>( dev, grid, lds, cb, C, CT, ct_off, x, nrhs, rchunk, ld );
+        if ( ncmax <= 64 )       pa_solve_diag_launch<T, BWD, UNIT, 64>( dev, grid, lds, cb, C, CT, ct_off, x, ld, xo, ldo, nrhs, rchunk );
+        else if ( ncmax <= 128 ) pa_solve_diag_launch<T, BWD, UNIT, 128>( dev, grid, lds, cb, C, CT, ct_off, x, ld, xo, ldo, nrhs, rchunk );
+        else if ( ncmax <= 256 ) pa_solve_diag_launch<T, BWD, UNIT, 256>( dev, grid, lds, cb, C, CT, ct_off, x, ld, xo, ldo, nrhs, rchunk );
+        else                     pa_solve_diag_launch<T, BWD, UNIT, 512>( dev, grid, lds, cb, C, CT, ct_off, x, ld, xo, ldo, nrhs, rchunk );
     }
     else {
         size_t lds = (size_t)ncmax * nrhs * sizeof( T );
         if ( lds > (size_t)dev->lds_limit - 4096 ) return PASTIX_ERR_BADPARAMETER;
-        k_solve_diag<T, BWD, UNIT><<<ncb, 256, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, ld );
+        k_solve_diag<T, BWD, UNIT><<<ncb, 256, lds, dev->stream>>>( dev->S, cb, C, x, nrhs, ld, xo, ldo );
     }
     PA_HIP( hipGetLastError() );
     return PASTIX_SUCCESS;
@@ -1571,28 +1601,35 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
 
 /* One forward sweep (op(C) y = b, C in lower 2D-tile storage, C's own
  * diagonal blocks) or backward sweep (op(C)^T x = y; CT/ct_off locate the
- * transposes of C's diagonal blocks) over every level. */
+ * transposes of C's diagonal blocks) over every level, from the right-hand
+ * sides in x to the solution in xo (another array of the same shape).  x is
+ * the working array: the updates are added to it and the diagonal steps read
+ * it; the diagonal steps write their results to xo, which the updates of
+ * the same sweep then read (forward: the solved x_k of the level's own
+ * cblks; backward: the solved rows of the cblks above).  Every level's
+ * diagonal step can thus split a cblk over several workgroups. */
 template <typename T>
 static int
-pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, int64_t ld, const T *Dinv )
+pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int64_t ld, T *xo, int64_t ldo, int32_t nrhs,
+                const T *Dinv )
 {
     const size_t nl = dev->levels.size();
     int rc;
     for ( size_t li = 0; li < nl; li++ ) {
         const pa_level_t &lev = dev->levels[li];
         if ( lev.cblk_end == lev.cblk_beg ) continue;
-        rc = unit_f ? pa_solve_diag_level<T, false, true>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld, Dinv )
-                    : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, nrhs, ld, Dinv );
+        rc = unit_f ? pa_solve_diag_level<T, false, true>( dev, lev, Cf, nullptr, nullptr, x, ld, xo, ldo, nrhs, Dinv )
+                    : pa_solve_diag_level<T, false, false>( dev, lev, Cf, nullptr, nullptr, x, ld, xo, ldo, nrhs, Dinv );
         if ( rc ) return rc;
         int64_t ns = lev.solve_end - lev.solve_beg;
         if ( ns > 0 && nrhs == 1 )
             k_solve_fwd_update<T, 1><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x,
-                                                                            nrhs, ld );
+                                                                            nrhs, ld, xo, ldo );
         else if ( ns > 0 )
             k_solve_fwd_update<T, 4><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_solve_tasks + lev.solve_beg, Cf, x,
-                                                                            nrhs, ld );
+                                                                            nrhs, ld, xo, ldo );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
-        if ( nsm > 0 ) pa_small_solve_launch<T, false>( dev, lev, Cf, x, nrhs, ld );
+        if ( nsm > 0 ) pa_small_solve_launch<T, false>( dev, lev, Cf, x, ld, xo, ldo, nrhs );
         PA_HIP( hipGetLastError() );
     }
     return PASTIX_SUCCESS;
@@ -1600,8 +1637,8 @@ pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int32_t nrhs, 
 
 template <typename T>
 static int
-pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off, T *x, int32_t nrhs,
-                int64_t ld, const T *Dinv )
+pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const int64_t *cbt_off, T *x, int64_t ld,
+                T *xo, int64_t ldo, int32_t nrhs, const T *Dinv )
 {
     const size_t nl = dev->levels.size();
     int rc;
@@ -1611,15 +1648,15 @@ pa_sweep_bwd_t( pa_device_s *dev, const T *Cb, bool unit_b, const T *CbT, const 
         int64_t ns = lev.bsolve_end - lev.bsolve_beg;
         if ( ns > 0 && nrhs == 1 )
             k_solve_bwd_update<T, false><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb,
-                                                                                x, nrhs, ld );
+                                                                                x, nrhs, ld, xo, ldo );
         else if ( ns > 0 )
             k_solve_bwd_update<T, true><<<(unsigned)ns, 256, 0, dev->stream>>>( dev->S, dev->d_bsolve_tasks + lev.bsolve_beg, Cb,
-                                                                               x, nrhs, ld );
+                                                                               x, nrhs, ld, xo, ldo );
         const int32_t nsm = (int32_t)( lev.cblk_end - lev.fac_end );
-        if ( nsm > 0 ) pa_small_solve_launch<T, true>( dev, lev, Cb, x, nrhs, ld );
+        if ( nsm > 0 ) pa_small_solve_launch<T, true>( dev, lev, Cb, x, ld, xo, ldo, nrhs );
         PA_HIP( hipGetLastError() );
-        rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld, Dinv )
-                    : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, nrhs, ld, Dinv );
+        rc = unit_b ? pa_solve_diag_level<T, true, true>( dev, lev, Cb, CbT, cbt_off, x, ld, xo, ldo, nrhs, Dinv )
+                    : pa_solve_diag_level<T, true, false>( dev, lev, Cb, CbT, cbt_off, x, ld, xo, ldo, nrhs, Dinv );
         if ( rc ) return rc;
     }
     return PASTIX_SUCCESS;
@@ -1642,11 +1679,18 @@ struct pa_sweep_op {
     int kind, side, unit;
 };
 
+/* Run the steps on x (n x nrhs, leading dimension ld) with alt (n x nrhs,
+ * leading dimension n) as the second array of the sweeps: each sweep reads
+ * one of the two and leaves its result in the other.  On return *res / *ldres
+ * give the array that holds the result (x after an even number of sweeps). */
 template <typename T>
 static int
-pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t nrhs, int64_t ld )
+pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t nrhs, int64_t ld, T *alt, T **res,
+              int64_t *ldres )
 {
     int rc;
+    T      *cur = x, *oth = alt;
+    int64_t ldc = ld, ldt = dev->n;
     for ( int o = 0; o < nops; o++ ) {
         const pa_sweep_op &op = ops[o];
         const T *C = (const T *)( op.side ? dev->d_U : dev->d_L );
@@ -1657,7 +1701,9 @@ pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t 
         const T  *Dinv = ( dev->d_dinv && !dev->ldlt && op.unit == natural_unit )
                              ? (const T *)dev->d_dinv + ( op.side ? dev->dtnbr : 0 ) : nullptr;
         if ( op.kind == PA_OP_FWD ) {
-            if ( ( rc = pa_sweep_fwd_t<T>( dev, C, op.unit != 0, x, nrhs, ld, Dinv ) ) ) return rc;
+            if ( ( rc = pa_sweep_fwd_t<T>( dev, C, op.unit != 0, cur, ldc, oth, ldt, nrhs, Dinv ) ) ) return rc;
+            std::swap( cur, oth );
+            std::swap( ldc, ldt );
         }
         else if ( op.kind == PA_OP_BWD ) {
             /* transposed diagonal blocks of the swept factor: for two-array
@@ -1673,34 +1719,53 @@ pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t 
                                                                              (T *)dev->d_DT );
                 PA_HIP( hipGetLastError() );
             }
-            if ( ( rc = pa_sweep_bwd_t<T>( dev, C, op.unit != 0, ct, ct_off, x, nrhs, ld, Dinv ) ) ) return rc;
+            if ( ( rc = pa_sweep_bwd_t<T>( dev, C, op.unit != 0, ct, ct_off, cur, ldc, oth, ldt, nrhs, Dinv ) ) ) return rc;
+            std::swap( cur, oth );
+            std::swap( ldc, ldt );
         }
         else {
             if ( dev->n > 0 ) {
                 dim3 grid( pa_grid( dev->n ), (unsigned)std::min<int32_t>( nrhs, 64 ) );
                 k_solve_dscale<T><<<grid, 256, 0, dev->stream>>>( dev->S, (int32_t)dev->n, dev->d_col2cblk,
-                                                                 (const T *)dev->d_L, x, nrhs, ld );
+                                                                 (const T *)dev->d_L, cur, nrhs, ldc );
                 PA_HIP( hipGetLastError() );
             }
         }
     }
+    *res   = cur;
+    *ldres = ldc;
     return PASTIX_SUCCESS;
 }
 
-/* Launch a sequence of solve steps in place on a device FP64 buffer in the
- * factorization numbering; FP32 factors work on the FP32 copy xs (n*nrhs,
- * leading dimension n; down- and up-conversion, reference
- * sopalin/pastix_task_solve.c:583-592, sopalin/pastix_rhs.c:162-242). */
+/* Launch a sequence of solve steps on a device FP64 buffer in the
+ * factorization numbering, the result back in d_x.  FP64 factors use alt
+ * (n*nrhs doubles) as the second array of the sweeps; FP32 factors work on
+ * the FP32 copy xs with alt (n*nrhs floats) as the second array (down- and
+ * up-conversion, reference sopalin/pastix_task_solve.c:583-592,
+ * sopalin/pastix_rhs.c:162-242). */
 static int
-pa_launch_ops( pa_device_s *dev, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld, float *xs )
+pa_launch_ops( pa_device_s *dev, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld, float *xs,
+               void *alt )
 {
-    if ( dev->prec == PA_PREC_DOUBLE ) return pa_run_ops_t<double>( dev, ops, nops, d_x, nrhs, ld );
+    const int64_t n = dev->n;
+    int           rc;
+    if ( dev->prec == PA_PREC_DOUBLE ) {
+        double *res;
+        int64_t ldr;
+        if ( ( rc = pa_run_ops_t<double>( dev, ops, nops, d_x, nrhs, ld, (double *)alt, &res, &ldr ) ) ) return rc;
+        if ( res != d_x )
+            for ( int32_t r = 0; r < nrhs; r++ )
+                k_convert<double, double><<<pa_grid( n ), 256, 0, dev->stream>>>( n, res + r * ldr, d_x + r * ld );
+        PA_HIP( hipGetLastError() );
+        return PASTIX_SUCCESS;
+    }
     for ( int32_t r = 0; r < nrhs; r++ )
-        k_convert<float, double><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, d_x + r * ld, xs + r * dev->n );
-    int rc = pa_run_ops_t<float>( dev, ops, nops, xs, nrhs, dev->n );
-    if ( rc ) return rc;
+        k_convert<float, double><<<pa_grid( n ), 256, 0, dev->stream>>>( n, d_x + r * ld, xs + r * n );
+    float  *res;
+    int64_t ldr;
+    if ( ( rc = pa_run_ops_t<float>( dev, ops, nops, xs, nrhs, n, (float *)alt, &res, &ldr ) ) ) return rc;
     for ( int32_t r = 0; r < nrhs; r++ )
-        k_convert<double, float><<<pa_grid( dev->n ), 256, 0, dev->stream>>>( dev->n, xs + r * dev->n, d_x + r * ld );
+        k_convert<double, float><<<pa_grid( n ), 256, 0, dev->stream>>>( n, res + r * ldr, d_x + r * ld );
     PA_HIP( hipGetLastError() );
     return PASTIX_SUCCESS;
 }
@@ -1729,14 +1794,22 @@ pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrh
         PA_HIP( hipMalloc( (void **)&dev->d_xs, std::max<int64_t>( len, 1 ) * sizeof( float ) ) );
         dev->xs_len = len;
     }
+    const size_t alt = (size_t)std::max<int64_t>( len, 1 ) * ( dev->prec == PA_PREC_DOUBLE ? sizeof( double ) : sizeof( float ) );
+    if ( dev->alt_bytes < alt ) {
+        hipFree( dev->d_alt );
+        dev->d_alt = nullptr;
+        PA_HIP( hipMalloc( &dev->d_alt, alt ) );
+        dev->alt_bytes = alt;
+    }
     int64_t code = 0;
     for ( int o = 0; o < nops; o++ ) code = code * 16 + ops[o].kind * 4 + ops[o].side * 2 + ops[o].unit;
-    auto launch = [&]() -> int { return pa_launch_ops( dev, ops, nops, nrhs, d_x, ld, dev->d_xs ); };
+    auto launch = [&]() -> int { return pa_launch_ops( dev, ops, nops, nrhs, d_x, ld, dev->d_xs, dev->d_alt ); };
     if ( !dev->use_graphs ) return launch();
     /* cached graph per (buffer, nrhs, ld, steps) among the most recent ones */
     pa_graph_entry *slot = nullptr, *oldest = &dev->solve_graphs[0];
     for ( auto &g : dev->solve_graphs ) {
         if ( g.exec && g.key[0] == d_x && g.key[1] == dev->d_xs && g.key[2] == dev->d_L && g.key[3] == dev->d_U &&
+             g.key[4] == dev->d_alt &&
              g.ikey[0] == nrhs && g.ikey[1] == ld && g.ikey[2] == code && g.ikey[3] == dev->prec ) {
             slot = &g;
             break;
@@ -1748,6 +1821,7 @@ pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrh
     int rc = pa_graph_run( dev, *slot, hit, launch );
     if ( rc ) return rc;
     slot->key[0] = d_x; slot->key[1] = dev->d_xs; slot->key[2] = dev->d_L; slot->key[3] = dev->d_U;
+    slot->key[4] = dev->d_alt;
     slot->ikey[0] = nrhs; slot->ikey[1] = ld; slot->ikey[2] = code; slot->ikey[3] = dev->prec;
     slot->stamp = ++dev->graph_clock;
     return PASTIX_SUCCESS;
@@ -1789,21 +1863,29 @@ pa_dev_solve_perm( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, double
     return pa_dev_run_ops( pd, ops, nops, nrhs, d_x, ld );
 }
 
-/* Largest n * nrhs the allocation-free solve takes for FP32 factors (FP64
- * and FP32 copies in the scratch) and for FP64 factors */
+/* Scratch bytes the allocation-free solve needs for n * nrhs entries:
+ * FP64 copy and second array (FP64 factors), or FP64 copy, FP32 copy and
+ * FP32 second array (FP32 factors); 16 bytes per entry either way */
+static size_t
+pa_noalloc_bytes( const pa_device_s *dev, int64_t len, size_t *xb, size_t *sb )
+{
+    *xb = ( (size_t)len * sizeof( double ) + 255 ) & ~(size_t)255;
+    *sb = ( dev->prec != PA_PREC_DOUBLE ) ? ( ( (size_t)len * sizeof( float ) + 255 ) & ~(size_t)255 ) : 0;
+    return *xb + *sb + (size_t)len * ( dev->prec != PA_PREC_DOUBLE ? sizeof( float ) : sizeof( double ) );
+}
+
+/* Largest n * nrhs the allocation-free solve takes */
 extern "C" int64_t
 pa_dev_noalloc_max( pastix_data_t *pd )
 {
-    const pa_device_s *dev = DEV( pd );
-    const size_t       per = sizeof( double ) + ( dev->prec != PA_PREC_DOUBLE ? sizeof( float ) : 0 );
-    return (int64_t)( ( PA_SCRATCH_BYTES - 256 ) / per );
+    return (int64_t)( ( PA_SCRATCH_BYTES - 512 ) / 16 );
 }
 
 /* The full solve of pa_dev_solve_perm plus the permutations P b and P^T x,
  * on d_b in the user numbering, with no device memory allocated: the
- * permuted right-hand sides and their FP32 copy live in this device's
- * `scratch` array (each fully written before it is read: nothing needs
- * clearing).  Eager launches on the handle's stream, then a stream
+ * permuted right-hand sides, their FP32 copy and the second array of the
+ * sweeps live in this device's `scratch` array (every entry is written
+ * before it is read: nothing needs clearing).  Eager launches on the handle's stream, then a stream
  * synchronization.  Calls that overlap on one device share the array and
  * race. */
 extern "C" int
@@ -1814,9 +1896,9 @@ pa_dev_solve_noalloc( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, dou
     pa_sweep_op  ops[3];
     const int    nops = pa_solve_ops( dev->fact_kind, trans, ops );
     if ( pa_check_ops( dev, ops, nops ) ) return PASTIX_ERR_BADPARAMETER;
-    const int64_t n  = dev->n;
-    const size_t  xb = ( ( (size_t)n * nrhs * sizeof( double ) ) + 255 ) & ~(size_t)255;
-    const size_t  need = xb + ( dev->prec != PA_PREC_DOUBLE ? (size_t)n * nrhs * sizeof( float ) : 0 );
+    const int64_t n = dev->n;
+    size_t        xb, sb;
+    const size_t  need = pa_noalloc_bytes( dev, n * (int64_t)nrhs, &xb, &sb );
     if ( nrhs < 1 || need > PA_SCRATCH_BYTES || !dev->d_scratch ) {
         fprintf( stderr,
                  "pastix_amd_solve_dev_noalloc: n * nrhs = %lld needs %zu bytes of scratch, the limit is %zu\n",
@@ -1824,10 +1906,11 @@ pa_dev_solve_noalloc( pastix_data_t *pd, pastix_trans_t trans, int32_t nrhs, dou
         return PASTIX_ERR_BADPARAMETER;
     }
     if ( n == 0 ) return PASTIX_SUCCESS;
-    double *xp = (double *)dev->d_scratch;
-    float  *xs = (float *)( (char *)dev->d_scratch + xb );
-    int     rc = pa_dev_apply_perm( pd, 1, nrhs, d_b, ldb, xp, n );
-    if ( !rc ) rc = pa_launch_ops( dev, ops, nops, nrhs, xp, n, xs );
+    double *xp  = (double *)dev->d_scratch;
+    float  *xs  = sb ? (float *)( (char *)dev->d_scratch + xb ) : nullptr;
+    void   *alt = (char *)dev->d_scratch + xb + sb;
+    int     rc  = pa_dev_apply_perm( pd, 1, nrhs, d_b, ldb, xp, n );
+    if ( !rc ) rc = pa_launch_ops( dev, ops, nops, nrhs, xp, n, xs, alt );
     if ( !rc ) rc = pa_dev_apply_perm( pd, 0, nrhs, xp, n, d_b, ldb );
     if ( rc ) return rc;
     PA_HIP( hipStreamSynchronize( dev->stream ) );

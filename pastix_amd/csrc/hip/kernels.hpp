@@ -1498,13 +1498,13 @@ template <typename T> struct pa_mfma;
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, bool OFF32>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value] )
+              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value], int64_t cbase0 )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int NBM = TM / 32;          /* MFMA blocks per wave, rho          */
@@ -1598,6 +1598,61 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
         __syncthreads();
     }
 
+    if ( OFF32 ) {
+        /* Every target of the tile lies in one cblk, at an offset below 2^32
+         * from its first coefficient (C + cbase0): the wave's NBM * NBN * 4
+         * target offsets are formed first, then all the target reads are
+         * issued together and the writes follow, so the read-modify-write
+         * costs one memory latency instead of one per output.  The outputs
+         * of a tile are distinct entries, and a target that other tiles of
+         * the same launch update is only ever updated with atomics. */
+        T *Cb = C + cbase0;
+        uint32_t off[NBN][NBM][4];
+        bool     ok[NBN][NBM][4];
+#pragma unroll
+        for ( int mj = 0; mj < NBM; mj++ ) {
+            const int      rl   = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
+            const bool     aok  = a_ld[rl] != 0;
+            const uint32_t rb   = (uint32_t)( c_off[rl] - cbase0 );
+            const uint32_t cld  = (uint32_t)c_ld[rl];
+            const int32_t  ablk = a_blk[rl];
+#pragma unroll
+            for ( int mi = 0; mi < NBN; mi++ )
+#pragma unroll
+                for ( int r = 0; r < 4; r++ ) {
+                    const int sl = wn * ( TN / 2 ) + mi * 16 + pa_mfma<T>::row( lane, r );
+                    ok[mi][mj][r]  = aok && b_ld[sl] != 0 && ablk >= b_blk[sl] + shift;
+                    off[mi][mj][r] = ok[mi][mj][r] ? rb + (uint32_t)c_col[sl] * cld : 0u;
+                }
+        }
+        if ( atomic ) {
+#pragma unroll
+            for ( int mi = 0; mi < NBN; mi++ )
+#pragma unroll
+                for ( int mj = 0; mj < NBM; mj++ )
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ )
+                        if ( ok[mi][mj][r] ) pa_atomic_add( Cb + off[mi][mj][r], -acc[mi][mj][r] );
+        }
+        else {
+            /* a masked-off output reads the cblk's first coefficient (in range) */
+            T old[NBN][NBM][4];
+#pragma unroll
+            for ( int mi = 0; mi < NBN; mi++ )
+#pragma unroll
+                for ( int mj = 0; mj < NBM; mj++ )
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ ) old[mi][mj][r] = Cb[off[mi][mj][r]];
+#pragma unroll
+            for ( int mi = 0; mi < NBN; mi++ )
+#pragma unroll
+                for ( int mj = 0; mj < NBM; mj++ )
+#pragma unroll
+                    for ( int r = 0; r < 4; r++ )
+                        if ( ok[mi][mj][r] ) Cb[off[mi][mj][r]] = old[mi][mj][r] - acc[mi][mj][r];
+        }
+        return;
+    }
 #pragma unroll
     for ( int mj = 0; mj < NBM; mj++ ) {
         int rl = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
@@ -1714,8 +1769,8 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *ne
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T, TM, TN>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
-                           uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
+    pa_gemm_tile<T, TM, TN, true>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+                                 uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs, coefF );
 }
 
 /*
@@ -1762,7 +1817,8 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
         b_blk[s] = 0;
     }
     __syncthreads();
-    pa_gemm_tile<T, PA_GT, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
+    pa_gemm_tile<T, PA_GT, PA_GT, false>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false,
+                                        As, Bs, 0 );
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1771,13 +1827,15 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
 /* ------------------------------------------------------------------------- */
 
 /* x_k <- D_kk^{-1} x_k (lower, forward) or D_kk^{-T} x_k (backward), one
- * workgroup per cblk of the level, x_k staged in LDS.  Blocked by 16
+ * workgroup per cblk of the level, x_k read from x, staged in LDS and
+ * written to xo.  Blocked by 16
  * columns: the 16 x 16 triangle is loaded into registers (one row, or one
  * column for the transposed sweep, per lane) and solved by wave 0 with
  * shuffles; the rest of the block column is applied by all threads. */
 template <typename T, bool BACKWARD, bool UNIT>
 __global__ __launch_bounds__( 256 ) void
-k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
+k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, const T *x, int32_t nrhs, int64_t ldx, T *xo,
+              int64_t ldo )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nrhs][nc] */
@@ -1867,7 +1925,7 @@ k_solve_diag( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t n
     }
     for ( int32_t idx = t; idx < nc * nrhs; idx += 256 ) {
         int32_t i = idx % nc, r = idx / nc;
-        x[fcol + i + r * ldx] = xs[r * nc + i];
+        xo[fcol + i + r * ldo] = xs[r * nc + i];
     }
 }
 
@@ -1907,7 +1965,7 @@ pa_wave_sum16( T v[16], int lane )
 
 /*
  * Diagonal solves for cblks of at most TH columns, one workgroup per (cblk,
- * RHS chunk), x_k staged in LDS.  Thread t owns row t (forward) or column t
+ * RHS chunk), x_k read from x, staged in LDS and written to xo.  Thread t owns row t (forward) or column t
  * (backward) of the diagonal block and keeps the 16 entries of a 16-column
  * block in registers.  A ring of PA_SD_RING blocks is kept in flight, so the
  * loads of block b + PA_SD_RING - 1 overlap the work on blocks b..: the
@@ -1952,7 +2010,8 @@ pa_sd_load_cols( T (&v)[16], const T *DT, int32_t nc, int t, int32_t jb )
 
 template <typename T, bool UNIT, int TH>
 __global__ __launch_bounds__( TH ) void
-k_solve_diag_fwd( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
+k_solve_diag_fwd( pa_dstruct S, const int32_t *cblks, const T *Coef, const T *x, int32_t nrhs, int32_t rchunk,
+                  int64_t ldx, T *xo, int64_t ldo )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
@@ -2016,14 +2075,14 @@ k_solve_diag_fwd( pa_dstruct S, const int32_t *cblks, const T *Coef, T *x, int32
     __syncthreads();
     for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
         int32_t i = idx % nc, r = idx / nc;
-        x[fcol + i + ( r0 + r ) * ldx] = xs[r * nc + i];
+        xo[fcol + i + ( r0 + r ) * ldo] = xs[r * nc + i];
     }
 }
 
 template <typename T, bool UNIT, int TH>
 __global__ __launch_bounds__( TH ) void
-k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64_t *dt_off, T *x, int32_t nrhs,
-                  int32_t rchunk, int64_t ldx )
+k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64_t *dt_off, const T *x, int32_t nrhs,
+                  int32_t rchunk, int64_t ldx, T *xo, int64_t ldo )
 {
     extern __shared__ unsigned char pa_smem_raw[];
     T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] */
@@ -2095,88 +2154,110 @@ k_solve_diag_bwd( pa_dstruct S, const int32_t *cblks, const T *DTab, const int64
     __syncthreads();
     for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
         int32_t i = idx % nc, r = idx / nc;
-        x[fcol + i + ( r0 + r ) * ldx] = xs[r * nc + i];
+        xo[fcol + i + ( r0 + r ) * ldo] = xs[r * nc + i];
     }
 }
 
 /* Diagonal-block solves by products with the inverted blocks that the TRSM
  * launches write (M_k = inverse of the lower triangle of the swept factor's
  * diagonal block, column major, c_dtoff offsets, zeros above the diagonal):
- * forward x_k <- M_k x_k, backward x_k <- M_k^T x_k.  One workgroup of TH
- * threads per (cblk, chunk of rchunk right-hand sides); the right-hand
- * sides are staged in LDS (xs) and the products accumulate in LDS (ys).
- * The work is cut into wave tasks whose loads are all independent, so that
- * each wave has many loads in flight at once instead of a chain:
- *   forward:  (64 rows, 32 columns) tiles of the lower triangle; lane = row,
- *             32 coalesced column loads, partial row sums added to ys;
- *   backward: (16 columns, 64 rows) tiles from the group's first column
- *             down; lane = row, 16 coalesced loads, the 16 column sums
- *             reduced by the transposing butterfly and added to ys. */
+ * forward x_k <- M_k x_k, backward x_k <- M_k^T x_k.  A task is a block of
+ * 64 output entries of one cblk (k, b), so that a wide cblk spreads over
+ * several workgroups; the inputs come from xin and the products go to xout,
+ * a different array (the blocks of one cblk run concurrently).  One
+ * workgroup of TH threads per (task, chunk of rchunk right-hand sides); the
+ * x entries the block needs are staged in LDS.  Each wave takes tasks whose
+ * loads are all independent, so that many loads are in flight at once:
+ *   forward:  rows 64b.. (lane = row) by chunks of 32 columns left of the
+ *             block's end, 32 coalesced loads; the waves' partial row sums
+ *             are added in LDS;
+ *   backward: columns 64b.. by groups of 16, rows from the group's first
+ *             column down in chunks of 128 (lane = row, two rows per lane),
+ *             32 coalesced loads; the 16 column sums are reduced by the
+ *             transposing butterfly and added in LDS. */
 template <typename T, bool BWD, int TH>
 __global__ __launch_bounds__( TH ) void
-k_solve_dinv( pa_dstruct S, const int32_t *cblks, const T *Dinv, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
+k_solve_dinv( pa_dstruct S, const int2 *tasks, const T *Dinv, const T *xin, int64_t ldi, T *xout, int64_t ldo,
+              int32_t nrhs, int32_t rchunk )
 {
     extern __shared__ unsigned char pa_smem_raw[];
-    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    constexpr int NW = TH / 64;
+    const int2    tk = tasks[blockIdx.x];
+    const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t b0 = 64 * tk.y, nb = min( 64, nc - b0 );
     const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
-    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] inputs   */
-    T *ys = xs + (int64_t)rchunk * nc;            /* [nr][nc] products */
     const T      *M = Dinv + S.c_dtoff[k];
     const int     t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    constexpr int NW = TH / 64;
-    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
-        const int32_t i = idx % nc, r = idx / nc;
-        xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
-        ys[r * nc + i] = T( 0 );
+    /* x entries of the block: forward columns [0, b0 + nb), backward rows
+     * [b0, nc) */
+    const int32_t x0 = BWD ? b0 : 0, xn = BWD ? nc - b0 : b0 + nb;
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][xn]                          */
+    T *ps = xs + (int64_t)nr * xn;                /* forward [NW][nr][64], backward [nr][64] */
+    for ( int32_t idx = t; idx < xn * nr; idx += TH ) {
+        const int32_t i = idx % xn, r = idx / xn;
+        xs[idx] = xin[fcol + x0 + i + ( r0 + r ) * ldi];
     }
+    for ( int32_t idx = t; idx < ( BWD ? 1 : NW ) * nr * 64; idx += TH ) ps[idx] = T( 0 );
     __syncthreads();
     if ( !BWD ) {
-        const int32_t nrb = ( nc + 63 ) >> 6, ncb = ( nc + 31 ) >> 5;
-        for ( int32_t task = wave; task < nrb * ncb; task += NW ) {
-            const int32_t rb = task / ncb, cb = task % ncb;
-            const int32_t c0 = 32 * cb;
-            if ( c0 > 64 * rb + 63 ) continue; /* above the diagonal: zero */
-            const int32_t i = 64 * rb + lane, ic = min( i, nc - 1 ), cnt = min( 32, nc - c0 );
+        const int32_t i = b0 + lane, ic = min( i, nc - 1 ), nch = ( xn + 31 ) >> 5;
+        T *pw = ps + (int64_t)wave * nr * 64;
+        for ( int32_t ch = wave; ch < nch; ch += NW ) {
+            const int32_t c0 = 32 * ch, cnt = min( 32, xn - c0 );
             T m[32];
 #pragma unroll
             for ( int j = 0; j < 32; j++ ) m[j] = M[ic + (int64_t)( c0 + min( j, cnt - 1 ) ) * nc];
             for ( int32_t r = 0; r < nr; r++ ) {
-                const T *xr  = xs + r * nc + c0;
+                const T *xr  = xs + r * xn + c0;
                 T        acc = T( 0 );
 #pragma unroll
                 for ( int j = 0; j < 32; j++ ) acc += ( j < cnt ? m[j] : T( 0 ) ) * xr[min( j, cnt - 1 )];
-                if ( i < nc ) atomicAdd( ys + r * nc + i, acc );
+                pw[r * 64 + lane] += acc; /* this wave's own slot */
             }
+        }
+        __syncthreads();
+        for ( int32_t idx = t; idx < nr * 64; idx += TH ) {
+            const int32_t r = idx >> 6, l = idx & 63;
+            if ( l >= nb ) continue;
+            T s = T( 0 );
+#pragma unroll
+            for ( int w = 0; w < NW; w++ ) s += ps[( (int64_t)w * nr + r ) * 64 + l];
+            xout[fcol + b0 + l + ( r0 + r ) * ldo] = s;
         }
     }
     else {
-        const int32_t ng = ( nc + 15 ) >> 4;
         const int     col = lane >> 2;
-        /* tasks: column group g, row chunk q from the group's first column */
-        int32_t task = 0;
+        const int32_t ng  = ( nb + 15 ) >> 4;
+        int32_t       task = 0;
         for ( int32_t g = 0; g < ng; g++ ) {
-            const int32_t c0 = 16 * g, ncols = min( 16, nc - c0 ), nq = ( nc - c0 + 63 ) >> 6;
+            const int32_t jg = b0 + 16 * g, ncols = min( 16, nc - jg ), nq = ( nc - jg + 127 ) >> 7;
             for ( int32_t q = 0; q < nq; q++, task++ ) {
                 if ( task % NW != wave ) continue;
-                const int32_t i = c0 + 64 * q + lane, ic = min( i, nc - 1 );
-                T m[16];
+                const int32_t ia = jg + 128 * q + lane, ib = ia + 64;
+                const int32_t ica = min( ia, nc - 1 ), icb = min( ib, nc - 1 );
+                T ma[16], mb[16];
 #pragma unroll
-                for ( int cc = 0; cc < 16; cc++ ) m[cc] = M[ic + (int64_t)min( c0 + cc, nc - 1 ) * nc];
+                for ( int cc = 0; cc < 16; cc++ ) {
+                    const int64_t cofs = (int64_t)min( jg + cc, nc - 1 ) * nc;
+                    ma[cc] = M[ica + cofs];
+                    mb[cc] = M[icb + cofs];
+                }
                 for ( int32_t r = 0; r < nr; r++ ) {
-                    const T yi = ( i < nc ) ? xs[r * nc + ic] : T( 0 );
+                    const T ya = ( ia < nc ) ? xs[r * xn + ica - x0] : T( 0 );
+                    const T yb = ( ib < nc ) ? xs[r * xn + icb - x0] : T( 0 );
                     T       v[16];
 #pragma unroll
-                    for ( int cc = 0; cc < 16; cc++ ) v[cc] = ( cc < ncols ? m[cc] : T( 0 ) ) * yi;
-                    const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c0 + c */
-                    if ( ( lane & 3 ) == 0 && col < ncols ) atomicAdd( ys + r * nc + c0 + col, sum );
+                    for ( int cc = 0; cc < 16; cc++ ) v[cc] = ( cc < ncols ) ? ma[cc] * ya + mb[cc] * yb : T( 0 );
+                    const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column jg + c */
+                    if ( ( lane & 3 ) == 0 && col < ncols ) atomicAdd( ps + r * 64 + 16 * g + col, sum );
                 }
             }
         }
-    }
-    __syncthreads();
-    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
-        const int32_t i = idx % nc, r = idx / nc;
-        x[fcol + i + ( r0 + r ) * ldx] = ys[r * nc + i];
+        __syncthreads();
+        for ( int32_t idx = t; idx < nr * 64; idx += TH ) {
+            const int32_t r = idx >> 6, l = idx & 63;
+            if ( l < nb ) xout[fcol + b0 + l + ( r0 + r ) * ldo] = ps[idx];
+        }
     }
 }
 
@@ -2214,11 +2295,14 @@ k_diag_transpose( pa_dstruct S, const T *Coef, T *DTab )
  * search).  A task is (cblk k, first off-diagonal row r0, first column c0,
  * descriptor index of row r0).
  * Forward:  x[rows] -= A[rows, cols] x_k[cols]: 256 rows x 64 columns, one
- *           row per lane, one atomic per row.
+ *           row per lane, one atomic per row; x_k is read from xk (the
+ *           array the diagonal step wrote its results to).
  * Backward: x_k[cols] -= A[rows, cols]^T x[rows]: each wave takes 16 columns
  *           and 256 rows (4 groups of 64) and reduces over its lanes with a
- *           transposing butterfly, one atomic per column.  A task has
- *           PA_SB_NCG(nc) column groups and 4 / PA_SB_NCG(nc) row parts.
+ *           transposing butterfly, one atomic per column; the solved rows
+ *           x[rows] are read from xa (the array the diagonal steps wrote
+ *           their results to).  A task has PA_SB_NCG(nc) column groups and
+ *           4 / PA_SB_NCG(nc) row parts.
  */
 #define PA_SF_ROWS 256
 #define PA_SF_COLS 64
@@ -2226,9 +2310,10 @@ k_diag_transpose( pa_dstruct S, const T *Coef, T *DTab )
 
 template <typename T, int RC>
 __global__ __launch_bounds__( 256 ) void
-k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
+k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx, const T *xk,
+                    int64_t ldk )
 {
-    __shared__ T xk[RC][PA_SF_COLS];
+    __shared__ T xks[RC][PA_SF_COLS];
     const int4    tk = tasks[blockIdx.x];
     const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
     const int32_t R = S.c_stride[k] - nc, c0 = tk.z, ncols = min( PA_SF_COLS, nc - c0 );
@@ -2249,7 +2334,7 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
         const int32_t nr = min( RC, nrhs - r0 );
         for ( int32_t idx = t; idx < PA_SF_COLS * nr; idx += 256 ) {
             const int32_t c = idx % PA_SF_COLS, u = idx / PA_SF_COLS;
-            xk[u][c] = ( c < ncols ) ? x[fcol + c0 + c + (int64_t)( r0 + u ) * ldx] : T( 0 );
+            xks[u][c] = ( c < ncols ) ? xk[fcol + c0 + c + (int64_t)( r0 + u ) * ldk] : T( 0 );
         }
         __syncthreads();
         if ( valid ) {
@@ -2259,7 +2344,7 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 #pragma unroll
             for ( int w = 0; w < PA_SF_COLS; w++ )
 #pragma unroll
-                for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xk[u][w];
+                for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xks[u][w];
 #pragma unroll
             for ( int u = 0; u < RC; u++ )
                 if ( u < nr ) pa_atomic_add( x + ri.x + (int64_t)( r0 + u ) * ldx, -acc[u] );
@@ -2270,7 +2355,8 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 
 template <typename T, bool MR>
 __global__ __launch_bounds__( 256 ) void
-k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx )
+k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_t nrhs, int64_t ldx, const T *xa,
+                    int64_t lda )
 {
     const int4    tk = tasks[blockIdx.x];
     const int32_t k = tk.x, nc = S.c_ncol[k], fcol = S.c_fcol[k];
@@ -2297,7 +2383,7 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
         T xv[4];
 #pragma unroll
         for ( int q = 0; q < 4; q++ ) {
-            const T v = x[ok[q] ? ri[q].x : 0];
+            const T v = xa[ok[q] ? ri[q].x : 0];
             xv[q] = ok[q] ? v : T( 0 );
         }
         T v[16];
@@ -2327,7 +2413,7 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     for ( int32_t r = 0; r < nrhs; r++ ) {
         T xv[4];
 #pragma unroll
-        for ( int q = 0; q < 4; q++ ) xv[q] = ok[q] ? x[ri[q].x + r * ldx] : T( 0 );
+        for ( int q = 0; q < 4; q++ ) xv[q] = ok[q] ? xa[ri[q].x + r * lda] : T( 0 );
         T v[16];
 #pragma unroll
         for ( int cc = 0; cc < 16; cc++ ) v[cc] = a[0][cc] * xv[0] + a[1][cc] * xv[1] + a[2][cc] * xv[2] + a[3][cc] * xv[3];
@@ -2341,16 +2427,17 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 /* wavefront per cblk for all right-hand sides: the panel is read once into  */
 /* registers (lane l: rows l, l+64, l+128; lanes < nc also the columns of    */
 /* the diagonal block) and reused for every RHS.  Forward (reference          */
-/* kernels/solve_ztrsmsp.c:302-466): x_k = L_kk^{-1} x_k, then the rows      */
-/* below are updated with atomic adds.  Backward (:468-661): x_k -= L_bk^T   */
-/* x_b gathered from the final rows below (transposing wave reduction), then */
-/* x_k = L_kk^{-T} x_k.                                                      */
+/* kernels/solve_ztrsmsp.c:302-466): x_k = L_kk^{-1} x_k, written to xo,    */
+/* then the rows below are updated in x with atomic adds.  Backward          */
+/* (:468-661): x_k -= L_bk^T x_b gathered from the final rows below (in xo,  */
+/* transposing wave reduction), then x_k = L_kk^{-T} x_k, written to xo.     */
 /* ------------------------------------------------------------------------- */
 #define PA_SM_RC 4
 
 template <typename T, bool BWD, int NS>
 __global__ __launch_bounds__( 256 ) void
-k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx )
+k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx, T *xo,
+               int64_t ldo )
 {
     const int     lane = threadIdx.x & 63;
     const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
@@ -2417,7 +2504,7 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
             for ( int u = 0; u < PA_SM_RC; u++ ) {
                 if ( !rok[u] ) continue;
                 T *xr = x + (int64_t)( r0 + u ) * ldx;
-                if ( lane < nc ) xr[fcol + lane] = xv[u];
+                if ( lane < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + lane] = xv[u];
                 T acc[NS];
 #pragma unroll
                 for ( int q = 0; q < NS; q++ ) acc[q] = T( 0 );
@@ -2440,7 +2527,7 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
             for ( int u = 0; u < PA_SM_RC; u++ )
 #pragma unroll
                 for ( int q = 0; q < NS; q++ )
-                    y[u][q] = ( rok[u] && offd[q] ) ? x[(int64_t)( r0 + u ) * ldx + grow[q]] : T( 0 );
+                    y[u][q] = ( rok[u] && offd[q] ) ? xo[(int64_t)( r0 + u ) * ldo + grow[q]] : T( 0 );
 #pragma unroll
             for ( int u = 0; u < PA_SM_RC; u++ ) {
                 T v[PA_SM_NC];
@@ -2467,7 +2554,7 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
             }
 #pragma unroll
             for ( int u = 0; u < PA_SM_RC; u++ )
-                if ( rok[u] && lane < nc ) x[(int64_t)( r0 + u ) * ldx + fcol + lane] = xv[u];
+                if ( rok[u] && lane < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + lane] = xv[u];
         }
     }
 }

@@ -63,7 +63,7 @@ def test_noalloc_solve_refuses_calls_beyond_the_scratch():
     P = Problem(problems.laplacian(*grid), grid, pa.factotype.LLT)
     lib = _lib.load()
     try:
-        nrhs = (64 << 20) // (8 * P.n) + 1   # n * nrhs * 8 bytes > 64 MiB
+        nrhs = (128 << 20) // (16 * P.n) + 1   # n * nrhs * 16 bytes > 128 MiB
         b = torch.ones((nrhs, P.n), dtype=torch.float64, device="cuda:0")
         torch.cuda.synchronize()
         rc = lib.pastix_amd_solve_dev_noalloc(P.h, NOTRANS, nrhs, C.c_void_p(b.data_ptr()), P.n)
