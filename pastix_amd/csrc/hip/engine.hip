@@ -859,14 +859,6 @@ pa_build_schedule( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_dinv_tasks, dinv_tasks ) ) ) return rc;
 
-    /* the scatter-GEMM addresses its targets by 32-bit offsets inside the
-     * target cblk */
-    for ( pastix_int_t k = 0; k < cblknbr; k++ )
-        if ( (int64_t)c_stride[k] * c_ncol[k] >= ( (int64_t)1 << 32 ) ) {
-            fprintf( stderr, "pastix_amd: cblk %lld holds 2^32 or more coefficients; lower IPARM_MAX_BLOCKSIZE\n",
-                     (long long)k );
-            return PASTIX_ERR_BADPARAMETER;
-        }
     /* LDS needs of the diagonal, TRSM and solve kernels */
     size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 * 2 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {

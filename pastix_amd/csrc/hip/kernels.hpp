@@ -1479,8 +1479,6 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 #define PA_GKC 16
 #endif
 
-template <typename T> struct pa_mfma;
-
 
 /*
  * Tile core shared by the level-batched kernel and the reference-signature
@@ -1498,13 +1496,13 @@ template <typename T> struct pa_mfma;
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T, int TM, int TN, bool OFF32>
+template <typename T, int TM, int TN>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value], int64_t cbase0 )
+              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value] )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int NBM = TM / 32;          /* MFMA blocks per wave, rho          */
@@ -1598,61 +1596,6 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
         __syncthreads();
     }
 
-    if ( OFF32 ) {
-        /* Every target of the tile lies in one cblk, at an offset below 2^32
-         * from its first coefficient (C + cbase0): the wave's NBM * NBN * 4
-         * target offsets are formed first, then all the target reads are
-         * issued together and the writes follow, so the read-modify-write
-         * costs one memory latency instead of one per output.  The outputs
-         * of a tile are distinct entries, and a target that other tiles of
-         * the same launch update is only ever updated with atomics. */
-        T *Cb = C + cbase0;
-        uint32_t off[NBN][NBM][4];
-        bool     ok[NBN][NBM][4];
-#pragma unroll
-        for ( int mj = 0; mj < NBM; mj++ ) {
-            const int      rl   = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
-            const bool     aok  = a_ld[rl] != 0;
-            const uint32_t rb   = (uint32_t)( c_off[rl] - cbase0 );
-            const uint32_t cld  = (uint32_t)c_ld[rl];
-            const int32_t  ablk = a_blk[rl];
-#pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ )
-#pragma unroll
-                for ( int r = 0; r < 4; r++ ) {
-                    const int sl = wn * ( TN / 2 ) + mi * 16 + pa_mfma<T>::row( lane, r );
-                    ok[mi][mj][r]  = aok && b_ld[sl] != 0 && ablk >= b_blk[sl] + shift;
-                    off[mi][mj][r] = ok[mi][mj][r] ? rb + (uint32_t)c_col[sl] * cld : 0u;
-                }
-        }
-        if ( atomic ) {
-#pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ )
-#pragma unroll
-                for ( int mj = 0; mj < NBM; mj++ )
-#pragma unroll
-                    for ( int r = 0; r < 4; r++ )
-                        if ( ok[mi][mj][r] ) pa_atomic_add( Cb + off[mi][mj][r], -acc[mi][mj][r] );
-        }
-        else {
-            /* a masked-off output reads the cblk's first coefficient (in range) */
-            T old[NBN][NBM][4];
-#pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ )
-#pragma unroll
-                for ( int mj = 0; mj < NBM; mj++ )
-#pragma unroll
-                    for ( int r = 0; r < 4; r++ ) old[mi][mj][r] = Cb[off[mi][mj][r]];
-#pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ )
-#pragma unroll
-                for ( int mj = 0; mj < NBM; mj++ )
-#pragma unroll
-                    for ( int r = 0; r < 4; r++ )
-                        if ( ok[mi][mj][r] ) Cb[off[mi][mj][r]] = old[mi][mj][r] - acc[mi][mj][r];
-        }
-        return;
-    }
 #pragma unroll
     for ( int mj = 0; mj < NBM; mj++ ) {
         int rl = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
@@ -1769,8 +1712,8 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *ne
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T, TM, TN, true>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
-                                 uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs, coefF );
+    pa_gemm_tile<T, TM, TN>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+                           uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
 }
 
 /*
@@ -1817,8 +1760,7 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
         b_blk[s] = 0;
     }
     __syncthreads();
-    pa_gemm_tile<T, PA_GT, PA_GT, false>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false,
-                                        As, Bs, 0 );
+    pa_gemm_tile<T, PA_GT, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
 }
 
 /* ------------------------------------------------------------------------- */
