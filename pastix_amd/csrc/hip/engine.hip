@@ -250,6 +250,10 @@ pa_lds_attr_sd( int bytes )
 {
     if ( !UNIT ) {
         pa_lds_attr( k_solve_dinv<T, false, 256>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 256>, bytes );
+        pa_lds_attr( k_solve_dinv_cblk<T, false, 64>, bytes );  pa_lds_attr( k_solve_dinv_cblk<T, true, 64>, bytes );
+        pa_lds_attr( k_solve_dinv_cblk<T, false, 128>, bytes ); pa_lds_attr( k_solve_dinv_cblk<T, true, 128>, bytes );
+        pa_lds_attr( k_solve_dinv_cblk<T, false, 256>, bytes ); pa_lds_attr( k_solve_dinv_cblk<T, true, 256>, bytes );
+        pa_lds_attr( k_solve_dinv_cblk<T, false, 512>, bytes ); pa_lds_attr( k_solve_dinv_cblk<T, true, 512>, bytes );
         pa_lds_attr( k_solve_dinv<T, false, 512>, bytes ); pa_lds_attr( k_solve_dinv<T, true, 512>, bytes );
     }
     pa_lds_attr( k_solve_diag_fwd<T, UNIT, 64>, bytes );  pa_lds_attr( k_solve_diag_bwd<T, UNIT, 64>, bytes );
@@ -1187,10 +1191,15 @@ pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x
         const int32_t n = (int32_t)( lev.sm_end[ns - 1] - beg );
         const int32_t *cb = dev->d_lev_cblks + beg;
         const unsigned g  = (unsigned)( ( n + 3 ) / 4 );
-        if ( n > 0 ) {
-            if ( ns == 1 )      k_small_solve<T, BWD, 1><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
-            else if ( ns == 2 ) k_small_solve<T, BWD, 2><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
-            else                k_small_solve<T, BWD, 3><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+        if ( n > 0 && xo == x ) {
+            if ( ns == 1 )      k_small_solve<T, BWD, 1, true><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else if ( ns == 2 ) k_small_solve<T, BWD, 2, true><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else                k_small_solve<T, BWD, 3, true><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+        }
+        else if ( n > 0 ) {
+            if ( ns == 1 )      k_small_solve<T, BWD, 1, false><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else if ( ns == 2 ) k_small_solve<T, BWD, 2, false><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
+            else                k_small_solve<T, BWD, 3, false><<<g, 256, 0, dev->stream>>>( dev->S, cb, n, C, x, nrhs, ld, xo, ldo );
         }
         beg = lev.sm_end[ns - 1];
     }
@@ -1544,10 +1553,11 @@ pa_solve_diag_launch( pa_device_s *dev, dim3 grid, size_t lds, const int32_t *cb
 
 /* diagonal solves of one level, from x (the right-hand sides with every
  * update of the other cblks applied) to xo: the products with the inverted
- * diagonal blocks on dinv levels (64-entry blocks, 256 or 512 threads), else
- * the register-prefetching kernels (workgroup of the level's widest cblk
- * rounded up to 64/128/256/512 threads) for cblks of up to PA_SD_MAXCOL
- * columns, the LDS-blocked kernel above that */
+ * diagonal blocks on dinv levels (64-entry blocks, 256 or 512 threads; in
+ * place, xo == x, one workgroup per cblk), else the register-prefetching
+ * kernels (workgroup of the level's widest cblk rounded up to 64/128/256/512
+ * threads) for cblks of up to PA_SD_MAXCOL columns, the LDS-blocked kernel
+ * above that */
 template <typename T, bool BWD, bool UNIT>
 static int
 pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const T *CT, const int64_t *ct_off,
@@ -1558,7 +1568,18 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
     const int32_t *cb = dev->d_lev_cblks + lev.cblk_beg;
     const int32_t ncmax = lev.ncmax_fac;
     if ( ncb == 0 ) return PASTIX_SUCCESS;
-    if ( Dinv && lev.dinv ) {
+    if ( Dinv && lev.dinv && xo == x ) {
+        const size_t  avail  = (size_t)dev->lds_limit - 4096;
+        const int32_t rchunk = (int32_t)std::max<size_t>( 1, std::min<size_t>( nrhs, avail / ( 2 * (size_t)ncmax * sizeof( T ) ) ) );
+        dim3          grid( ncb, ( nrhs + rchunk - 1 ) / rchunk );
+        size_t        lds = 2 * (size_t)ncmax * rchunk * sizeof( T );
+        T            *xi  = xo;
+        if ( ncmax <= 64 )       k_solve_dinv_cblk<T, BWD, 64><<<grid, 64, lds, dev->stream>>>( dev->S, cb, Dinv, xi, nrhs, rchunk, ld );
+        else if ( ncmax <= 128 ) k_solve_dinv_cblk<T, BWD, 128><<<grid, 128, lds, dev->stream>>>( dev->S, cb, Dinv, xi, nrhs, rchunk, ld );
+        else if ( ncmax <= 256 ) k_solve_dinv_cblk<T, BWD, 256><<<grid, 256, lds, dev->stream>>>( dev->S, cb, Dinv, xi, nrhs, rchunk, ld );
+        else                     k_solve_dinv_cblk<T, BWD, 512><<<grid, 512, lds, dev->stream>>>( dev->S, cb, Dinv, xi, nrhs, rchunk, ld );
+    }
+    else if ( Dinv && lev.dinv ) {
         /* products with the inverted diagonal blocks */
         const int     th     = ncmax <= 128 ? 256 : 512;
         const size_t  per    = ( (size_t)ncmax + ( th / 64 ) * 64 ) * sizeof( T );
@@ -1594,12 +1615,13 @@ pa_solve_diag_level( pa_device_s *dev, const pa_level_t &lev, const T *C, const 
 /* One forward sweep (op(C) y = b, C in lower 2D-tile storage, C's own
  * diagonal blocks) or backward sweep (op(C)^T x = y; CT/ct_off locate the
  * transposes of C's diagonal blocks) over every level, from the right-hand
- * sides in x to the solution in xo (another array of the same shape).  x is
- * the working array: the updates are added to it and the diagonal steps read
- * it; the diagonal steps write their results to xo, which the updates of
- * the same sweep then read (forward: the solved x_k of the level's own
- * cblks; backward: the solved rows of the cblks above).  Every level's
- * diagonal step can thus split a cblk over several workgroups. */
+ * sides in x to the solution in xo.  x is the working array: the updates
+ * are added to it and the diagonal steps read it; the diagonal steps write
+ * their results to xo, which the updates of the same sweep then read
+ * (forward: the solved x_k of the level's own cblks; backward: the solved
+ * rows of the cblks above).  With xo another array, every level's diagonal
+ * step can split a cblk over several workgroups; with xo == x the sweep
+ * runs in place, one workgroup per cblk. */
 template <typename T>
 static int
 pa_sweep_fwd_t( pa_device_s *dev, const T *Cf, bool unit_f, T *x, int64_t ld, T *xo, int64_t ldo, int32_t nrhs,
@@ -1671,18 +1693,24 @@ struct pa_sweep_op {
     int kind, side, unit;
 };
 
-/* Run the steps on x (n x nrhs, leading dimension ld) with alt (n x nrhs,
- * leading dimension n) as the second array of the sweeps: each sweep reads
- * one of the two and leaves its result in the other.  On return *res / *ldres
- * give the array that holds the result (x after an even number of sweeps). */
+/* Run the steps on x (n x nrhs, leading dimension ld).  One right-hand
+ * side: alt (n, leading dimension n) is the second array of the sweeps,
+ * each sweep reads one of the two and leaves its result in the other, so
+ * that the diagonal steps spread the wide cblks of the top levels over
+ * several workgroups.  Several right-hand sides: the sweeps run in place on
+ * x (alt is not used): the work of a level is wide enough, and the narrow
+ * cblks' kernels then update each right-hand-side entry where they read it
+ * instead of streaming a second array.  On return *res / *ldres give the
+ * array that holds the result (x after an even number of sweeps). */
 template <typename T>
 static int
 pa_run_ops_t( pa_device_s *dev, const pa_sweep_op *ops, int nops, T *x, int32_t nrhs, int64_t ld, T *alt, T **res,
               int64_t *ldres )
 {
     int rc;
-    T      *cur = x, *oth = alt;
-    int64_t ldc = ld, ldt = dev->n;
+    const bool inplace = nrhs > 1 || alt == nullptr;
+    T      *cur = x, *oth = inplace ? x : alt;
+    int64_t ldc = ld, ldt = inplace ? ld : dev->n;
     for ( int o = 0; o < nops; o++ ) {
         const pa_sweep_op &op = ops[o];
         const T *C = (const T *)( op.side ? dev->d_U : dev->d_L );
@@ -1786,8 +1814,9 @@ pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrh
         PA_HIP( hipMalloc( (void **)&dev->d_xs, std::max<int64_t>( len, 1 ) * sizeof( float ) ) );
         dev->xs_len = len;
     }
-    const size_t alt = (size_t)std::max<int64_t>( len, 1 ) * ( dev->prec == PA_PREC_DOUBLE ? sizeof( double ) : sizeof( float ) );
-    if ( dev->alt_bytes < alt ) {
+    /* the second array of the one-right-hand-side sweeps */
+    const size_t alt = (size_t)std::max<int64_t>( dev->n, 1 ) * ( dev->prec == PA_PREC_DOUBLE ? sizeof( double ) : sizeof( float ) );
+    if ( nrhs == 1 && dev->alt_bytes < alt ) {
         hipFree( dev->d_alt );
         dev->d_alt = nullptr;
         PA_HIP( hipMalloc( &dev->d_alt, alt ) );

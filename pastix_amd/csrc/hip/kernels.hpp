@@ -2203,6 +2203,86 @@ k_solve_dinv( pa_dstruct S, const int2 *tasks, const T *Dinv, const T *xin, int6
     }
 }
 
+/* The same products in place (x_k read and overwritten), one workgroup of
+ * TH threads per (cblk, chunk of rchunk right-hand sides): the sweeps of
+ * several right-hand sides work in place on one array, and the whole cblk's
+ * inputs are staged in LDS (xs) before any product is written back; the
+ * products accumulate in LDS (ys).
+ * The work is cut into wave tasks whose loads are all independent, so that
+ * each wave has many loads in flight at once instead of a chain:
+ *   forward:  (64 rows, 32 columns) tiles of the lower triangle; lane = row,
+ *             32 coalesced column loads, partial row sums added to ys;
+ *   backward: (16 columns, 64 rows) tiles from the group's first column
+ *             down; lane = row, 16 coalesced loads, the 16 column sums
+ *             reduced by the transposing butterfly and added to ys. */
+template <typename T, bool BWD, int TH>
+__global__ __launch_bounds__( TH ) void
+k_solve_dinv_cblk( pa_dstruct S, const int32_t *cblks, const T *Dinv, T *x, int32_t nrhs, int32_t rchunk, int64_t ldx )
+{
+    extern __shared__ unsigned char pa_smem_raw[];
+    const int32_t k = cblks[blockIdx.x], nc = S.c_ncol[k], fcol = S.c_fcol[k];
+    const int32_t r0 = blockIdx.y * rchunk, nr = min( rchunk, nrhs - r0 );
+    T *xs = reinterpret_cast<T *>( pa_smem_raw ); /* [nr][nc] inputs   */
+    T *ys = xs + (int64_t)rchunk * nc;            /* [nr][nc] products */
+    const T      *M = Dinv + S.c_dtoff[k];
+    const int     t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int NW = TH / 64;
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        const int32_t i = idx % nc, r = idx / nc;
+        xs[r * nc + i] = x[fcol + i + ( r0 + r ) * ldx];
+        ys[r * nc + i] = T( 0 );
+    }
+    __syncthreads();
+    if ( !BWD ) {
+        const int32_t nrb = ( nc + 63 ) >> 6, ncb = ( nc + 31 ) >> 5;
+        for ( int32_t task = wave; task < nrb * ncb; task += NW ) {
+            const int32_t rb = task / ncb, cb = task % ncb;
+            const int32_t c0 = 32 * cb;
+            if ( c0 > 64 * rb + 63 ) continue; /* above the diagonal: zero */
+            const int32_t i = 64 * rb + lane, ic = min( i, nc - 1 ), cnt = min( 32, nc - c0 );
+            T m[32];
+#pragma unroll
+            for ( int j = 0; j < 32; j++ ) m[j] = M[ic + (int64_t)( c0 + min( j, cnt - 1 ) ) * nc];
+            for ( int32_t r = 0; r < nr; r++ ) {
+                const T *xr  = xs + r * nc + c0;
+                T        acc = T( 0 );
+#pragma unroll
+                for ( int j = 0; j < 32; j++ ) acc += ( j < cnt ? m[j] : T( 0 ) ) * xr[min( j, cnt - 1 )];
+                if ( i < nc ) atomicAdd( ys + r * nc + i, acc );
+            }
+        }
+    }
+    else {
+        const int32_t ng = ( nc + 15 ) >> 4;
+        const int     col = lane >> 2;
+        /* tasks: column group g, row chunk q from the group's first column */
+        int32_t task = 0;
+        for ( int32_t g = 0; g < ng; g++ ) {
+            const int32_t c0 = 16 * g, ncols = min( 16, nc - c0 ), nq = ( nc - c0 + 63 ) >> 6;
+            for ( int32_t q = 0; q < nq; q++, task++ ) {
+                if ( task % NW != wave ) continue;
+                const int32_t i = c0 + 64 * q + lane, ic = min( i, nc - 1 );
+                T m[16];
+#pragma unroll
+                for ( int cc = 0; cc < 16; cc++ ) m[cc] = M[ic + (int64_t)min( c0 + cc, nc - 1 ) * nc];
+                for ( int32_t r = 0; r < nr; r++ ) {
+                    const T yi = ( i < nc ) ? xs[r * nc + ic] : T( 0 );
+                    T       v[16];
+#pragma unroll
+                    for ( int cc = 0; cc < 16; cc++ ) v[cc] = ( cc < ncols ? m[cc] : T( 0 ) ) * yi;
+                    const T sum = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c0 + c */
+                    if ( ( lane & 3 ) == 0 && col < ncols ) atomicAdd( ys + r * nc + c0 + col, sum );
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for ( int32_t idx = t; idx < nc * nr; idx += TH ) {
+        const int32_t i = idx % nc, r = idx / nc;
+        x[fcol + i + ( r0 + r ) * ldx] = ys[r * nc + i];
+    }
+}
+
 /* DT_k = D_k^T for the lower triangle of every diagonal block (LL^t), through
  * 32 x 32 LDS tiles so that reads and writes are coalesced; one workgroup
  * per cblk.  The upper triangle of DT_k is left as it is (zero). */
@@ -2376,11 +2456,15 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 /* ------------------------------------------------------------------------- */
 #define PA_SM_RC 4
 
-template <typename T, bool BWD, int NS>
+template <typename T, bool BWD, int NS, bool INPLACE>
 __global__ __launch_bounds__( 256 ) void
-k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx, T *xo,
-               int64_t ldo )
+k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx, T *xo_arg,
+               int64_t ldo_arg )
 {
+    /* in place (xo == x): one base for every access, which keeps the
+     * backward variant at three waves per SIMD in FP64 */
+    T *const      xo  = INPLACE ? x : xo_arg;
+    const int64_t ldo = INPLACE ? ldx : ldo_arg;
     const int     lane = threadIdx.x & 63;
     const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
     if ( idx >= ncb ) return; /* whole waves only; no workgroup barrier below */
