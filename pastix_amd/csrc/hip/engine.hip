@@ -863,6 +863,14 @@ pa_build_schedule( pastix_data_t *pd )
     if ( ( rc = pa_upload( &dev->d_bsolve_tasks, bsolve_tasks ) ) ) return rc;
     if ( ( rc = pa_upload( &dev->d_dinv_tasks, dinv_tasks ) ) ) return rc;
 
+    /* the scatter-GEMM reads its operands by 32-bit byte offsets inside the
+     * source cblk */
+    for ( pastix_int_t k = 0; k < cblknbr; k++ )
+        if ( (int64_t)c_stride[k] * c_ncol[k] * (int64_t)sizeof( double ) >= ( (int64_t)1 << 32 ) ) {
+            fprintf( stderr, "pastix_amd: cblk %lld holds 4 GiB or more of FP64 coefficients; lower IPARM_MAX_BLOCKSIZE\n",
+                     (long long)k );
+            return PASTIX_ERR_BADPARAMETER;
+        }
     /* LDS needs of the diagonal, TRSM and solve kernels */
     size_t need = std::max( pa_trsm_lds<double>( colmax ), (size_t)colmax * 16 * 8 * 2 ) + 4096;
     if ( need > (size_t)dev->lds_limit ) {

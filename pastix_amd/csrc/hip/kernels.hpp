@@ -1480,6 +1480,25 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 #endif
 
 
+/* Raw buffer loads at a 32-bit byte offset from a wave-uniform base (the
+ * resource lives in scalar registers, so a lane keeps one 32-bit offset per
+ * access instead of a 64-bit address). */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t
+pa_buf_rsrc( const void *base )
+{
+    return __builtin_amdgcn_make_buffer_rsrc( const_cast<void *>( base ), 0, -1, 0x00020000 );
+}
+__device__ __forceinline__ float
+pa_buf_load( __amdgpu_buffer_rsrc_t r, uint32_t off, float * )
+{
+    return __builtin_bit_cast( float, __builtin_amdgcn_raw_buffer_load_b32( r, off, 0, 0 ) );
+}
+__device__ __forceinline__ double
+pa_buf_load( __amdgpu_buffer_rsrc_t r, uint32_t off, double * )
+{
+    return __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 ) );
+}
+
 /*
  * Tile core shared by the level-batched kernel and the reference-signature
  * entry points.  TM x TN output (rho x sigma: 128 x 128, 128 x 64, 128 x 32,
@@ -1496,7 +1515,7 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, bool BUF>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
@@ -1535,23 +1554,58 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
      * chunk at the end of K, if any, uses clamped in-range addresses and is
      * masked when stored. */
     const int32_t nfull = K / PA_GKC;
-    const T *pA[NQA], *pB[NQB];
+    /* BUF: both operands lie in one source cblk, less than 2^32 bytes past
+     * Asrc / Bsrc (checked when the schedule is built), and are read with
+     * raw buffer loads at 32-bit byte offsets: one register per load
+     * instead of a 64-bit pointer */
+    const T *pA[BUF ? 1 : NQA], *pB[BUF ? 1 : NQB];
+    uint32_t oA[BUF ? NQA : 1], oB[BUF ? NQB : 1];
+    const __amdgpu_buffer_rsrc_t rA = pa_buf_rsrc( Asrc ), rB = pa_buf_rsrc( Bsrc );
+    if ( BUF ) {
 #pragma unroll
-    for ( int q = 0; q < NQA; q++ ) pA[q] = Asrc + aof + (int64_t)( ak + KLA * q ) * alds;
+        for ( int q = 0; q < NQA; q++ ) oA[q] = (uint32_t)( ( aof + (int64_t)( ak + KLA * q ) * alds ) * (int64_t)sizeof( T ) );
 #pragma unroll
-    for ( int q = 0; q < NQB; q++ ) pB[q] = Bsrc + bof + (int64_t)( bk + KLB * q ) * blds;
+        for ( int q = 0; q < NQB; q++ ) oB[q] = (uint32_t)( ( bof + (int64_t)( bk + KLB * q ) * blds ) * (int64_t)sizeof( T ) );
+    }
+    else {
+#pragma unroll
+        for ( int q = 0; q < NQA; q++ ) pA[BUF ? 0 : q] = Asrc + aof + (int64_t)( ak + KLA * q ) * alds;
+#pragma unroll
+        for ( int q = 0; q < NQB; q++ ) pB[BUF ? 0 : q] = Bsrc + bof + (int64_t)( bk + KLB * q ) * blds;
+    }
     const int64_t ca = (int64_t)PA_GKC * alds, cb = (int64_t)PA_GKC * blds;
+    const uint32_t cab = (uint32_t)( ca * (int64_t)sizeof( T ) ), cbb = (uint32_t)( cb * (int64_t)sizeof( T ) );
     auto load_full = [&]() {
+        if ( BUF ) {
 #pragma unroll
-        for ( int q = 0; q < NQA; q++ ) pa[q] = *pA[q];
+            for ( int q = 0; q < NQA; q++ ) pa[q] = pa_buf_load( rA, oA[BUF ? q : 0], (T *)nullptr );
 #pragma unroll
-        for ( int q = 0; q < NQB; q++ ) pb[q] = *pB[q];
+            for ( int q = 0; q < NQB; q++ ) pb[q] = pa_buf_load( rB, oB[BUF ? q : 0], (T *)nullptr );
+        }
+        else {
+#pragma unroll
+            for ( int q = 0; q < NQA; q++ ) pa[q] = *pA[BUF ? 0 : q];
+#pragma unroll
+            for ( int q = 0; q < NQB; q++ ) pb[q] = *pB[BUF ? 0 : q];
+        }
     };
     auto load_tail = [&]( int32_t k0 ) {
+        if ( BUF ) {
 #pragma unroll
-        for ( int q = 0; q < NQA; q++ ) pa[q] = Asrc[aof + (int64_t)min( k0 + ak + KLA * q, K - 1 ) * alds];
+            for ( int q = 0; q < NQA; q++ )
+                pa[q] = pa_buf_load( rA, (uint32_t)( ( aof + (int64_t)min( k0 + ak + KLA * q, K - 1 ) * alds ) * (int64_t)sizeof( T ) ),
+                                     (T *)nullptr );
 #pragma unroll
-        for ( int q = 0; q < NQB; q++ ) pb[q] = Bsrc[bof + (int64_t)min( k0 + bk + KLB * q, K - 1 ) * blds];
+            for ( int q = 0; q < NQB; q++ )
+                pb[q] = pa_buf_load( rB, (uint32_t)( ( bof + (int64_t)min( k0 + bk + KLB * q, K - 1 ) * blds ) * (int64_t)sizeof( T ) ),
+                                     (T *)nullptr );
+        }
+        else {
+#pragma unroll
+            for ( int q = 0; q < NQA; q++ ) pa[q] = Asrc[aof + (int64_t)min( k0 + ak + KLA * q, K - 1 ) * alds];
+#pragma unroll
+            for ( int q = 0; q < NQB; q++ ) pb[q] = Bsrc[bof + (int64_t)min( k0 + bk + KLB * q, K - 1 ) * blds];
+        }
     };
     if ( nfull > 0 ) load_full();
     else             load_tail( 0 );
@@ -1573,10 +1627,18 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
         }
         __syncthreads();
         if ( k0 + PA_GKC < K ) {
+            if ( BUF ) {
 #pragma unroll
-            for ( int q = 0; q < NQA; q++ ) pA[q] += ca;
+                for ( int q = 0; q < NQA; q++ ) oA[BUF ? q : 0] += cab;
 #pragma unroll
-            for ( int q = 0; q < NQB; q++ ) pB[q] += cb;
+                for ( int q = 0; q < NQB; q++ ) oB[BUF ? q : 0] += cbb;
+            }
+            else {
+#pragma unroll
+                for ( int q = 0; q < NQA; q++ ) pA[BUF ? 0 : q] += ca;
+#pragma unroll
+                for ( int q = 0; q < NQB; q++ ) pB[BUF ? 0 : q] += cb;
+            }
             if ( c + 1 < nfull ) load_full();
             else                 load_tail( k0 + PA_GKC );
         }
@@ -1712,8 +1774,8 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *ne
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T, TM, TN>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
-                           uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
+    pa_gemm_tile<T, TM, TN, true>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+                                 uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
 }
 
 /*
@@ -1760,7 +1822,8 @@ k_gemm_call( const T *A, const T *B, T *C, int32_t K, pa_call_rows Ar, pa_call_r
         b_blk[s] = 0;
     }
     __syncthreads();
-    pa_gemm_tile<T, PA_GT, PA_GT>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false, As, Bs );
+    pa_gemm_tile<T, PA_GT, PA_GT, false>( A, B, C, K, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk, 0, false,
+                                        As, Bs );
 }
 
 /* ------------------------------------------------------------------------- */
