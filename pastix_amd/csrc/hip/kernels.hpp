@@ -2408,9 +2408,44 @@ k_solve_fwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
     int4 ri = make_int4( 0, 0, 0, 0 ); /* an empty row reads Coef[0] */
     if ( valid ) ri = S.r_info[S.r_base[k] + v];
     const T *A = Coef + pa_rinfo_off( ri ) + (int64_t)c0 * ri.y;
-    /* the row's PA_SF_COLS coefficients, all loads issued at once from
-     * in-range addresses (columns past the cblk repeat its last one and are
-     * masked below): one memory latency per task instead of a chain */
+    /* One FP32 right-hand side: the row's PA_SF_COLS coefficients, all loads
+     * issued at once from in-range addresses (columns past the cblk repeat
+     * its last one and are masked below), one memory latency per task
+     * instead of a chain.  FP64 or several right-hand sides: that many
+     * registers would leave one or two waves per SIMD (512 VGPRs for four
+     * FP64 right-hand sides), so the coefficients are read NG columns at a
+     * time, again for each chunk of RC right-hand sides; the staged x_k
+     * entries past the cblk are zero, so those products vanish. */
+    constexpr int NG = ( RC == 1 && sizeof( T ) == 4 ) ? PA_SF_COLS : ( RC == 1 ? 32 : 16 );
+    if ( NG != PA_SF_COLS ) {
+        for ( int32_t r0 = 0; r0 < nrhs; r0 += RC ) {
+            const int32_t nr = min( RC, nrhs - r0 );
+            for ( int32_t idx = t; idx < PA_SF_COLS * nr; idx += 256 ) {
+                const int32_t c = idx % PA_SF_COLS, u = idx / PA_SF_COLS;
+                xks[u][c] = ( c < ncols ) ? xk[fcol + c0 + c + (int64_t)( r0 + u ) * ldk] : T( 0 );
+            }
+            __syncthreads();
+            if ( valid ) {
+                T acc[RC];
+#pragma unroll
+                for ( int u = 0; u < RC; u++ ) acc[u] = T( 0 );
+                for ( int32_t cg = 0; cg < ncols; cg += NG ) {
+                    T a[NG];
+#pragma unroll
+                    for ( int w = 0; w < NG; w++ ) a[w] = A[(int64_t)min( cg + w, ncols - 1 ) * ri.y];
+#pragma unroll
+                    for ( int w = 0; w < NG; w++ )
+#pragma unroll
+                        for ( int u = 0; u < RC; u++ ) acc[u] += a[w] * xks[u][cg + w];
+                }
+#pragma unroll
+                for ( int u = 0; u < RC; u++ )
+                    if ( u < nr ) pa_atomic_add( x + ri.x + (int64_t)( r0 + u ) * ldx, -acc[u] );
+            }
+            __syncthreads();
+        }
+        return;
+    }
     T a[PA_SF_COLS];
 #pragma unroll
     for ( int w = 0; w < PA_SF_COLS; w++ ) a[w] = A[(int64_t)min( w, ncols - 1 ) * ri.y];
