@@ -1250,6 +1250,19 @@ pa_launch_panel( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
     return PASTIX_SUCCESS;
 }
 
+/* k_trsm_inv with the accumulators sized for the level's widest cblk */
+template <typename T, int MODE>
+static void
+pa_trsm_inv_launch( int32_t ncmax, dim3 grid, hipStream_t st, const pa_dstruct &S, const int2 *tr, T *L, T *U,
+                    const T *Ti, int64_t ti2, T *Di, int64_t di2 )
+{
+    const int nj = ( ncmax + 63 ) / 64;
+    if ( nj <= 1 )      k_trsm_inv<T, MODE, 1><<<grid, 256, 0, st>>>( S, tr, L, U, Ti, ti2, Di, di2 );
+    else if ( nj <= 2 ) k_trsm_inv<T, MODE, 2><<<grid, 256, 0, st>>>( S, tr, L, U, Ti, ti2, Di, di2 );
+    else if ( nj <= 4 ) k_trsm_inv<T, MODE, 4><<<grid, 256, 0, st>>>( S, tr, L, U, Ti, ti2, Di, di2 );
+    else                k_trsm_inv<T, MODE, PA_TI_NJ><<<grid, 256, 0, st>>>( S, tr, L, U, Ti, ti2, Di, di2 );
+}
+
 template <typename T>
 static int
 pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
@@ -1269,16 +1282,16 @@ pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
         if ( dev->ldlt ) {
             /* L panel against D L^t, with its copy L D into the second array */
             k_tile_inv<T, 4><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            k_trsm_inv<T, 4><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
+            pa_trsm_inv_launch<T, 4>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         else if ( lu ) {
             /* both LU modes (L and U^T panels) in one launch each */
             k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
-            k_trsm_inv<T, 3><<<dim3( (unsigned)ntr, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2, Di, di2 );
+            pa_trsm_inv_launch<T, 3>( lev.ncmax_fac, dim3( (unsigned)ntr, 2 ), st, dev->S, tr, L, U, Ti, ti2, Di, di2 );
         }
         else {
             k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            k_trsm_inv<T, 0><<<(unsigned)ntr, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
+            pa_trsm_inv_launch<T, 0>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         PA_HIP( hipGetLastError() );
     }
@@ -1320,11 +1333,11 @@ pa_launch_dinv( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
     const int64_t di2  = dev->dtnbr;
     if ( dev->lu ) {
         k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
-        k_trsm_inv<T, 3><<<dim3( (unsigned)nid, 2 ), 256, 0, st>>>( dev->S, tr, L, U, Ti, ti2, Di, di2 );
+        pa_trsm_inv_launch<T, 3>( lev.ncmax_fac, dim3( (unsigned)nid, 2 ), st, dev->S, tr, L, U, Ti, ti2, Di, di2 );
     }
     else {
         k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-        k_trsm_inv<T, 0><<<(unsigned)nid, 256, 0, st>>>( dev->S, tr, L, U, Ti, 0, Di, di2 );
+        pa_trsm_inv_launch<T, 0>( lev.ncmax_fac, dim3( (unsigned)nid ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
     }
     PA_HIP( hipGetLastError() );
     return PASTIX_SUCCESS;

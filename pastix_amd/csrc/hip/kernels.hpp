@@ -1220,10 +1220,13 @@ k_tile_inv( pa_dstruct S, const int2 *tasks, int32_t ntask, const T *Lc, const T
     }
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int NJ>
 __global__ __launch_bounds__( 256 ) void
 k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_t tinv2, T *dinv, int64_t dinv2 )
 {
+    /* NJ: column tiles per wave, at least the level's widest cblk / 64
+     * (the accumulators are sized by it: a narrow level's launch keeps
+     * fewer registers per lane and more waves per SIMD) */
     /* MODE 3: both LU modes in one launch, mode 1 + blockIdx.y.  MODE 4
      * (LDL^t, reference cpucblk_zsytrfsp1d_panel, kernels/core_zsytrfsp.c:
      * 296-318): X <- X (DL^t)^{-1} as mode 1, the upper part of the factored
@@ -1234,7 +1237,6 @@ k_trsm_inv( pa_dstruct S, const int2 *tasks, T *Lc, T *Uc, const T *tinv, int64_
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int SR  = PA_TRSM_TR;
     constexpr int NRT = SR / 16;
-    constexpr int NJ  = PA_TI_NJ;
     __shared__ int64_t xoff[SR];
     __shared__ int32_t xld[SR];
     __shared__ T       xb[2][NRT][4][64]; /* solved column tile, B-operand layout */
@@ -2522,23 +2524,34 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
         if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col, -s );
         return;
     }
-    /* several right-hand sides: the tile is read once into registers */
-    T a[4][16];
+    /* several right-hand sides: the tile is read into registers once, NQ
+     * row groups at a time (all four in FP32; two in FP64, whose 64 values
+     * per lane would leave two waves per SIMD), and reused for every RHS */
+    constexpr int NQ = sizeof( T ) == 8 ? 2 : 4;
 #pragma unroll
-    for ( int q = 0; q < 4; q++ ) {
-        const T *A = Coef + pa_rinfo_off( ri[q] ) + (int64_t)cb * ri[q].y;
+    for ( int q0 = 0; q0 < 4; q0 += NQ ) {
+        T a[NQ][16];
 #pragma unroll
-        for ( int cc = 0; cc < 16; cc++ ) a[q][cc] = ( ok[q] && cc < ncols ) ? A[(int64_t)cc * ri[q].y] : T( 0 );
-    }
-    for ( int32_t r = 0; r < nrhs; r++ ) {
-        T xv[4];
+        for ( int q = 0; q < NQ; q++ ) {
+            const T *A = Coef + pa_rinfo_off( ri[q0 + q] ) + (int64_t)cb * ri[q0 + q].y;
 #pragma unroll
-        for ( int q = 0; q < 4; q++ ) xv[q] = ok[q] ? xa[ri[q].x + r * lda] : T( 0 );
-        T v[16];
+            for ( int cc = 0; cc < 16; cc++ )
+                a[q][cc] = ( ok[q0 + q] && cc < ncols ) ? A[(int64_t)cc * ri[q0 + q].y] : T( 0 );
+        }
+        for ( int32_t r = 0; r < nrhs; r++ ) {
+            T xv[NQ];
 #pragma unroll
-        for ( int cc = 0; cc < 16; cc++ ) v[cc] = a[0][cc] * xv[0] + a[1][cc] * xv[1] + a[2][cc] * xv[2] + a[3][cc] * xv[3];
-        T s = pa_wave_sum16<T>( v, lane );
-        if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col + r * ldx, -s );
+            for ( int q = 0; q < NQ; q++ ) xv[q] = ok[q0 + q] ? xa[ri[q0 + q].x + r * lda] : T( 0 );
+            T v[16];
+#pragma unroll
+            for ( int cc = 0; cc < 16; cc++ ) {
+                v[cc] = a[0][cc] * xv[0];
+#pragma unroll
+                for ( int q = 1; q < NQ; q++ ) v[cc] += a[q][cc] * xv[q];
+            }
+            T s = pa_wave_sum16<T>( v, lane );
+            if ( ( lane & 3 ) == 0 && col < ncols ) pa_atomic_add( x + fcol + cb + col + r * ldx, -s );
+        }
     }
 }
 
