@@ -2603,13 +2603,15 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
 #pragma unroll
         for ( int c = 0; c < PA_SM_NC; c++ ) P[q][c] = ( offd[q] && c < nc ) ? Lc[base + (int64_t)c * ld] : T( 0 );
     }
-    /* diagonal block: row `lane` (forward) or column `lane` (backward) */
-    T D[PA_SM_NC];
+    /* diagonal block: row `lane` (forward) or column `lane` (backward), in
+     * the first slot of the lanes below nc, which hold no off-diagonal row
+     * (one register array for both: the chains below only read it on lanes
+     * below nc, the update parts only use the off-diagonal lanes) */
+    T (&D)[PA_SM_NC] = P[0];
 #pragma unroll
-    for ( int c = 0; c < PA_SM_NC; c++ ) {
-        const bool in = lane < nc && c < nc;
-        D[c] = in ? ( BWD ? Lc[coef + (int64_t)lane * nc + c] : Lc[coef + (int64_t)c * nc + lane] ) : T( 0 );
-    }
+    for ( int c = 0; c < PA_SM_NC; c++ )
+        if ( lane < nc )
+            D[c] = ( c < nc ) ? ( BWD ? Lc[coef + (int64_t)lane * nc + c] : Lc[coef + (int64_t)c * nc + lane] ) : T( 0 );
 
     /* reciprocals of the diagonal, uniform across the wave */
     T dinv[PA_SM_NC];
@@ -2633,7 +2635,7 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
                     for ( int u = 0; u < PA_SM_RC; u++ ) {
                         const T xj = pa_readlane( xv[u], j ) * dinv[j];
                         if ( lane == j ) xv[u] = xj;
-                        else if ( lane > j ) xv[u] -= D[j] * xj; /* L[lane][j] x_j */
+                        else if ( lane > j && lane < nc ) xv[u] -= D[j] * xj; /* L[lane][j] x_j */
                     }
                 }
             }
