@@ -58,6 +58,9 @@ static void *pa_scratch_addr[PA_SCRATCH_MAXDEV];
  * a column group narrower than the tile would pad the MFMA work, so the
  * sigma extent follows the group width */
 #define PA_NGC 5
+/* flops of the current level's GEMM launched ahead of the next level's
+ * TRSM in the look-ahead (about 150 us of scatter-GEMM in FP32) */
+#define PA_LA_SPLIT_FLOP 1.0e10
 static const int pa_gc_tm[PA_NGC] = { 64, 64, 128, 128, 128 };
 static const int pa_gc_tn[PA_NGC] = { 64, 32, 128, 64, 32 };
 
@@ -76,6 +79,9 @@ struct pa_level_t {
     bool    dinv;                 /* the TRSM launch also inverts the level's
                                      diagonal blocks (solve by products)    */
     int64_t gc_beg[PA_NGC], gc_end[PA_NGC]; /* into d_gemm_tasks, per tile class */
+    int64_t gc_split[PA_NGC];     /* [gc_mid, gc_split): the part of the other
+                                     tasks launched before the next level's
+                                     TRSM is ready (look-ahead)            */
     int64_t gc_mid[PA_NGC];       /* [gc_beg, gc_mid): tasks whose target is a
                                      cblk of the next level (look-ahead)   */
     int64_t solve_beg, solve_end; /* into d_solve_tasks (forward updates)   */
@@ -826,6 +832,21 @@ pa_build_schedule( pastix_data_t *pd )
             auto mid = std::stable_partition( b0, b1, [&]( const pa_gemm_task &t ) { return level[t.F] == l + 1; } );
             L.gc_mid[c] = mid - gtasks[c].begin();
         }
+        /* look-ahead: about PA_LA_SPLIT_FLOP of the remaining tasks (widest
+         * class first, the launch order) cover the next level's diagonal
+         * factorizations and tile inverses; the rest waits for those, so
+         * that it is queued together with the next level's TRSM */
+        {
+            double acc = 0.;
+            for ( int c = PA_NGC - 1; c >= 0; c-- ) {
+                int64_t t = L.gc_mid[c];
+                for ( ; t < L.gc_end[c] && acc < PA_LA_SPLIT_FLOP; t++ ) {
+                    const pa_gemm_task &g = gtasks[c][t];
+                    acc += 2. * pa_gc_tm[c] * pa_gc_tn[c] * c_ncol[g.k];
+                }
+                L.gc_split[c] = t;
+            }
+        }
         L.solve_end = solve_tasks.size();
         L.bsolve_end = bsolve_tasks.size();
         L.dvt_end = dinv_tasks.size();
@@ -854,6 +875,7 @@ pa_build_schedule( pastix_data_t *pd )
         for ( pa_level_t &L : dev->levels ) {
             L.gc_beg[c] += base;
             L.gc_mid[c] += base;
+            L.gc_split[c] += base;
             L.gc_end[c] += base;
         }
         gemm_tasks.insert( gemm_tasks.end(), gtasks[c].begin(), gtasks[c].end() );
@@ -1263,9 +1285,12 @@ pa_trsm_inv_launch( int32_t ncmax, dim3 grid, hipStream_t st, const pa_dstruct &
     else                k_trsm_inv<T, MODE, PA_TI_NJ><<<grid, 256, 0, st>>>( S, tr, L, U, Ti, ti2, Di, di2 );
 }
 
+/* parts of a level's TRSM launch sequence: the tile inverses, the TRSM */
+enum { PA_TRSM_TINV = 1, PA_TRSM_SLABS = 2, PA_TRSM_ALL = 3 };
+
 template <typename T>
 static int
-pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
+pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st, int parts = PA_TRSM_ALL )
 {
     T *L = (T *)dev->d_L, *U = (T *)dev->d_U;
     const int lu = dev->lu;
@@ -1279,23 +1304,24 @@ pa_launch_trsm( pa_device_s *dev, const pa_level_t &lev, hipStream_t st )
         const int64_t ti2  = dev->tinv_tiles * 256; /* LU: the unit-L^T inverses follow the U ones */
         T            *Di   = (T *)dev->d_dinv;      /* inverted diagonal blocks (identity tasks) */
         const int64_t di2  = dev->dtnbr;             /* LU: the U^T array's inverses */
+        const bool tinv = ( parts & PA_TRSM_TINV ) != 0, slabs = ( parts & PA_TRSM_SLABS ) != 0;
         if ( dev->ldlt ) {
             /* L panel against D L^t, with its copy L D into the second array */
-            k_tile_inv<T, 4><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            pa_trsm_inv_launch<T, 4>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
+            if ( tinv ) k_tile_inv<T, 4><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+            if ( slabs ) pa_trsm_inv_launch<T, 4>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         else if ( lu ) {
             /* both LU modes (L and U^T panels) in one launch each */
-            k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
-            pa_trsm_inv_launch<T, 3>( lev.ncmax_fac, dim3( (unsigned)ntr, 2 ), st, dev->S, tr, L, U, Ti, ti2, Di, di2 );
+            if ( tinv ) k_tile_inv<T, 3><<<dim3( gti, 2 ), 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, ti2 );
+            if ( slabs ) pa_trsm_inv_launch<T, 3>( lev.ncmax_fac, dim3( (unsigned)ntr, 2 ), st, dev->S, tr, L, U, Ti, ti2, Di, di2 );
         }
         else {
-            k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
-            pa_trsm_inv_launch<T, 0>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
+            if ( tinv ) k_tile_inv<T, 0><<<gti, 256, 0, st>>>( dev->S, tt, ntl, L, U, Ti, 0 );
+            if ( slabs ) pa_trsm_inv_launch<T, 0>( lev.ncmax_fac, dim3( (unsigned)ntr ), st, dev->S, tr, L, U, Ti, 0, Di, di2 );
         }
         PA_HIP( hipGetLastError() );
     }
-    else if ( ntr > 0 ) {
+    else if ( ntr > 0 && ( parts & PA_TRSM_SLABS ) ) {
         size_t lds_trsm = pa_trsm_lds<T>( lev.ncmax );
         if ( dev->ldlt ) {
             k_trsm<T, 1><<<(unsigned)ntr, 256, lds_trsm, st>>>( dev->S, dev->d_trsm_tasks + lev.trsm_beg, L, U );
@@ -1396,9 +1422,9 @@ pa_factorize_t( pa_device_s *dev )
     const size_t nl = dev->levels.size();
     const bool   la = dev->lookahead && !dev->profiling;
     int rc;
-    if ( la && dev->la_ev.size() < 2 * nl ) {
+    if ( la && dev->la_ev.size() < 3 * nl ) {
         size_t old = dev->la_ev.size();
-        dev->la_ev.resize( 2 * nl );
+        dev->la_ev.resize( 3 * nl );
         for ( size_t i = old; i < dev->la_ev.size(); i++ )
             PA_HIP( hipEventCreateWithFlags( &dev->la_ev[i], hipEventDisableTiming ) );
     }
@@ -1433,9 +1459,18 @@ pa_factorize_t( pa_device_s *dev )
             PA_HIP( hipEventRecord( dev->la_ev[2 * ( li + 1 )], dev->stream ) );
             PA_HIP( hipStreamWaitEvent( dev->stream2, dev->la_ev[2 * ( li + 1 )], 0 ) );
             if ( ( rc = pa_launch_panel<T>( dev, nx, dev->stream2 ) ) ) return rc;
-            if ( ( rc = pa_launch_trsm<T>( dev, nx, dev->stream2 ) ) ) return rc;
+            if ( ( rc = pa_launch_trsm<T>( dev, nx, dev->stream2, PA_TRSM_TINV ) ) ) return rc;
+            /* the rest of this level's GEMM in two parts: the second waits
+             * until the next level's TRSM is ready to start, so that the
+             * TRSM's workgroups are queued with it instead of after a full
+             * GPU of GEMM tiles (its last slabs would otherwise run alone
+             * after this level's GEMM, holding up the next one) */
+            PA_HIP( hipEventRecord( dev->la_ev[2 * nl + li + 1], dev->stream2 ) );
+            if ( ( rc = pa_launch_trsm<T>( dev, nx, dev->stream2, PA_TRSM_SLABS ) ) ) return rc;
             PA_HIP( hipEventRecord( dev->la_ev[2 * ( li + 1 ) + 1], dev->stream2 ) );
-            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_mid, lev.gc_end ) ) ) return rc;
+            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_mid, lev.gc_split ) ) ) return rc;
+            PA_HIP( hipStreamWaitEvent( dev->stream, dev->la_ev[2 * nl + li + 1], 0 ) );
+            if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_split, lev.gc_end ) ) ) return rc;
         }
         else {
             if ( ( rc = pa_launch_gemm<T>( dev, lev.gc_beg, lev.gc_end ) ) ) return rc;
