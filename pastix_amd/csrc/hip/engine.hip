@@ -59,8 +59,10 @@ static void *pa_scratch_addr[PA_SCRATCH_MAXDEV];
  * sigma extent follows the group width */
 #define PA_NGC 5
 /* flops of the current level's GEMM launched ahead of the next level's
- * TRSM in the look-ahead (about 150 us of scatter-GEMM in FP32) */
-#define PA_LA_SPLIT_FLOP 1.0e10
+ * TRSM in the look-ahead (about 0.3 ms of scatter-GEMM in FP32, twice the
+ * next diagonal factorization; measured on the 100^3 FP32 step: 0 117.3,
+ * 5e9 113.2, 1e10 111.0, 2e10 111.0, 4e10 111.0 ms per step) */
+#define PA_LA_SPLIT_FLOP 2.0e10
 static const int pa_gc_tm[PA_NGC] = { 64, 64, 128, 128, 128 };
 static const int pa_gc_tn[PA_NGC] = { 64, 32, 128, 64, 32 };
 
@@ -652,6 +654,9 @@ pa_build_schedule( pastix_data_t *pd )
     /* PASTIX_AMD_DINV=0 keeps the substitution diagonal solves */
     const char *di_env  = getenv( "PASTIX_AMD_DINV" );
     const bool  dinv_on = !( di_env && di_env[0] == '0' );
+    /* PASTIX_AMD_LA_SPLIT: flops of the look-ahead's first GEMM part */
+    const char  *ls_env   = getenv( "PASTIX_AMD_LA_SPLIT" );
+    const double la_split = ( ls_env && atof( ls_env ) >= 0. ) ? atof( ls_env ) : PA_LA_SPLIT_FLOP;
     bool        any_dinv = false, all_dinv = !ldlt && dinv_on;
     std::vector<int2> ident_tasks, dinv_tasks;
     std::vector<int4>         solve_tasks, bsolve_tasks;
@@ -840,7 +845,7 @@ pa_build_schedule( pastix_data_t *pd )
             double acc = 0.;
             for ( int c = PA_NGC - 1; c >= 0; c-- ) {
                 int64_t t = L.gc_mid[c];
-                for ( ; t < L.gc_end[c] && acc < PA_LA_SPLIT_FLOP; t++ ) {
+                for ( ; t < L.gc_end[c] && acc < la_split; t++ ) {
                     const pa_gemm_task &g = gtasks[c][t];
                     acc += 2. * pa_gc_tm[c] * pa_gc_tn[c] * c_ncol[g.k];
                 }
