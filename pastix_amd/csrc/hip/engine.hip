@@ -1861,7 +1861,8 @@ pa_check_ops( const pa_device_s *dev, const pa_sweep_op *ops, int nops )
 }
 
 /* Run a sequence of solve steps (pa_launch_ops) with the handle's FP32
- * buffer, replayed from a cached hipGraph per (buffer, nrhs, ld, steps). */
+ * buffer and second sweep array (one right-hand side), replayed from a
+ * cached hipGraph per (buffer, nrhs, ld, steps). */
 static int
 pa_dev_run_ops( pastix_data_t *pd, const pa_sweep_op *ops, int nops, int32_t nrhs, double *d_x, int64_t ld )
 {

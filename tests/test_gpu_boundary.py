@@ -106,6 +106,60 @@ def test_subtasks_compose_the_solve(case):
         P.close()
 
 
+
+@pytest.mark.parametrize("case", ["llt", "lu-trans", "llt-mixed", "ldlt"])
+def test_one_rhs_sweeps_one_at_a_time_and_together(case):
+    """One right-hand side: the sweeps run between the right-hand sides and a
+    second array (the forward sweep leaves the solution there, the backward
+    sweep brings it back), so a lone sweep (pastix_subtask_trsm) copies its
+    result back.  Step by step with one sweep per call, the whole solve, and
+    the same columns solved three at a time (in place, one array) all agree
+    with the sparse direct solution."""
+    lu = case.startswith("lu")
+    ldlt = case == "ldlt"
+    tr = case.endswith("trans")
+    mixed = case.endswith("mixed")
+    grid = (9, 8, 7)
+    print("case", case, flush=True)
+    A = problems.convection_diffusion_27(*grid) if lu else problems.laplacian(*grid)
+    ft = pa.factotype.LU if lu else (pa.factotype.LDLT if ldlt else pa.factotype.LLT)
+    P = Problem(A, grid, ft, mixed=mixed)
+    lib = _lib.load()
+    try:
+        P.iparm[pa.iparm.transpose_solve] = TRANS if tr else NOTRANS
+        n = P.n
+        rng = np.random.default_rng(11)
+        B = np.asfortranarray(rng.uniform(-1, 1, (n, 3)))
+        Aop = sp.csc_matrix(A.T if tr else A)
+        xs = spla.spsolve(Aop, B)
+        tol = 1e-4 if mixed else 1e-11
+        # three at a time (in place)
+        X3 = pa.task_solve(P.h, P.spm, B.copy(order="F"))
+        assert _rel(X3, xs) < tol
+        for r in range(3):
+            b = np.ascontiguousarray(B[:, r])
+            # the whole solve, one right-hand side
+            x1 = pa.task_solve(P.h, P.spm, b.copy())
+            assert _rel(x1, xs[:, r]) < tol
+            assert np.abs(x1 - X3[:, r]).max() <= (1e-5 if mixed else 1e-12) * np.abs(X3[:, r]).max()
+            if ldlt:
+                continue
+            # one sweep per call
+            x = b.copy()
+            Bp = _rhs_handle()
+            assert lib.pastix_subtask_applyorder(P.h, pa.dir.Forward, n, 1, C.c_void_p(x.ctypes.data), n, Bp) == 0
+            if lu and tr:
+                steps = [(UPPER, TRANS, NONUNIT), (LOWER, TRANS, UNIT)]
+            else:
+                steps = [(LOWER, NOTRANS, NONUNIT), (LOWER, TRANS, NONUNIT)]
+            for uplo, trans, diag in steps:
+                assert lib.pastix_subtask_trsm(P.h, LEFT, uplo, trans, diag, Bp) == 0
+            assert lib.pastix_subtask_applyorder(P.h, pa.dir.Backward, n, 1, C.c_void_p(x.ctypes.data), n, Bp) == 0
+            lib.pastixRhsFinalize(Bp)
+            assert _rel(x, x1) < (1e-6 if mixed else 1e-13)
+    finally:
+        P.close()
+
 def test_subtask_diag_divides_by_the_diagonal_of_the_factor():
     """pastix_subtask_diag divides row j by the diagonal entry of its L
     diagonal block (reference kernels/solve_ztrsmsp.c:664-716)."""
