@@ -15,6 +15,7 @@
 
 typedef double pa_d4 __attribute__( ( ext_vector_type( 4 ) ) );
 typedef float  pa_f4 __attribute__( ( ext_vector_type( 4 ) ) );
+typedef float  pa_f16 __attribute__( ( ext_vector_type( 16 ) ) );
 
 /* Supernodal structure on the device (one entry per cblk / per block). */
 struct pa_dstruct {
@@ -1537,13 +1538,30 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
     const int wave = t >> 6;
     const int wm   = wave & 1;  /* rho half   */
     const int wn   = wave >> 1; /* sigma half */
-    acc_t acc[NBN][NBM];
+    /* FP32 with wave blocks of at least 32 x 32: 32x32x2 MFMAs, whose output
+     * puts 32 consecutive rho rows (consecutive target addresses) in the 32
+     * lanes of a half wave, so each epilogue instruction covers two target
+     * columns of 128 bytes instead of four of 64 */
+    constexpr bool M32 = sizeof( T ) == 4 && TM >= 64 && TN >= 64;
+    constexpr int NBM32 = M32 ? TM / 64 : 1, NBN32 = M32 ? TN / 64 : 1;
+    acc_t acc[M32 ? 1 : NBN][M32 ? 1 : NBM];
+    pa_f16 acc32[NBN32][NBM32];
+    if constexpr ( M32 ) {
 #pragma unroll
-    for ( int i = 0; i < NBN; i++ )
+        for ( int i = 0; i < NBN32; i++ )
 #pragma unroll
-        for ( int j = 0; j < NBM; j++ )
+            for ( int j = 0; j < NBM32; j++ )
 #pragma unroll
-            for ( int r = 0; r < 4; r++ ) acc[i][j][r] = T( 0 );
+                for ( int r = 0; r < 16; r++ ) acc32[i][j][r] = 0.f;
+    }
+    else {
+#pragma unroll
+        for ( int i = 0; i < NBN; i++ )
+#pragma unroll
+            for ( int j = 0; j < NBM; j++ )
+#pragma unroll
+                for ( int r = 0; r < 4; r++ ) acc[i][j][r] = T( 0 );
+    }
 
     const int arow = t % TM, ak = t / TM, brow = t % TN, bk = t / TN;
     const int32_t ald = a_ld[arow], bld = b_ld[brow];
@@ -1644,22 +1662,63 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
             if ( c + 1 < nfull ) load_full();
             else                 load_tail( k0 + PA_GKC );
         }
+        if constexpr ( M32 ) {
 #pragma unroll
-        for ( int ks = 0; ks < PA_GKC / 4; ks++ ) {
-            int kk = ks * 4 + ( lane >> 4 );
-            T xa[NBN], yb[NBM];
+            for ( int ks = 0; ks < PA_GKC / 2; ks++ ) {
+                int kk = ks * 2 + ( lane >> 5 );
+                float xa[NBN32], yb[NBM32];
 #pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ ) xa[mi] = Bs[kk][wn * ( TN / 2 ) + mi * 16 + ( lane & 15 )];
+                for ( int mi = 0; mi < NBN32; mi++ ) xa[mi] = Bs[kk][wn * ( TN / 2 ) + mi * 32 + ( lane & 31 )];
 #pragma unroll
-            for ( int mj = 0; mj < NBM; mj++ ) yb[mj] = As[kk][wm * ( TM / 2 ) + mj * 16 + ( lane & 15 )];
+                for ( int mj = 0; mj < NBM32; mj++ ) yb[mj] = As[kk][wm * ( TM / 2 ) + mj * 32 + ( lane & 31 )];
 #pragma unroll
-            for ( int mi = 0; mi < NBN; mi++ )
+                for ( int mi = 0; mi < NBN32; mi++ )
 #pragma unroll
-                for ( int mj = 0; mj < NBM; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
+                    for ( int mj = 0; mj < NBM32; mj++ )
+                        acc32[mi][mj] = __builtin_amdgcn_mfma_f32_32x32x2f32( xa[mi], yb[mj], acc32[mi][mj], 0, 0, 0 );
+            }
+        }
+        else {
+#pragma unroll
+            for ( int ks = 0; ks < PA_GKC / 4; ks++ ) {
+                int kk = ks * 4 + ( lane >> 4 );
+                T xa[NBN], yb[NBM];
+#pragma unroll
+                for ( int mi = 0; mi < NBN; mi++ ) xa[mi] = Bs[kk][wn * ( TN / 2 ) + mi * 16 + ( lane & 15 )];
+#pragma unroll
+                for ( int mj = 0; mj < NBM; mj++ ) yb[mj] = As[kk][wm * ( TM / 2 ) + mj * 16 + ( lane & 15 )];
+#pragma unroll
+                for ( int mi = 0; mi < NBN; mi++ )
+#pragma unroll
+                    for ( int mj = 0; mj < NBM; mj++ ) acc[mi][mj] = pa_mfma<T>::run( xa[mi], yb[mj], acc[mi][mj] );
+            }
         }
         __syncthreads();
     }
 
+    if constexpr ( M32 ) {
+#pragma unroll
+        for ( int mj = 0; mj < NBM32; mj++ ) {
+            const int rl = wm * ( TM / 2 ) + mj * 32 + ( lane & 31 );
+            if ( a_ld[rl] == 0 ) continue;
+            const int64_t cbase = c_off[rl];
+            const int32_t cld = c_ld[rl], ablk = a_blk[rl];
+#pragma unroll
+            for ( int mi = 0; mi < NBN32; mi++ ) {
+#pragma unroll
+                for ( int r = 0; r < 16; r++ ) {
+                    /* output element r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4 */
+                    const int sl = wn * ( TN / 2 ) + mi * 32 + 8 * ( r >> 2 ) + 4 * ( lane >> 5 ) + ( r & 3 );
+                    if ( b_ld[sl] == 0 || ablk < b_blk[sl] + shift ) continue;
+                    T *dst = C + cbase + (int64_t)c_col[sl] * cld;
+                    const T v = acc32[mi][mj][r];
+                    if ( atomic ) pa_atomic_add( dst, -v );
+                    else          *dst -= v;
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for ( int mj = 0; mj < NBM; mj++ ) {
         int rl = wm * ( TM / 2 ) + mj * 16 + ( lane & 15 );
