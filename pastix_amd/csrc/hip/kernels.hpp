@@ -1518,13 +1518,13 @@ pa_buf_load( __amdgpu_buffer_rsrc_t r, uint32_t off, double * )
  *   c_col:      target column of each sigma row
  *   a_blk/b_blk: block index of each row for the i >= j (+shift) mask
  */
-template <typename T, int TM, int TN, bool BUF>
+template <typename T, int TM, int TN, bool BUF, bool XP = false>
 __device__ __forceinline__ void
 pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int32_t K,
               const int64_t *a_off, const int32_t *a_ld, const int64_t *b_off, const int32_t *b_ld,
               const int64_t *c_off, const int32_t *c_ld, const int32_t *c_col,
               const int32_t *a_blk, const int32_t *b_blk, int32_t shift, bool atomic,
-              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value] )
+              T ( *As )[TM + pa_gpad<T>::value], T ( *Bs )[TN + pa_gpad<T>::value], T *Cs = nullptr )
 {
     typedef typename pa_mfma<T>::acc_t acc_t;
     constexpr int NBM = TM / 32;          /* MFMA blocks per wave, rho          */
@@ -1696,6 +1696,31 @@ pa_gemm_tile( const T *__restrict__ Asrc, const T *__restrict__ Bsrc, T *C, int3
         __syncthreads();
     }
 
+    if constexpr ( XP ) {
+        /* 64 x 64 FP32 tile: the output goes through LDS (Cs[sigma][rho],
+         * the operand area, free after the K loop's last barrier), then each
+         * wave updates 16 target columns, one column of 64 rho rows per
+         * instruction (lane = rho row) */
+        static_assert( M32 && TM == 64 && TN == 64, "output staging is for 64 x 64 FP32 tiles" );
+#pragma unroll
+        for ( int r = 0; r < 16; r++ )
+            Cs[( wn * 32 + 8 * ( r >> 2 ) + 4 * ( lane >> 5 ) + ( r & 3 ) ) * TM + wm * 32 + ( lane & 31 )] =
+                acc32[0][0][r];
+        __syncthreads();
+        if ( a_ld[lane] == 0 ) return;
+        const int64_t cbase = c_off[lane];
+        const int32_t cld = c_ld[lane], ablk = a_blk[lane];
+#pragma unroll
+        for ( int j = 0; j < 16; j++ ) {
+            const int sl = wave * 16 + j;
+            if ( b_ld[sl] == 0 || ablk < b_blk[sl] + shift ) continue;
+            T *dst = C + cbase + (int64_t)c_col[sl] * cld;
+            const T v = Cs[sl * TM + lane];
+            if ( atomic ) pa_atomic_add( dst, -v );
+            else          *dst -= v;
+        }
+        return;
+    }
     if constexpr ( M32 ) {
 #pragma unroll
         for ( int mj = 0; mj < NBM32; mj++ ) {
@@ -1750,12 +1775,23 @@ __global__ __launch_bounds__( 256 ) void
 k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *nerr )
 {
     constexpr int PAD = pa_gpad<T>::value;
-    __shared__ T As[PA_GKC][TM + PAD];
-    __shared__ T Bs[PA_GKC][TN + PAD];
+    /* FP32 64 x 64 tiles stage their output through LDS for the epilogue */
+    constexpr bool XP = sizeof( T ) == 4 && TM == 64 && TN == 64;
+    /* one LDS area, used in turn by the target-block staging (prologue),
+     * the operand chunks (K loop) and the output tile (epilogue) */
+    constexpr size_t AB_BYTES = sizeof( T ) * PA_GKC * ( TM + PAD + TN + PAD );
+    constexpr size_t FB_BYTES = sizeof( int32_t ) * 3 * PA_GF_STAGE;
+    constexpr size_t CS_BYTES = XP ? sizeof( T ) * TM * TN : 0;
+    constexpr size_t SH_BYTES = AB_BYTES > FB_BYTES ? ( AB_BYTES > CS_BYTES ? AB_BYTES : CS_BYTES )
+                                                    : ( FB_BYTES > CS_BYTES ? FB_BYTES : CS_BYTES );
+    __shared__ __attribute__( ( aligned( 16 ) ) ) unsigned char smem[SH_BYTES];
+    T ( *As )[TM + PAD] = reinterpret_cast<T ( * )[TM + PAD]>( smem );
+    T ( *Bs )[TN + PAD] = reinterpret_cast<T ( * )[TN + PAD]>( smem + sizeof( T ) * PA_GKC * ( TM + PAD ) );
     __shared__ int64_t a_off[TM], b_off[TN], c_off[TM];
     __shared__ int32_t a_ld[TM], b_ld[TN], c_ld[TM], c_col[TN], a_blk[TM], b_blk[TN], grow_s[TM];
     /* first rows / last rows / row offsets of the target cblk's blocks */
-    __shared__ int32_t f_frow[PA_GF_STAGE], f_lrow[PA_GF_STAGE], f_roff[PA_GF_STAGE];
+    int32_t *f_frow = reinterpret_cast<int32_t *>( smem );
+    int32_t *f_lrow = f_frow + PA_GF_STAGE, *f_roff = f_lrow + PA_GF_STAGE;
 
     const pa_gemm_task tk = tasks[blockIdx.x];
     const int32_t k = tk.k, F = tk.F, nc = S.c_ncol[k];
@@ -1835,8 +1871,9 @@ k_gemm_level( pa_dstruct S, const pa_gemm_task *tasks, T *Lc, T *Uc, int32_t *ne
     const T *Asrc = ( uside ? Uc : Lc ) + cbase;
     const T *Bsrc = ( ( Uc != nullptr && !uside ) ? Uc : Lc ) + cbase;
     T *C = uside ? Uc : Lc;
-    pa_gemm_tile<T, TM, TN, true>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
-                                 uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs );
+    pa_gemm_tile<T, TM, TN, true, XP>( Asrc, Bsrc, C, nc, a_off, a_ld, b_off, b_ld, c_off, c_ld, c_col, a_blk, b_blk,
+                                     uside ? 1 : 0, ( tk.flags & PA_TASK_ATOMIC ) != 0, As, Bs,
+                                     reinterpret_cast<T *>( smem ) );
 }
 
 /*
