@@ -7,7 +7,8 @@
  *
  * Precision is a template parameter: T = double (FP64 path) or float (FP32
  * factor of the mixed-precision path).  FP32 and FP64 matrix products run on
- * v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64.
+ * v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, the FP32 scatter-GEMM tiles
+ * with 32 x 32 wave blocks on v_mfma_f32_32x32x2f32.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -1472,10 +1473,12 @@ k_trsm_call( T *C, const T *D, int32_t nc, pa_call_rows R )
 /* tiled into 64 x 64 output tiles (sigma = columns, rho = rows).  At tile   */
 /* start the row map of the tile -- source offsets and leading dimensions,   */
 /* and the target offset of each rho row inside F -- is staged in LDS; the   */
-/* product runs on MFMA 16x16x4 and the epilogue writes straight into F.     */
-/* The product is computed transposed (sigma x rho) so that the 16 lanes of  */
-/* an MFMA output row hold 16 consecutive rows of one target column: the     */
-/* scatter store is coalesced.                                               */
+/* product runs on MFMA 16x16x4 (FP32 32 x 32 wave blocks: 32x32x2) and the */
+/* epilogue writes into F.  The product is computed transposed (sigma x rho) */
+/* so that the lanes of an MFMA output row hold 16 (32x32x2: 32) consecutive */
+/* rows of one target column: the scatter store is coalesced; 64 x 64 FP32   */
+/* tiles go through an LDS copy of the output so that an instruction covers  */
+/* 64 consecutive rows.                                                      */
 /* ------------------------------------------------------------------------- */
 #define PA_GT  64
 #ifndef PA_GKC
@@ -1506,8 +1509,8 @@ pa_buf_load( __amdgpu_buffer_rsrc_t r, uint32_t off, double * )
  * Tile core shared by the level-batched kernel and the reference-signature
  * entry points.  TM x TN output (rho x sigma: 128 x 128, 128 x 64, 128 x 32,
  * 64 x 64, 64 x 32), 4 waves in a 2 x 2 arrangement, K in chunks of PA_GKC staged in LDS, the next chunk prefetched
- * into registers while the current one feeds the MFMAs (a chunk of 32 keeps
- * the MFMAs busy for longer than the global load latency).  The loads are
+ * into registers while the current one feeds the MFMAs (deeper chunks cost
+ * more registers and LDS than their longer MFMA runs win back).  The loads are
  * unconditional from clamped in-range addresses (empty rows are zeroed when
  * the chunk is stored to LDS), so no per-load branch holds the prefetch back.
  * Row maps (in LDS)
