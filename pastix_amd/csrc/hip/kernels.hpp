@@ -878,10 +878,8 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
     }
 
     /* scatter update into the facing cblks, one group of consecutive blocks
-     * facing the same cblk F at a time.  Every block of the cblk lies inside
-     * one block of F, found by walking F's blocks in row order (the blocks
-     * of both are sorted); the lanes of the rows of block i then take that
-     * target block. */
+     * facing the same cblk F at a time; the rows of the blocks from the
+     * group on are written into the blocks of F that hold them. */
     int32_t bidx[NS];
 #pragma unroll
     for ( int q = 0; q < NS; q++ ) {
@@ -905,17 +903,18 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
             tld[q]   = 0;
             tok[q]   = false;
         }
-        int32_t f        = S.c_fblok[F];
-        for ( int32_t i = gb; i < lblk; i++ ) {
-            const int32_t fr = S.b_frow[i];
-            while ( f < flast && S.b_lrow[f] < fr ) f++;
-            const int32_t ffr = S.b_frow[f], flr = S.b_lrow[f];
-            const int64_t fbase = cF + (int64_t)S.b_rowoff[f] * ncF - ffr;
+        /* each lane finds the block of F holding its row by a binary
+         * search over F's blocks (a few loads in parallel, not a walk of
+         * one dependent scalar load per block) */
+        const int32_t ffb = S.c_fblok[F];
 #pragma unroll
-            for ( int q = 0; q < NS; q++ ) {
-                if ( bidx[q] == i && grow[q] >= ffr && grow[q] <= flr ) {
+        for ( int q = 0; q < NS; q++ ) {
+            if ( bidx[q] >= gb ) {
+                const int32_t f   = pa_find_globrow( S.b_frow, ffb, flast, grow[q] );
+                const int32_t ffr = S.b_frow[f], flr = S.b_lrow[f];
+                if ( grow[q] >= ffr && grow[q] <= flr ) {
                     tok[q]   = true;
-                    tbase[q] = fbase + grow[q];
+                    tbase[q] = cF + (int64_t)S.b_rowoff[f] * ncF - ffr + grow[q];
                     tld[q]   = flr - ffr + 1;
                 }
             }
