@@ -117,7 +117,7 @@ struct pa_tune_t {
     int32_t small;
     int32_t tall64;
 };
-static const pa_tune_t pa_tune_default = { 384, 1, 1, 0 };
+static const pa_tune_t pa_tune_default = { 1 << 30, 1, 1, 0 };
 
 struct pa_wblock {
     void  *p;
@@ -910,8 +910,8 @@ pa_build_schedule( pastix_data_t *pd )
 
 /* Candidate schedules of the autotuner (pastix_amd_tune), the default first */
 static const pa_tune_t pa_tune_cands[] = {
-    { 384, 1, 1, 0 }, { 256, 1, 1, 0 }, { 512, 1, 1, 0 }, { 384, 0, 1, 0 }, { 1 << 30, 1, 1, 0 }, { 384, 1, 0, 0 },
-    { 384, 1, 1, 1 }, { 256, 1, 1, 1 },
+    { 1 << 30, 1, 1, 0 }, { 384, 1, 1, 0 }, { 256, 1, 1, 0 }, { 512, 1, 1, 0 }, { 1 << 30, 0, 1, 0 },
+    { 1 << 30, 1, 0, 0 }, { 384, 1, 1, 1 }, { 256, 1, 1, 1 },
 };
 #define PA_NTUNE ( (int)( sizeof( pa_tune_cands ) / sizeof( pa_tune_cands[0] ) ) )
 

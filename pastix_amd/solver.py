@@ -236,7 +236,7 @@ def get_structure(handle):
 
 TUNE_NCAND = 8   # PASTIX_AMD_TUNE_NCAND
 TUNE_CANDIDATES = [  # {tm_min, narrow, small, tall64}: the order of the library's candidate list
-    (384, 1, 1, 0), (256, 1, 1, 0), (512, 1, 1, 0), (384, 0, 1, 0), (1 << 30, 1, 1, 0), (384, 1, 0, 0),
+    (1 << 30, 1, 1, 0), (384, 1, 1, 0), (256, 1, 1, 0), (512, 1, 1, 0), (1 << 30, 0, 1, 0), (1 << 30, 1, 0, 0),
     (384, 1, 1, 1), (256, 1, 1, 1)]
 
 

@@ -133,7 +133,7 @@ def test_tuning_candidate_count_matches_the_header():
     txt = open(os.path.join(ROOT, "include", "pastix.h")).read()
     m = re.search(r"#define\s+PASTIX_AMD_TUNE_NCAND\s+(\d+)", txt)
     assert m and int(m.group(1)) == TUNE_NCAND == len(TUNE_CANDIDATES)
-    assert TUNE_CANDIDATES[0] == (384, 1, 1, 0)   # the default schedule is candidate 0
+    assert TUNE_CANDIDATES[0] == (1 << 30, 1, 1, 0)   # the default schedule is candidate 0
 
 
 def _integration_program():
