@@ -73,6 +73,9 @@ struct pa_level_t {
                                   /* cblks of the fused one-wave kernels,   */
     int64_t sm_end[3];            /* by row slots: [fac_end, sm_end[0]) of  */
                                   /* <= 64 rows, then <= 128, <= 192        */
+    int64_t sm_pair_end;          /* [fac_end, sm_pair_end): the <= 64-row  */
+                                  /* ones of at most 32 rows, solved two    */
+                                  /* per wavefront                          */
     int64_t trsm_beg, trsm_end;   /* into d_trsm_tasks (32-row slabs)       */
     int64_t trsm_mid;             /* [trsm_mid, trsm_end): identity slabs of
                                      the diagonal-block inverses            */
@@ -723,9 +726,21 @@ pa_build_schedule( pastix_data_t *pd )
                 L.ncmax_fac = std::max( L.ncmax_fac, c_ncol[k] );
             }
         L.fac_end = lev_cblks.size();
+        /* narrow cblks by row slots; those of at most 32 rows first, by
+         * width, so that the two cblks of a wavefront in the solve have
+         * about the same number of columns */
+        {
+            const size_t p0 = lev_cblks.size();
+            for ( int32_t k : bylev[l] )
+                if ( is_small( k ) && c_stride[k] <= 32 ) lev_cblks.push_back( k );
+            std::stable_sort( lev_cblks.begin() + p0, lev_cblks.end(),
+                              [&]( int32_t a, int32_t b ) { return c_ncol[a] < c_ncol[b]; } );
+        }
+        L.sm_pair_end = lev_cblks.size();
         for ( int ns = 1; ns <= 3; ns++ ) {
             for ( int32_t k : bylev[l] )
-                if ( is_small( k ) && ( c_stride[k] + 63 ) / 64 == ns ) lev_cblks.push_back( k );
+                if ( is_small( k ) && ( c_stride[k] + 63 ) / 64 == ns && ( ns > 1 || c_stride[k] > 32 ) )
+                    lev_cblks.push_back( k );
             L.sm_end[ns - 1] = lev_cblks.size();
         }
         /* number of distinct source cblks of this level per target cblk */
@@ -1222,6 +1237,15 @@ pa_small_solve_launch( pa_device_s *dev, const pa_level_t &lev, const T *C, T *x
                        int32_t nrhs )
 {
     int64_t beg = lev.fac_end;
+    /* cblks of at most 32 rows: two per wavefront (eight per workgroup) */
+    const int32_t np = (int32_t)( lev.sm_pair_end - beg );
+    if ( np > 0 ) {
+        const int32_t *cb = dev->d_lev_cblks + beg;
+        const unsigned g  = (unsigned)( ( np + 7 ) / 8 );
+        if ( xo == x ) k_small_solve<T, BWD, 1, true, true><<<g, 256, 0, dev->stream>>>( dev->S, cb, np, C, x, nrhs, ld, xo, ldo );
+        else           k_small_solve<T, BWD, 1, false, true><<<g, 256, 0, dev->stream>>>( dev->S, cb, np, C, x, nrhs, ld, xo, ldo );
+        beg = lev.sm_pair_end;
+    }
     for ( int ns = 1; ns <= 3; ns++ ) {
         const int32_t n = (int32_t)( lev.sm_end[ns - 1] - beg );
         const int32_t *cb = dev->d_lev_cblks + beg;

@@ -2067,6 +2067,37 @@ pa_wave_sum16( T v[16], int lane )
     return s;
 }
 
+/* The same over each half wavefront (32 lanes): column c of the half at
+ * lanes hb + 2c and hb + 2c + 1. */
+template <typename T>
+__device__ __forceinline__ T
+pa_half_sum16( T v[16], int lane )
+{
+    const bool b4 = lane & 16, b3 = lane & 8, b2 = lane & 4, b1 = lane & 2;
+#pragma unroll
+    for ( int k = 0; k < 8; k++ ) {
+        T send = b4 ? v[k] : v[k + 8];
+        T keep = b4 ? v[k + 8] : v[k];
+        v[k]   = keep + __shfl_xor( send, 16 );
+    }
+#pragma unroll
+    for ( int k = 0; k < 4; k++ ) {
+        T send = b3 ? v[k] : v[k + 4];
+        T keep = b3 ? v[k + 4] : v[k];
+        v[k]   = keep + __shfl_xor( send, 8 );
+    }
+#pragma unroll
+    for ( int k = 0; k < 2; k++ ) {
+        T send = b2 ? v[k] : v[k + 2];
+        T keep = b2 ? v[k + 2] : v[k];
+        v[k]   = keep + __shfl_xor( send, 4 );
+    }
+    T send = b1 ? v[0] : v[1];
+    T s    = ( b1 ? v[1] : v[0] ) + __shfl_xor( send, 2 );
+    s += __shfl_xor( s, 1 );
+    return s;
+}
+
 
 /*
  * Diagonal solves for cblks of at most TH columns, one workgroup per (cblk,
@@ -2665,21 +2696,35 @@ k_solve_bwd_update( pa_dstruct S, const int4 *tasks, const T *Coef, T *x, int32_
 /* ------------------------------------------------------------------------- */
 #define PA_SM_RC 4
 
-template <typename T, bool BWD, int NS, bool INPLACE>
+template <typename T, bool BWD, int NS, bool INPLACE, bool PAIR = false>
 __global__ __launch_bounds__( 256 ) void
 k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *x, int32_t nrhs, int64_t ldx, T *xo_arg,
                int64_t ldo_arg )
 {
+    /* PAIR: the two half wavefronts take one cblk of at most 32 rows each
+     * (hl: lane in the half, hb: first lane of the half); broadcasts stay
+     * inside a half.  A half without a cblk (odd count) has nc = stride = 0,
+     * so its loads, stores and atomics are all masked off. */
+    static_assert( !PAIR || NS == 1, "two cblks per wavefront: one row slot of 32" );
     /* in place (xo == x): one base for every access, which keeps the
      * backward variant at three waves per SIMD in FP64 */
     T *const      xo  = INPLACE ? x : xo_arg;
     const int64_t ldo = INPLACE ? ldx : ldo_arg;
     const int     lane = threadIdx.x & 63;
-    const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
-    if ( idx >= ncb ) return; /* whole waves only; no workgroup barrier below */
-    const int32_t k = cblks[idx];
-    const int32_t nc = S.c_ncol[k], stride = S.c_stride[k], fcol = S.c_fcol[k];
+    const int     hl = PAIR ? ( lane & 31 ) : lane, hb = PAIR ? ( lane & 32 ) : 0;
+    const int32_t w0   = ( (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 ) ) * ( PAIR ? 2 : 1 );
+    if ( w0 >= ncb ) return; /* whole waves only; no workgroup barrier below */
+    const int32_t idx = w0 + ( PAIR ? ( lane >> 5 ) : 0 );
+    const bool    act = idx < ncb;
+    const int32_t k = cblks[act ? idx : w0];
+    const int32_t nc = act ? S.c_ncol[k] : 0, stride = act ? S.c_stride[k] : 0, fcol = S.c_fcol[k];
     const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
+    /* value of lane j of this lane's cblk */
+    auto bcast = [&]( T v, int j ) -> T { return PAIR ? __shfl( v, hb + j, 64 ) : pa_readlane( v, j ); };
+    /* column loops: the wave-uniform bound nc, or with two cblks the larger
+     * of their widths (columns past a cblk's nc are no-ops for its half:
+     * zero reciprocals and panel entries) */
+    const int32_t ncl = PAIR ? __builtin_amdgcn_readfirstlane( max( nc, __shfl_xor( nc, 32, 64 ) ) ) : nc;
 
     /* off-diagonal rows: slots q with row lane + 64 q in [nc, stride) */
     T       P[NS][PA_SM_NC];
@@ -2687,7 +2732,7 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
     bool    offd[NS];
 #pragma unroll
     for ( int q = 0; q < NS; q++ ) {
-        const int32_t r = lane + 64 * q;
+        const int32_t r = hl + 64 * q;
         offd[q]          = r >= nc && r < stride;
         int64_t base     = 0;
         int32_t ld       = 1;
@@ -2708,13 +2753,16 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
     T (&D)[PA_SM_NC] = P[0];
 #pragma unroll
     for ( int c = 0; c < PA_SM_NC; c++ )
-        if ( lane < nc )
-            D[c] = ( c < nc ) ? ( BWD ? Lc[coef + (int64_t)lane * nc + c] : Lc[coef + (int64_t)c * nc + lane] ) : T( 0 );
+        if ( hl < nc )
+            D[c] = ( c < nc ) ? ( BWD ? Lc[coef + (int64_t)hl * nc + c] : Lc[coef + (int64_t)c * nc + hl] ) : T( 0 );
 
     /* reciprocals of the diagonal, uniform across the wave */
     T dinv[PA_SM_NC];
 #pragma unroll
-    for ( int j = 0; j < PA_SM_NC; j++ ) dinv[j] = ( j < nc ) ? T( 1 ) / pa_readlane( D[j], j ) : T( 0 );
+    for ( int j = 0; j < PA_SM_NC; j++ ) {
+        const T dj = bcast( D[j], j );
+        dinv[j]    = ( j < nc ) ? T( 1 ) / dj : T( 0 );
+    }
 
     /* right-hand sides PA_SM_RC at a time: independent chains interleave */
     for ( int32_t r0 = 0; r0 < nrhs; r0 += PA_SM_RC ) {
@@ -2723,17 +2771,17 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
 #pragma unroll
         for ( int u = 0; u < PA_SM_RC; u++ ) {
             rok[u] = r0 + u < nrhs;
-            xv[u]  = ( rok[u] && lane < nc ) ? x[(int64_t)( r0 + u ) * ldx + fcol + lane] : T( 0 );
+            xv[u]  = ( rok[u] && hl < nc ) ? x[(int64_t)( r0 + u ) * ldx + fcol + hl] : T( 0 );
         }
         if ( !BWD ) {
 #pragma unroll
             for ( int j = 0; j < PA_SM_NC; j++ ) {
-                if ( j < nc ) {
+                if ( j < ncl ) {
 #pragma unroll
                     for ( int u = 0; u < PA_SM_RC; u++ ) {
-                        const T xj = pa_readlane( xv[u], j ) * dinv[j];
-                        if ( lane == j ) xv[u] = xj;
-                        else if ( lane > j && lane < nc ) xv[u] -= D[j] * xj; /* L[lane][j] x_j */
+                        const T xj = bcast( xv[u], j ) * dinv[j];
+                        if ( hl == j ) xv[u] = xj;
+                        else if ( hl > j && hl < nc ) xv[u] -= D[j] * xj; /* L[hl][j] x_j */
                     }
                 }
             }
@@ -2741,14 +2789,14 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
             for ( int u = 0; u < PA_SM_RC; u++ ) {
                 if ( !rok[u] ) continue;
                 T *xr = x + (int64_t)( r0 + u ) * ldx;
-                if ( lane < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + lane] = xv[u];
+                if ( hl < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + hl] = xv[u];
                 T acc[NS];
 #pragma unroll
                 for ( int q = 0; q < NS; q++ ) acc[q] = T( 0 );
 #pragma unroll
                 for ( int c = 0; c < PA_SM_NC; c++ ) {
-                    if ( c < nc ) {
-                        const T xc = pa_readlane( xv[u], c );
+                    if ( c < ncl ) {
+                        const T xc = bcast( xv[u], c );
 #pragma unroll
                         for ( int q = 0; q < NS; q++ ) acc[q] += P[q][c] * xc;
                     }
@@ -2774,24 +2822,30 @@ k_small_solve( pa_dstruct S, const int32_t *cblks, int32_t ncb, const T *Lc, T *
 #pragma unroll
                     for ( int q = 0; q < NS; q++ ) v[c] += P[q][c] * y[u][q];
                 }
-                const T s = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c */
-                xv[u] -= __shfl( s, ( lane & 15 ) << 2 );
+                if ( PAIR ) {
+                    const T s = pa_half_sum16<T>( v, lane ); /* lane hb + 2c: column c */
+                    xv[u] -= __shfl( s, hb + ( ( hl & 15 ) << 1 ) );
+                }
+                else {
+                    const T s = pa_wave_sum16<T>( v, lane ); /* lane 4c: column c */
+                    xv[u] -= __shfl( s, ( lane & 15 ) << 2 );
+                }
             }
             /* x_k = L_kk^{-T} x_k: lane c holds column c of L_kk in D */
 #pragma unroll
             for ( int j = PA_SM_NC - 1; j >= 0; j-- ) {
-                if ( j < nc ) {
+                if ( j < ncl ) {
 #pragma unroll
                     for ( int u = 0; u < PA_SM_RC; u++ ) {
-                        const T xj = pa_readlane( xv[u], j ) * dinv[j];
-                        if ( lane == j ) xv[u] = xj;
-                        else if ( lane < j ) xv[u] -= D[j] * xj; /* L[j][lane] x_j */
+                        const T xj = bcast( xv[u], j ) * dinv[j];
+                        if ( hl == j ) xv[u] = xj;
+                        else if ( hl < j ) xv[u] -= D[j] * xj; /* L[j][hl] x_j */
                     }
                 }
             }
 #pragma unroll
             for ( int u = 0; u < PA_SM_RC; u++ )
-                if ( rok[u] && lane < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + lane] = xv[u];
+                if ( rok[u] && hl < nc ) xo[(int64_t)( r0 + u ) * ldo + fcol + hl] = xv[u];
         }
     }
 }
