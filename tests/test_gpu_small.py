@@ -2,7 +2,8 @@
 factors as the general diagonal / TRSM / scatter-GEMM kernels (switched off
 with PASTIX_AMD_SMALL=0), in FP64 and FP32, on 2D and 3D grids where most
 cblks are narrow; both agree with the CPU oracle, and so do the multi-RHS
-solves of the narrow cblks (k_small_solve) and of the general kernels."""
+and one-RHS solves of the narrow cblks (k_small_solve, two cblks of at most
+32 rows per wavefront) and of the general kernels."""
 import os
 
 import numpy as np
@@ -57,6 +58,8 @@ def test_narrow_cblk_kernel_matches_general_kernels(grid, mixed):
         cblk, blok, perm, invp, dims = pa.get_structure(h1)
         nc = cblk[:, 1] - cblk[:, 0] + 1
         assert (nc <= 16).mean() > 0.5          # the narrow path is exercised
+        # narrow cblks of at most 32 rows: solved two per wavefront
+        assert ((nc <= 16) & (cblk[:, 3] <= 32)).sum() > 0
         keep = _lower_mask(cblk, int(dims[3]))
         L1, L0 = pa.get_coeftab(h1, 0)[keep], pa.get_coeftab(h0, 0)[keep]
         tol = 2e-6 if mixed else 1e-13
@@ -75,6 +78,9 @@ def test_narrow_cblk_kernel_matches_general_kernels(grid, mixed):
         assert np.abs(X1 - X0).max() <= (1e-4 if mixed else 1e-12) * np.abs(X0).max()
         for j in range(5):
             assert problems.backward_error(A, X1[:, j], B[:, j]) <= (1e-6 if mixed else 1e-14)
+        x1 = B[:, :1].copy(order="F")
+        pa.task_solve(h1, s1, x1)                  # one RHS: the two-array sweeps
+        assert np.abs(x1[:, 0] - X0[:, 0]).max() <= (1e-4 if mixed else 1e-12) * np.abs(X0).max()
     finally:
         pa.finalize(h1)
         pa.finalize(h0)
