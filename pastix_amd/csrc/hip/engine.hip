@@ -1218,6 +1218,13 @@ static void
 pa_small_llt_launch( pa_device_s *dev, const pa_level_t &lev, T *L, const double *crit, int32_t *c0, int32_t *c1 )
 {
     int64_t beg = lev.fac_end;
+    /* cblks of at most 32 rows: two per wavefront (eight per workgroup) */
+    const int32_t np = (int32_t)( lev.sm_pair_end - beg );
+    if ( np > 0 ) {
+        const int32_t *cb = dev->d_lev_cblks + beg;
+        k_small_llt<T, 1, true><<<(unsigned)( ( np + 7 ) / 8 ), 256, 0, dev->stream>>>( dev->S, cb, np, L, crit, c0, c1, c0 + 3 );
+        beg = lev.sm_pair_end;
+    }
     for ( int ns = 1; ns <= 3; ns++ ) {
         const int32_t n = (int32_t)( lev.sm_end[ns - 1] - beg );
         const int32_t *cb = dev->d_lev_cblks + beg;

@@ -798,18 +798,30 @@ k_diag_potrf_reg_one( float *D, int32_t nc, double crit, int32_t *npiv, int32_t 
 #define PA_SM_NC   16
 #define PA_SM_ROWS 192
 
-template <typename T, int NS>
+template <typename T, int NS, bool PAIR = false>
 __global__ __launch_bounds__( 256 ) void
 k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const double *critp, int32_t *npiv,
              int32_t *nneg, int32_t *nerr )
 {
+    /* PAIR: the two half wavefronts take one cblk of at most 32 rows each
+     * (hl: lane in the half, hb: first lane of the half); broadcasts stay
+     * inside a half, and the update loops below run per half.  A half
+     * without a cblk (odd count) has nc = stride = 0 and no blocks. */
+    static_assert( !PAIR || NS == 1, "two cblks per wavefront: one row slot of 32" );
     const int     lane = threadIdx.x & 63;
-    const int32_t idx  = (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 );
-    if ( idx >= ncb ) return; /* whole waves only; no workgroup barrier below */
-    const int32_t k = cblks[idx];
-    const int32_t nc = S.c_ncol[k], stride = S.c_stride[k], R = stride - nc;
+    const int     hl = PAIR ? ( lane & 31 ) : lane, hb = PAIR ? ( lane & 32 ) : 0;
+    const int32_t w0   = ( (int32_t)blockIdx.x * 4 + ( threadIdx.x >> 6 ) ) * ( PAIR ? 2 : 1 );
+    if ( w0 >= ncb ) return; /* whole waves only; no workgroup barrier below */
+    const int32_t idx = w0 + ( PAIR ? ( lane >> 5 ) : 0 );
+    const bool    act = idx < ncb;
+    const int32_t k = cblks[act ? idx : w0];
+    const int32_t nc = act ? S.c_ncol[k] : 0, stride = act ? S.c_stride[k] : 0, R = stride - nc;
     const int64_t coef = S.c_coef[k], rb0 = S.r_base[k];
     const T       tcrit = (T)*critp;
+    auto bcast = [&]( T v, int j ) -> T { return PAIR ? __shfl( v, hb + j, 64 ) : pa_readlane( v, j ); };
+    /* column loop bound: nc, or with two cblks the larger width (the
+     * columns past a cblk's own nc are scratch, cleared after the chain) */
+    const int32_t ncl = PAIR ? __builtin_amdgcn_readfirstlane( max( nc, __shfl_xor( nc, 32, 64 ) ) ) : nc;
 
     T       P[NS][PA_SM_NC];
     int64_t rbase[NS];
@@ -817,7 +829,7 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
     bool    valid[NS];
 #pragma unroll
     for ( int q = 0; q < NS; q++ ) {
-        const int32_t r = lane + 64 * q;
+        const int32_t r = hl + 64 * q;
         valid[q] = r < stride;
         rbase[q] = coef + r;
         rld[q]   = nc;
@@ -837,42 +849,49 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
     int32_t mypiv = 0, myneg = 0;
 #pragma unroll
     for ( int j = 0; j < PA_SM_NC; j++ ) {
-        if ( j < nc ) {
-            T d = pa_readlane( P[0][j], j );
-            const bool rep = pa_abs( d ) < tcrit;
+        if ( j < ncl ) {
+            const bool jv = j < nc; /* a column of this lane's cblk */
+            T d = bcast( P[0][j], j );
+            const bool rep = jv && pa_abs( d ) < tcrit;
             if ( rep ) d = tcrit;
             mypiv += rep ? 1 : 0;
-            myneg += ( d < T( 0 ) ) ? 1 : 0;
+            myneg += ( jv && d < T( 0 ) ) ? 1 : 0;
+            if ( !jv ) d = T( 1 ); /* scratch column: kept finite */
             d = sqrt( d );
             /* column scaled by the pivot's reciprocal (the reference kernel's
              * order); selects instead of per-lane branches */
             const T alpha = T( 1 ) / d;
 #pragma unroll
             for ( int q = 0; q < NS; q++ ) {
-                const int32_t r = lane + 64 * q;
+                const int32_t r = hl + 64 * q;
                 P[q][j] = ( r == j ) ? d : ( ( r > j ) ? P[q][j] * alpha : P[q][j] );
             }
 #pragma unroll
             for ( int c = 1; c < PA_SM_NC; c++ ) {
-                if ( c > j && c < nc ) {
-                    const T f = pa_readlane( P[0][j], c ); /* L[c][j] */
+                if ( c > j && c < ncl ) {
+                    const T f = bcast( P[0][j], c ); /* L[c][j] */
 #pragma unroll
                     for ( int q = 0; q < NS; q++ )
-                        P[q][c] = ( lane + 64 * q > j ) ? P[q][c] - P[q][j] * f : P[q][c];
+                        P[q][c] = ( hl + 64 * q > j ) ? P[q][c] - P[q][j] * f : P[q][c];
                 }
             }
         }
     }
+    if ( PAIR ) {
+#pragma unroll
+        for ( int c = 0; c < PA_SM_NC; c++ )
+            if ( c >= nc ) P[0][c] = T( 0 );
+    }
 #pragma unroll
     for ( int q = 0; q < NS; q++ ) {
-        const int32_t r = lane + 64 * q;
+        const int32_t r = hl + 64 * q;
         if ( valid[q] ) {
 #pragma unroll
             for ( int c = 0; c < PA_SM_NC; c++ )
                 if ( c < nc && ( r >= nc || c <= r ) ) Lc[rbase[q] + (int64_t)c * rld[q]] = P[q][c];
         }
     }
-    if ( lane == 0 ) {
+    if ( hl == 0 && act ) {
         if ( mypiv ) atomicAdd( npiv, mypiv );
         if ( myneg ) atomicAdd( nneg, myneg );
     }
@@ -883,10 +902,10 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
     int32_t bidx[NS];
 #pragma unroll
     for ( int q = 0; q < NS; q++ ) {
-        const int32_t r = lane + 64 * q;
+        const int32_t r = hl + 64 * q;
         bidx[q] = ( valid[q] && r >= nc ) ? S.r_blk[rb0 + ( r - nc )] : -1;
     }
-    const int32_t lblk = S.c_fblok[k + 1];
+    const int32_t lblk = act ? S.c_fblok[k + 1] : S.c_fblok[k] + 1;
     int32_t       gb   = S.c_fblok[k] + 1;
     while ( gb < lblk ) {
         const int32_t F = S.b_fcblk[gb];
@@ -931,7 +950,13 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
             /* row rbrow broadcast from its lane (uniform branch on its register slot) */
             T       xb[PA_SM_NC];
             int32_t growb;
-            if ( qb == 0 ) {
+            if ( PAIR ) {
+                /* per half: rbrow differs between the halves */
+#pragma unroll
+                for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = __shfl( P[0][c], hb + lb, 64 );
+                growb = __shfl( grow[0], hb + lb, 64 );
+            }
+            else if ( qb == 0 ) {
 #pragma unroll
                 for ( int c = 0; c < PA_SM_NC; c++ ) xb[c] = pa_readlane( P[0][c], lb );
                 growb = __builtin_amdgcn_readlane( grow[0], lb );
@@ -949,7 +974,7 @@ k_small_llt( pa_dstruct S, const int32_t *cblks, int32_t ncb, T *Lc, const doubl
             const int32_t col = growb - fcolF;
 #pragma unroll
             for ( int q = 0; q < NS; q++ ) {
-                const int32_t r = lane + 64 * q;
+                const int32_t r = hl + 64 * q;
                 if ( tok[q] && r >= rbrow ) {
                     T acc = T( 0 );
 #pragma unroll
